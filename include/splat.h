@@ -1,0 +1,267 @@
+/*
+ * splat.h -- C ABI of the MI355X (gfx950) splat training library `libsplat.so`.
+ *
+ * Drop-in boundary for the differentiable splat pipeline of harryprayiv/splatrs
+ * (a Rust crate).  Every entry point below names the reference function it replaces
+ * (reference file:line).  All entry points return an `int` status (SPLAT_OK = 0); on
+ * failure `splat_last_error(ctx)` holds a message.  Precondition failures (the cases in
+ * which the reference `assert!`s and panics) return SPLAT_ERR_PRECONDITION, so a Rust
+ * wrapper can map them back to `panic!`.
+ *
+ * Scalars are IEEE binary32 (the reference's default `Real = f32`, src/lib.rs:18-20).
+ *
+ * Two tiers:
+ *   - Host tier (`splat_project_all`, `splat_bin`, `splat_rasterize_forward`, ...):
+ *     the reference's functions with host arrays in and out, record layouts identical
+ *     to the reference structs.  They upload, compute on the GPU and download.
+ *   - Trainer (`splat_trainer_*`): model, gradients, Adam moments, statistics, views and
+ *     all per-view buffers stay resident in HBM; one call runs a whole training
+ *     iteration (src/train.rs:196-303) on the context's stream.
+ *
+ * Threading: one context owns one HIP stream.  Calls on one context must not overlap;
+ * distinct contexts are independent.
+ */
+#ifndef SPLAT_H
+#define SPLAT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- constants ------------------------------------------------------------------ */
+
+/* Scalars per Gaussian record: mean 3, log_scale 3, quat 4 (w,x,y,z), opacity logit 1,
+ * SH 16x3 (sh[k][c] at 11 + 3k + c).  Order of tests/common/mod.rs:111-139. */
+#define SPLAT_NUM_PARAMS 59
+/* Tile edge in pixels, src/project.rs:18. */
+#define SPLAT_TILE 16
+/* Gradient slots per tile entry, src/rasterize.rs:341. */
+#define SPLAT_SLOTS 11
+
+/* ---- status codes --------------------------------------------------------------- */
+enum {
+  SPLAT_OK = 0,
+  SPLAT_ERR_PRECONDITION = 1, /* the reference panics (assert!) on this input */
+  SPLAT_ERR_HIP = 2,          /* HIP runtime error */
+  SPLAT_ERR_OOM = 3,          /* device allocation failed */
+  SPLAT_ERR_CAPACITY = 4,     /* caller-provided output array too small; size reported */
+  SPLAT_ERR_INVALID = 5,      /* invalid argument (Error::Invalid in the reference) */
+  SPLAT_ERR_IO = 6,           /* file I/O (Error::Io) */
+  SPLAT_ERR_FORMAT = 7,       /* malformed file (Error::Format) */
+  SPLAT_ERR_NO_DEVICE = 8     /* no usable gfx950 device */
+};
+
+/* ---- plain records (field order of the reference structs) ----------------------- */
+
+/* Camera, src/camera.rs:13-32 (the `name` field is host-only and omitted).
+ * rot is row-major world-to-camera, x_cam = rot * x_world + trans. */
+typedef struct {
+  uint32_t width, height;
+  float fx, fy, cx, cy;
+  float rot[9];
+  float trans[3];
+} splat_camera;
+
+/* ProjOpts, src/project.rs:22-43 (defaults near 0.2, eps2d 0.3, antialias off, degree 3). */
+typedef struct {
+  float near_plane;
+  float eps2d;
+  int32_t antialias;
+  int32_t sh_degree;
+} splat_proj_opts;
+
+/* Proj, src/project.rs:47-64 (60 bytes).  radius == 0 means culled. */
+typedef struct {
+  float mean2d[2];
+  float conic[3];
+  float depth;
+  uint32_t radius;
+  float opacity;
+  float color[3];
+  uint32_t tile_min[2];
+  uint32_t tile_max[2];
+} splat_proj;
+
+/* ProjGrad, src/project.rs:84-93 (36 bytes). */
+typedef struct {
+  float mean2d[2];
+  float conic[3];
+  float opacity;
+  float color[3];
+} splat_proj_grad;
+
+/* RasterSettings, src/rasterize.rs:15-22. */
+typedef struct {
+  uint32_t width, height;
+  float background[3];
+} splat_raster_settings;
+
+/* Binning, src/rasterize.rs:26-39, as caller-owned arrays.  `sorted` is split into
+ * `sorted_keys` (tile << 32 | depth bits) and `sorted_entries`.  On input `capacity`
+ * is the length of entry_gaussian / sorted_keys / sorted_entries; if the binning needs
+ * more, the call returns SPLAT_ERR_CAPACITY with `num_entries` set and writes nothing
+ * else.  entry_offset has n+1 elements, tile_range 2 * tiles_x * tiles_y. */
+typedef struct {
+  uint32_t tiles_x, tiles_y;
+  uint64_t num_entries;
+  uint64_t capacity;
+  uint32_t* entry_gaussian;
+  uint32_t* entry_offset;
+  uint64_t* sorted_keys;
+  uint32_t* sorted_entries;
+  uint32_t* tile_range;
+} splat_binning;
+
+/* LearningRates, src/adam.rs:11-24. */
+typedef struct {
+  float mean, log_scale, quat, opacity, sh_dc, sh_rest;
+} splat_learning_rates;
+
+/* Adam hyper-parameters and step counter, src/adam.rs:28-41 (defaults 0.9, 0.999, 1e-15). */
+typedef struct {
+  float beta1, beta2, eps;
+  uint32_t pad_;
+  uint64_t step;
+} splat_adam_state;
+
+/* DensifyConfig, src/densify.rs:69-103.  max_screen_size is Option<u32>: used when
+ * has_max_screen_size != 0. */
+typedef struct {
+  float grad_threshold;
+  float percent_dense;
+  float min_opacity;
+  int32_t prune_big_world;
+  float max_world_size_frac;
+  int32_t has_max_screen_size;
+  uint32_t max_screen_size;
+  uint32_t split_children;
+} splat_densify_config;
+
+/* DensifyReport, src/densify.rs:107-114. */
+typedef struct {
+  uint64_t cloned, split, pruned;
+} splat_densify_report;
+
+/* ---- context --------------------------------------------------------------------- */
+
+typedef struct splat_ctx splat_ctx;
+
+/* Create a context on HIP device `device` with its own stream.  Fails with
+ * SPLAT_ERR_NO_DEVICE when the device is absent or not gfx950. */
+int splat_ctx_create(int device, splat_ctx** out);
+int splat_ctx_destroy(splat_ctx* ctx);
+/* Message of the last failure on this context ("" if none).  Valid until the next call. */
+const char* splat_last_error(const splat_ctx* ctx);
+/* Block until all work queued on the context's stream has finished. */
+int splat_sync(splat_ctx* ctx);
+/* The context's hipStream_t, as an opaque pointer. */
+void* splat_ctx_stream(splat_ctx* ctx);
+/* Per-kernel timing with HIP events on the context stream (off by default).  When on,
+ * every kernel launch is bracketed by events; splat_kernel_times reports, for each
+ * kernel name, the launch count and the summed duration in milliseconds since the last
+ * reset.  `names` receives up to `cap` NUL-separated names in one buffer of `names_len`
+ * bytes.  Returns the number of kernels in *count. */
+int splat_set_timing(splat_ctx* ctx, int enabled);
+int splat_reset_timing(splat_ctx* ctx);
+int splat_kernel_times(splat_ctx* ctx, char* names, size_t names_len, uint64_t* launches,
+                       double* total_ms, uint32_t cap, uint32_t* count);
+
+/* ---- host tier: the reference functions ----------------------------------------- */
+
+/* project_all, src/project.rs:320-330.  gaussians: n records of SPLAT_NUM_PARAMS
+ * floats.  opts->sh_degree is the model's active degree (the reference overrides the
+ * option with `model.sh_degree`). */
+int splat_project_all(splat_ctx* ctx, const float* gaussians, uint64_t n,
+                      const splat_camera* cam, const splat_proj_opts* opts,
+                      splat_proj* out);
+
+/* project_backward_all, src/project.rs:477-493.  out: n records of 59 floats. */
+int splat_project_backward_all(splat_ctx* ctx, const float* gaussians, uint64_t n,
+                               const splat_camera* cam, const splat_proj_opts* opts,
+                               const splat_proj_grad* grads, float* out);
+
+/* bin, src/rasterize.rs:60-128. */
+int splat_bin(splat_ctx* ctx, const splat_proj* projs, uint64_t n, uint32_t width,
+              uint32_t height, splat_binning* out);
+
+/* rasterize_forward, src/rasterize.rs:305-337.  image: width*height*3, final_t and
+ * n_contrib: width*height.  The binning arrays are those splat_bin produced. */
+int splat_rasterize_forward(splat_ctx* ctx, const splat_proj* projs, uint64_t n,
+                            const splat_binning* binning,
+                            const splat_raster_settings* settings, float* image,
+                            float* final_t, uint32_t* n_contrib);
+
+/* rasterize_backward, src/rasterize.rs:459-505.  grads: n ProjGrad, abs_grads: 2n.
+ * `ordered` != 0 sums each entry's per-pixel terms in the reference's row-major order
+ * (bitwise-identical sums); 0 uses a tree reduction (same values up to rounding). */
+int splat_rasterize_backward(splat_ctx* ctx, const splat_proj* projs, uint64_t n,
+                             const splat_binning* binning,
+                             const splat_raster_settings* settings, const float* final_t,
+                             const uint32_t* n_contrib, const float* dimage, int ordered,
+                             splat_proj_grad* grads, float* abs_grads);
+
+/* render, src/render.rs:14-23.  image: width*height*3. */
+int splat_render(splat_ctx* ctx, const float* gaussians, uint64_t n,
+                 const splat_camera* cam, const splat_proj_opts* opts,
+                 const float background[3], float* image);
+
+/* to_rgb8, src/render.rs:26-31 (host arithmetic). */
+int splat_to_rgb8(const float* image, uint64_t len, uint8_t* out);
+
+/* photometric_loss, src/loss.rs:137-171.  pred, target, grad: width*height*3. */
+int splat_photometric_loss(splat_ctx* ctx, const float* pred, const float* target,
+                           uint32_t width, uint32_t height, float lambda_dssim,
+                           float* loss, float* grad);
+
+/* ssim_with_grad, src/loss.rs:75-134. */
+int splat_ssim_with_grad(splat_ctx* ctx, const float* x, const float* y, uint32_t width,
+                         uint32_t height, float* ssim, float* grad);
+
+/* blur, src/loss.rs:31-72 (11-tap sigma 1.5 zero-padded separable blur). */
+int splat_blur(splat_ctx* ctx, const float* src, uint32_t width, uint32_t height,
+               uint32_t channels, float* dst);
+
+/* psnr, src/loss.rs:175-190 (host, f64 sum). */
+int splat_psnr(const float* pred, const float* target, uint64_t len, float* out);
+
+/* Adam::step, src/adam.rs:77-161.  params, m, v updated in place; n records of 59.
+ * state->step is incremented. */
+int splat_adam_step(splat_ctx* ctx, float* params, const float* grads, float* m, float* v,
+                    uint64_t n, splat_adam_state* state, const splat_learning_rates* lr);
+
+/* DensifyStats::accumulate, src/densify.rs:36-60.  grad_accum, denom, max_radii: n,
+ * updated in place.  abs_grads may be NULL (then ProjGrad.mean2d is used). */
+int splat_stats_accumulate(splat_ctx* ctx, const splat_proj* projs,
+                           const splat_proj_grad* grads, const float* abs_grads, uint64_t n,
+                           uint32_t width, uint32_t height, float* grad_accum,
+                           uint32_t* denom, uint32_t* max_radii);
+
+/* densify_and_prune, src/densify.rs:125-221.  Inputs: n records of params, m, v and the
+ * stats.  `normals` holds the standard normal draws the reference takes from its RNG for
+ * split children (src/densify.rs:116-121,174-179), in draw order: split parent (index
+ * order), child, axis x,y,z; its length must be at least 3 * split_children * (number of
+ * split parents) -- pass `normals_len` and the call fails with SPLAT_ERR_CAPACITY (and
+ * report->split set) when it is too short.  Outputs are written to the *_out arrays
+ * (capacity `out_capacity` records); *n_out receives the new count.  The statistics of
+ * the new set are zero (the reference resets them). */
+int splat_densify_and_prune(splat_ctx* ctx, const float* params, const float* m,
+                            const float* v, const float* grad_accum, const uint32_t* denom,
+                            const uint32_t* max_radii, uint64_t n,
+                            const splat_densify_config* cfg, float extent,
+                            const float* normals, uint64_t normals_len, float* params_out,
+                            float* m_out, float* v_out, uint64_t out_capacity,
+                            uint64_t* n_out, splat_densify_report* report);
+
+/* reset_opacity, src/densify.rs:224-230 with Adam::reset_opacity_state
+ * (src/adam.rs:176-183).  In place on n records. */
+int splat_reset_opacity(splat_ctx* ctx, float* params, float* m, float* v, uint64_t n,
+                        float max_alpha);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPLAT_H */

@@ -1,0 +1,356 @@
+"""ctypes binding of libsplat.so (include/splat.h) and a Python mirror of the reference's
+host API (src/project.rs, src/rasterize.rs, src/render.rs, src/loss.rs, src/adam.rs,
+src/densify.rs), for tests and tools.
+
+Every function here calls the HIP library; there is no CPU fallback.  Loading fails
+loudly when libsplat.so is missing or no gfx950 device is present.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import records as rec
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsplat.so")
+
+P = C.c_void_p
+u64, u32, i32, f32, f64 = C.c_uint64, C.c_uint32, C.c_int, C.c_float, C.c_double
+
+OK, ERR_PRECONDITION, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_INVALID, ERR_IO, ERR_FORMAT, \
+    ERR_NO_DEVICE = range(9)
+
+
+class SplatError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"libsplat status {status}: {msg}")
+        self.status = status
+
+
+class PreconditionError(SplatError):
+    """The reference panics (assert!) on this input."""
+
+
+class BinningStruct(C.Structure):
+    _fields_ = [("tiles_x", u32), ("tiles_y", u32), ("num_entries", u64), ("capacity", u64),
+                ("entry_gaussian", P), ("entry_offset", P), ("sorted_keys", P),
+                ("sorted_entries", P), ("tile_range", P)]
+
+
+class Binning:
+    """Binning, src/rasterize.rs:26-39, with host numpy arrays."""
+
+    def __init__(self, tiles_x, tiles_y, entry_gaussian, entry_offset, sorted_keys,
+                 sorted_entries, tile_range):
+        self.tiles_x, self.tiles_y = int(tiles_x), int(tiles_y)
+        self.entry_gaussian = entry_gaussian
+        self.entry_offset = entry_offset
+        self.sorted_keys = sorted_keys
+        self.sorted_entries = sorted_entries
+        self.tile_range = tile_range
+
+    @property
+    def num_entries(self):
+        return len(self.sorted_keys)
+
+    def as_struct(self):
+        return binning_struct(self)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def alloc_binning(n, tiles_x, tiles_y, capacity):
+    return Binning(tiles_x, tiles_y, np.zeros(capacity, np.uint32), np.zeros(n + 1, np.uint32),
+                   np.zeros(capacity, np.uint64), np.zeros(capacity, np.uint32),
+                   np.zeros((tiles_x * tiles_y, 2), np.uint32))
+
+
+def trim_binning(b, e):
+    return Binning(b.tiles_x, b.tiles_y, b.entry_gaussian[:e].copy(), b.entry_offset,
+                   b.sorted_keys[:e].copy(), b.sorted_entries[:e].copy(), b.tile_range)
+
+
+def binning_struct(b):
+    s = BinningStruct()
+    s.tiles_x, s.tiles_y = b.tiles_x, b.tiles_y
+    s.num_entries = len(b.sorted_keys)
+    s.capacity = len(b.sorted_keys)
+    s.entry_gaussian = _p(b.entry_gaussian)
+    s.entry_offset = _p(b.entry_offset)
+    s.sorted_keys = _p(b.sorted_keys)
+    s.sorted_entries = _p(b.sorted_entries)
+    s.tile_range = _p(b.tile_range)
+    return s
+
+
+# ---- library loading -----------------------------------------------------------------
+
+_SIGS = {
+    "splat_ctx_create": (i32, [i32, P]),
+    "splat_ctx_destroy": (i32, [P]),
+    "splat_last_error": (C.c_char_p, [P]),
+    "splat_sync": (i32, [P]),
+    "splat_ctx_stream": (P, [P]),
+    "splat_set_timing": (i32, [P, i32]),
+    "splat_reset_timing": (i32, [P]),
+    "splat_kernel_times": (i32, [P, P, C.c_size_t, P, P, u32, P]),
+    "splat_project_all": (i32, [P, P, u64, P, P, P]),
+    "splat_project_backward_all": (i32, [P, P, u64, P, P, P, P]),
+    "splat_bin": (i32, [P, P, u64, u32, u32, P]),
+    "splat_rasterize_forward": (i32, [P, P, u64, P, P, P, P, P]),
+    "splat_rasterize_backward": (i32, [P, P, u64, P, P, P, P, P, i32, P, P]),
+    "splat_render": (i32, [P, P, u64, P, P, P, P]),
+    "splat_to_rgb8": (i32, [P, u64, P]),
+    "splat_photometric_loss": (i32, [P, P, P, u32, u32, f32, P, P]),
+    "splat_ssim_with_grad": (i32, [P, P, P, u32, u32, P, P]),
+    "splat_blur": (i32, [P, P, u32, u32, u32, P]),
+    "splat_psnr": (i32, [P, P, u64, P]),
+    "splat_adam_step": (i32, [P, P, P, P, P, u64, P, P]),
+    "splat_stats_accumulate": (i32, [P, P, P, P, u64, u32, u32, P, P, P]),
+    "splat_densify_and_prune": (i32, [P, P, P, P, P, P, P, u64, P, f32, P, u64, P, P, P, u64,
+                                      P, P]),
+    "splat_reset_opacity": (i32, [P, P, P, P, u64, f32]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded libsplat.so.  Raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Function names declared in include/splat.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "splat.h")
+    text = open(hdr).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(splat_\w+)\s*\(", text, re.M)))
+
+
+class Context:
+    """A libsplat context (one HIP stream on one device)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        st = lib().splat_ctx_create(device, C.byref(self._h))
+        if st != OK:
+            raise SplatError(st, f"splat_ctx_create(device={device}) failed")
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().splat_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, status, what):
+        if status == OK:
+            return
+        msg = lib().splat_last_error(self._h)
+        msg = msg.decode() if msg else ""
+        cls = PreconditionError if status == ERR_PRECONDITION else SplatError
+        raise cls(status, f"{what}: {msg}")
+
+    def sync(self):
+        self.check(lib().splat_sync(self._h), "sync")
+
+    # ---- timing ------------------------------------------------------------------------
+    def set_timing(self, on):
+        self.check(lib().splat_set_timing(self._h, 1 if on else 0), "set_timing")
+
+    def reset_timing(self):
+        self.check(lib().splat_reset_timing(self._h), "reset_timing")
+
+    def kernel_times(self):
+        cap = 256
+        names = C.create_string_buffer(64 * cap)
+        launches = np.zeros(cap, np.uint64)
+        total = np.zeros(cap, np.float64)
+        count = C.c_uint32()
+        self.check(lib().splat_kernel_times(self._h, names, len(names), _p(launches), _p(total),
+                                            cap, C.byref(count)), "kernel_times")
+        raw = names.raw.split(b"\0")
+        out = {}
+        for i in range(count.value):
+            out[raw[i].decode()] = (int(launches[i]), float(total[i]))
+        return out
+
+    # ---- reference API -----------------------------------------------------------------
+    def project_all(self, gaussians, cam, opts):
+        """project_all, src/project.rs:320-330."""
+        g = np.ascontiguousarray(gaussians, np.float32)
+        out = np.zeros(len(g), rec.PROJ)
+        self.check(lib().splat_project_all(self._h, _p(g), len(g), _p(cam), _p(opts), _p(out)),
+                   "project_all")
+        return out
+
+    def project_backward_all(self, gaussians, cam, opts, grads):
+        """project_backward_all, src/project.rs:477-493."""
+        g = np.ascontiguousarray(gaussians, np.float32)
+        out = np.zeros((len(g), rec.NUM_PARAMS), np.float32)
+        self.check(lib().splat_project_backward_all(self._h, _p(g), len(g), _p(cam), _p(opts),
+                                                    _p(grads), _p(out)), "project_backward_all")
+        return out
+
+    def bin(self, projs, width, height):
+        """bin, src/rasterize.rs:60-128."""
+        n = len(projs)
+        tx, ty = (width + 15) // 16, (height + 15) // 16
+        cap = max(1024, 4 * n)
+        while True:
+            b = alloc_binning(n, tx, ty, cap)
+            s = b.as_struct()
+            st = lib().splat_bin(self._h, _p(projs), n, width, height, C.byref(s))
+            if st == ERR_CAPACITY:
+                cap = int(s.num_entries)
+                continue
+            self.check(st, "bin")
+            return trim_binning(b, int(s.num_entries))
+
+    def rasterize_forward(self, projs, binning, settings):
+        """rasterize_forward, src/rasterize.rs:305-337 -> (image, final_t, n_contrib)."""
+        w, h = int(settings["width"]), int(settings["height"])
+        image = np.zeros((h, w, 3), np.float32)
+        final_t = np.zeros((h, w), np.float32)
+        n_contrib = np.zeros((h, w), np.uint32)
+        s = binning.as_struct()
+        self.check(lib().splat_rasterize_forward(self._h, _p(projs), len(projs), C.byref(s),
+                                                 _p(settings), _p(image), _p(final_t),
+                                                 _p(n_contrib)), "rasterize_forward")
+        return image, final_t, n_contrib
+
+    def rasterize_backward(self, projs, binning, settings, final_t, n_contrib, dimage,
+                           ordered=True):
+        """rasterize_backward, src/rasterize.rs:459-505 -> (grads, abs_grads)."""
+        n = len(projs)
+        grads = np.zeros(n, rec.PROJ_GRAD)
+        absg = np.zeros((n, 2), np.float32)
+        s = binning.as_struct()
+        dimage = np.ascontiguousarray(dimage, np.float32)
+        self.check(lib().splat_rasterize_backward(self._h, _p(projs), n, C.byref(s),
+                                                  _p(settings), _p(final_t), _p(n_contrib),
+                                                  _p(dimage), 1 if ordered else 0, _p(grads),
+                                                  _p(absg)), "rasterize_backward")
+        return grads, absg
+
+    def render(self, gaussians, cam, opts, background=(0.0, 0.0, 0.0)):
+        """render, src/render.rs:14-23."""
+        g = np.ascontiguousarray(gaussians, np.float32)
+        bg = np.asarray(background, np.float32)
+        img = np.zeros((int(cam["height"]), int(cam["width"]), 3), np.float32)
+        self.check(lib().splat_render(self._h, _p(g), len(g), _p(cam), _p(opts), _p(bg),
+                                      _p(img)), "render")
+        return img
+
+    def photometric_loss(self, pred, target, width, height, lambda_dssim):
+        """photometric_loss, src/loss.rs:137-171 -> (loss, grad)."""
+        pred = np.ascontiguousarray(pred, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        grad = np.zeros_like(pred)
+        loss = np.zeros(1, np.float32)
+        self.check(lib().splat_photometric_loss(self._h, _p(pred), _p(target), width, height,
+                                                lambda_dssim, _p(loss), _p(grad)),
+                   "photometric_loss")
+        return float(loss[0]), grad
+
+    def ssim_with_grad(self, x, y, width, height):
+        """ssim_with_grad, src/loss.rs:75-134 -> (ssim, grad)."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        grad = np.zeros_like(x)
+        s = np.zeros(1, np.float32)
+        self.check(lib().splat_ssim_with_grad(self._h, _p(x), _p(y), width, height, _p(s),
+                                              _p(grad)), "ssim_with_grad")
+        return float(s[0]), grad
+
+    def blur(self, src, width, height, channels):
+        """blur, src/loss.rs:31-72."""
+        src = np.ascontiguousarray(src, np.float32)
+        dst = np.zeros_like(src)
+        self.check(lib().splat_blur(self._h, _p(src), width, height, channels, _p(dst)), "blur")
+        return dst
+
+    def adam_step(self, params, grads, m, v, state, lr):
+        """Adam::step, src/adam.rs:77-161, in place on params, m, v, state."""
+        grads = np.ascontiguousarray(grads, np.float32)
+        self.check(lib().splat_adam_step(self._h, _p(params), _p(grads), _p(m), _p(v),
+                                         len(params), _p(state), _p(lr)), "adam_step")
+
+    def stats_accumulate(self, projs, grads, abs_grads, width, height, grad_accum, denom,
+                         max_radii):
+        """DensifyStats::accumulate, src/densify.rs:36-60, in place."""
+        self.check(lib().splat_stats_accumulate(self._h, _p(projs), _p(grads), _p(abs_grads),
+                                                len(projs), width, height, _p(grad_accum),
+                                                _p(denom), _p(max_radii)), "stats_accumulate")
+
+    def densify_and_prune(self, params, m, v, grad_accum, denom, max_radii, cfg, extent,
+                          normals):
+        """densify_and_prune, src/densify.rs:125-221 -> (params, m, v, report)."""
+        n = len(params)
+        normals = np.ascontiguousarray(normals, np.float32)
+        cap = 3 * n + 16
+        po = np.zeros((cap, rec.NUM_PARAMS), np.float32)
+        mo = np.zeros_like(po)
+        vo = np.zeros_like(po)
+        n_out = np.zeros(1, np.uint64)
+        report = np.zeros((), rec.DENSIFY_REPORT)
+        self.check(lib().splat_densify_and_prune(
+            self._h, _p(np.ascontiguousarray(params, np.float32)),
+            _p(np.ascontiguousarray(m, np.float32)), _p(np.ascontiguousarray(v, np.float32)),
+            _p(np.ascontiguousarray(grad_accum, np.float32)),
+            _p(np.ascontiguousarray(denom, np.uint32)),
+            _p(np.ascontiguousarray(max_radii, np.uint32)), n, _p(cfg), extent, _p(normals),
+            normals.size, _p(po), _p(mo), _p(vo), cap, _p(n_out), _p(report)),
+            "densify_and_prune")
+        k = int(n_out[0])
+        return po[:k].copy(), mo[:k].copy(), vo[:k].copy(), report
+
+    def reset_opacity(self, params, m, v, max_alpha):
+        """reset_opacity, src/densify.rs:224-230 (+ Adam::reset_opacity_state)."""
+        self.check(lib().splat_reset_opacity(self._h, _p(params), _p(m), _p(v), len(params),
+                                             max_alpha), "reset_opacity")
+
+
+def to_rgb8(image):
+    """to_rgb8, src/render.rs:26-31."""
+    image = np.ascontiguousarray(image, np.float32)
+    out = np.zeros(image.shape, np.uint8)
+    st = lib().splat_to_rgb8(_p(image), image.size, _p(out))
+    if st != OK:
+        raise SplatError(st, "to_rgb8")
+    return out
+
+
+def psnr(pred, target):
+    """psnr, src/loss.rs:175-190."""
+    pred = np.ascontiguousarray(pred, np.float32)
+    target = np.ascontiguousarray(target, np.float32)
+    out = np.zeros(1, np.float32)
+    st = lib().splat_psnr(_p(pred), _p(target), pred.size, _p(out))
+    if st != OK:
+        raise SplatError(st, "psnr")
+    return float(out[0])

@@ -1,0 +1,97 @@
+"""numpy layouts of the plain records in include/splat.h.
+
+The records keep the field order of the reference's structs (src/project.rs:47-93,
+src/camera.rs:13-32), so a numpy structured array of these dtypes can be handed to the
+C ABI as a pointer.
+"""
+import numpy as np
+
+NUM_PARAMS = 59  # tests/common/mod.rs:109
+TILE = 16        # src/project.rs:18
+SLOTS = 11       # src/rasterize.rs:341
+
+CAMERA = np.dtype([
+    ("width", "<u4"), ("height", "<u4"),
+    ("fx", "<f4"), ("fy", "<f4"), ("cx", "<f4"), ("cy", "<f4"),
+    ("rot", "<f4", (9,)), ("trans", "<f4", (3,)),
+])
+PROJ_OPTS = np.dtype([("near_plane", "<f4"), ("eps2d", "<f4"), ("antialias", "<i4"),
+                      ("sh_degree", "<i4")])
+PROJ = np.dtype([
+    ("mean2d", "<f4", (2,)), ("conic", "<f4", (3,)), ("depth", "<f4"), ("radius", "<u4"),
+    ("opacity", "<f4"), ("color", "<f4", (3,)), ("tile_min", "<u4", (2,)),
+    ("tile_max", "<u4", (2,)),
+])
+PROJ_GRAD = np.dtype([("mean2d", "<f4", (2,)), ("conic", "<f4", (3,)), ("opacity", "<f4"),
+                      ("color", "<f4", (3,))])
+RASTER_SETTINGS = np.dtype([("width", "<u4"), ("height", "<u4"), ("background", "<f4", (3,))])
+LEARNING_RATES = np.dtype([(k, "<f4") for k in
+                           ("mean", "log_scale", "quat", "opacity", "sh_dc", "sh_rest")])
+ADAM_STATE = np.dtype([("beta1", "<f4"), ("beta2", "<f4"), ("eps", "<f4"), ("pad_", "<u4"),
+                       ("step", "<u8")])
+DENSIFY_CONFIG = np.dtype([
+    ("grad_threshold", "<f4"), ("percent_dense", "<f4"), ("min_opacity", "<f4"),
+    ("prune_big_world", "<i4"), ("max_world_size_frac", "<f4"),
+    ("has_max_screen_size", "<i4"), ("max_screen_size", "<u4"), ("split_children", "<u4"),
+])
+DENSIFY_REPORT = np.dtype([("cloned", "<u8"), ("split", "<u8"), ("pruned", "<u8")])
+
+assert CAMERA.itemsize == 72 and PROJ.itemsize == 60 and PROJ_GRAD.itemsize == 36
+assert ADAM_STATE.itemsize == 24 and DENSIFY_CONFIG.itemsize == 32
+
+
+def proj_opts(sh_degree=3, antialias=False, near_plane=0.2, eps2d=0.3):
+    """ProjOpts (src/project.rs:34-43 defaults)."""
+    o = np.zeros((), PROJ_OPTS)
+    o["near_plane"] = near_plane
+    o["eps2d"] = eps2d
+    o["antialias"] = int(bool(antialias))
+    o["sh_degree"] = sh_degree
+    return o
+
+
+def raster_settings(width, height, background=(0.0, 0.0, 0.0)):
+    s = np.zeros((), RASTER_SETTINGS)
+    s["width"] = width
+    s["height"] = height
+    s["background"] = np.asarray(background, np.float32)
+    return s
+
+
+def learning_rates(mean, log_scale=0.005, quat=0.001, opacity=0.025, sh_dc=0.0025,
+                   sh_rest=None):
+    """LearningRates with the reference's TrainConfig defaults (src/train.rs:84-90)."""
+    lr = np.zeros((), LEARNING_RATES)
+    lr["mean"] = mean
+    lr["log_scale"] = log_scale
+    lr["quat"] = quat
+    lr["opacity"] = opacity
+    lr["sh_dc"] = sh_dc
+    lr["sh_rest"] = np.float32(sh_dc) / np.float32(20.0) if sh_rest is None else sh_rest
+    return lr
+
+
+def adam_state(step=0, beta1=0.9, beta2=0.999, eps=1e-15):
+    """Adam state (src/adam.rs:65-74 defaults)."""
+    s = np.zeros((), ADAM_STATE)
+    s["beta1"] = beta1
+    s["beta2"] = beta2
+    s["eps"] = eps
+    s["step"] = step
+    return s
+
+
+def densify_config(grad_threshold=0.0002, percent_dense=0.01, min_opacity=0.005,
+                   prune_big_world=False, max_world_size_frac=0.1, max_screen_size=None,
+                   split_children=2):
+    """DensifyConfig (src/densify.rs:91-103 defaults)."""
+    c = np.zeros((), DENSIFY_CONFIG)
+    c["grad_threshold"] = grad_threshold
+    c["percent_dense"] = percent_dense
+    c["min_opacity"] = min_opacity
+    c["prune_big_world"] = int(bool(prune_big_world))
+    c["max_world_size_frac"] = max_world_size_frac
+    c["has_max_screen_size"] = 0 if max_screen_size is None else 1
+    c["max_screen_size"] = 0 if max_screen_size is None else max_screen_size
+    c["split_children"] = split_children
+    return c

@@ -1,0 +1,120 @@
+"""Shared test helpers: the reference test fixtures restated with numpy's generator
+(tests/common/mod.rs, tests/common/scene.rs).  Seeds name numpy PCG64 streams; the
+reference's ChaCha8 draws are not reproducible here."""
+import math
+
+import numpy as np
+
+from splatrs_amd import records as rec
+from splatrs_amd import scenes
+
+# Finite-difference settings of tests/common/mod.rs:16-34.
+F64 = dict(H=1e-6, RTOL=1e-5, ATOL=1e-8)
+F32 = dict(H=1e-2, RTOL=5e-2, ATOL=1e-3)
+
+
+def check(what, analytic, numeric, scale, tol):
+    """tests/common/mod.rs:37-43: |a - n| <= ATOL * scale + RTOL * max(|a|, |n|)."""
+    t = tol["ATOL"] * scale + tol["RTOL"] * max(abs(analytic), abs(numeric))
+    assert abs(analytic - numeric) <= t, \
+        f"{what}: analytic {analytic:.9e} vs finite difference {numeric:.9e} (tol {t:.3e})"
+
+
+def central(x, h, f):
+    """tests/common/mod.rs:46-48."""
+    return (f(x + h) - f(x - h)) / (2.0 * h)
+
+
+def rnd(rng, lo, hi):
+    return lo + (hi - lo) * rng.random()
+
+
+def random_gaussian(rng):
+    """tests/common/mod.rs:61-92: a Gaussian near the origin with positive colours."""
+    g = np.zeros(59, np.float64)
+    for k in range(16):
+        for c in range(3):
+            g[11 + 3 * k + c] = rnd(rng, 0.2, 1.5) if k == 0 else rnd(rng, -0.15, 0.15)
+    g[0:3] = [rnd(rng, -0.6, 0.6), rnd(rng, -0.4, 0.4), rnd(rng, -0.5, 0.5)]
+    g[3:6] = [rnd(rng, -2.5, -1.2) for _ in range(3)]
+    g[6:10] = [rnd(rng, 0.5, 1.5), rnd(rng, -0.7, 0.7), rnd(rng, -0.7, 0.7), rnd(rng, -0.7, 0.7)]
+    g[10] = rnd(rng, -1.0, 2.0)
+    return g
+
+
+def ref_test_camera(width, height):
+    """tests/common/mod.rs:95-106."""
+    return scenes.look_at(width, height, width * 0.9, width * 0.9, (0.3, -0.2, -3.0),
+                          (0.0, 0.0, 0.0))
+
+
+def ring_cameras(n, width, height, radius):
+    """tests/common/scene.rs:32-53."""
+    cams = []
+    for i in range(n):
+        a = np.float32(np.float32(i) / np.float32(n) * np.float32(2 * math.pi))
+        eye = (radius * math.cos(a), -0.6 + 0.3 * math.sin(3.0 * a), radius * math.sin(a))
+        cams.append(scenes.look_at(width, height, width * 1.1, width * 1.1, eye,
+                                   (0.0, 0.0, 0.0)))
+    return cams
+
+
+def ground_truth(rng, n):
+    """tests/common/scene.rs:56-94: random blobs, SH degree 1."""
+    sh_c0 = 0.28209479177387814
+    out = np.zeros((n, 59), np.float32)
+    for i in range(n):
+        g = np.zeros(59)
+        for c in range(3):
+            g[11 + c] = (rnd(rng, 0.1, 0.9) - 0.5) / sh_c0
+        for k in range(1, 4):
+            for c in range(3):
+                g[11 + 3 * k + c] = rnd(rng, -0.2, 0.2)
+        g[0:3] = [rnd(rng, -0.7, 0.7), rnd(rng, -0.5, 0.5), rnd(rng, -0.7, 0.7)]
+        g[3:6] = [rnd(rng, -2.6, -1.6) for _ in range(3)]
+        g[6:10] = [rnd(rng, 0.5, 1.0), rnd(rng, -0.5, 0.5), rnd(rng, -0.5, 0.5),
+                   rnd(rng, -0.5, 0.5)]
+        g[10] = rnd(rng, 0.5, 3.0)
+        out[i] = g
+    return out
+
+
+def naive_render(projs, width, height, bg):
+    """tests/pipeline.rs:134-186: per-pixel compositor over every visible Gaussian whose
+    tile rectangle covers the pixel's tile, in (depth bits, index) order."""
+    vis = [i for i in range(len(projs)) if projs[i]["radius"] > 0]
+    depth_bits = projs["depth"].astype(np.float32).view(np.uint32)
+    order = sorted(vis, key=lambda i: (int(depth_bits[i]), i))
+    f = np.float32
+    image = np.zeros((height, width, 3), np.float32)
+    P = [(projs[i]["tile_min"], projs[i]["tile_max"], projs[i]["mean2d"].astype(np.float32),
+          projs[i]["conic"].astype(np.float32), f(projs[i]["opacity"]),
+          projs[i]["color"].astype(np.float32)) for i in order]
+    for y in range(height):
+        for x in range(width):
+            tx, ty = x // 16, y // 16
+            px, py = f(x) + f(0.5), f(y) + f(0.5)
+            t = f(1.0)
+            c = [f(0.0)] * 3
+            for tmin, tmax, m, cn, o, col in P:
+                if tx < tmin[0] or tx >= tmax[0] or ty < tmin[1] or ty >= tmax[1]:
+                    continue
+                dx = f(px - m[0])
+                dy = f(py - m[1])
+                sigma = f(f(f(0.5) * f(f(f(cn[0] * dx) * dx) + f(f(cn[2] * dy) * dy))) +
+                          f(f(cn[1] * dx) * dy))
+                if sigma < 0:
+                    continue
+                og = f(o * np.exp(f(-sigma), dtype=np.float32))
+                if og < f(f(1.0) / f(255.0)):
+                    continue
+                alpha = min(og, f(0.99))
+                test_t = f(t * f(f(1.0) - alpha))
+                if test_t < f(1e-4):
+                    break
+                for ch in range(3):
+                    c[ch] = f(c[ch] + f(f(alpha * t) * col[ch]))
+                t = test_t
+            for ch in range(3):
+                image[y, x, ch] = f(c[ch] + f(t * f(bg[ch])))
+    return image

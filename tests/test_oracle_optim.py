@@ -1,0 +1,214 @@
+"""Pins of the oracle's optimiser and density control (src/adam.rs, src/densify.rs,
+src/model.rs).
+
+- Adam: textbook Adam (Kingma & Ba) in f64 and torch.optim.Adam (different op order,
+  so within tolerance); powi against exact small powers and float64 pow;
+- stats: numpy restatement of the NDC norm accumulation (src/densify.rs:36-60);
+- densify: the reference's bookkeeping KAT (src/densify.rs:247-288) and a brute-force
+  mask/order check on random inputs;
+- KNN init distances: brute force over all points (src/model.rs:192-246).
+"""
+import numpy as np
+
+import oracle_lib as O
+from splatrs_amd import records as rec
+
+
+def test_powi_matches_exact_powers():
+    for b in [0.9, 0.999, 1.5, 0.5]:
+        for e in [1, 2, 3, 7, 100, 1000]:
+            want = np.float32(b) ** np.float64(e)
+            if want > 1e38:
+                continue
+            got = O.lib().oracle_powi_f32(b, e)
+            assert abs(got - want) <= 1e-5 * abs(want) + 1e-30
+    assert O.lib().oracle_powi_f32(0.5, 3) == 0.125
+
+
+def test_adam_matches_textbook_and_torch():
+    import torch
+    rng = np.random.default_rng(3)
+    n = 50
+    p = rng.standard_normal((n, 59)).astype(np.float32)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    st = rec.adam_state()
+    lr = rec.learning_rates(1.6e-4 * 3.0)
+    rates = np.array([lr["mean"]] * 3 + [lr["log_scale"]] * 3 + [lr["quat"]] * 4 +
+                     [lr["opacity"]] + [lr["sh_dc"]] * 3 + [lr["sh_rest"]] * 45, np.float64)
+    p64 = p.astype(np.float64)
+    m64 = np.zeros_like(p64)
+    v64 = np.zeros_like(p64)
+    tp = torch.tensor(p, requires_grad=True)
+    groups = []
+    for lo, hi in [(0, 3), (3, 6), (6, 10), (10, 11), (11, 14), (14, 59)]:
+        groups.append({"params": [], "lr": float(rates[lo]), "sl": (lo, hi)})
+    tparams = [torch.tensor(p[:, lo:hi], requires_grad=True) for lo, hi in
+               [g["sl"] for g in groups]]
+    opt = torch.optim.Adam([{"params": [t], "lr": g["lr"]} for t, g in zip(tparams, groups)],
+                           betas=(0.9, 0.999), eps=1e-15)
+    for step in range(1, 21):
+        g = rng.standard_normal((n, 59)).astype(np.float32)
+        if step % 5 == 0:
+            g[::3] = 0.0
+        O.adam_step(p, g, m, v, st, lr)
+        m64 = 0.9 * m64 + 0.1 * g
+        v64 = 0.999 * v64 + 0.001 * g.astype(np.float64) ** 2
+        p64 -= rates * (m64 / (1 - 0.9 ** step)) / (np.sqrt(v64 / (1 - 0.999 ** step)) + 1e-15)
+        for t, gr in zip(tparams, groups):
+            lo, hi = gr["sl"]
+            t.grad = torch.tensor(g[:, lo:hi])
+        opt.step()
+    assert int(st["step"]) == 20
+    assert np.allclose(p, p64, rtol=1e-5, atol=5e-6)
+    tp = np.concatenate([t.detach().numpy() for t in tparams], axis=1)
+    assert np.allclose(p, tp, rtol=1e-5, atol=5e-6)
+
+
+def test_stats_accumulate_matches_numpy():
+    rng = np.random.default_rng(4)
+    n = 200
+    projs = np.zeros(n, rec.PROJ)
+    projs["radius"] = rng.integers(0, 5, n)
+    grads = np.zeros(n, rec.PROJ_GRAD)
+    grads["mean2d"] = rng.standard_normal((n, 2)).astype(np.float32) * 1e-4
+    absg = np.abs(rng.standard_normal((n, 2))).astype(np.float32) * 1e-4
+    for use_abs in [False, True]:
+        acc = np.zeros(n, np.float32)
+        den = np.zeros(n, np.uint32)
+        mr = np.zeros(n, np.uint32)
+        for _ in range(3):
+            O.stats_accumulate(projs, grads, absg if use_abs else None, 320, 240, acc, den, mr)
+        g = (absg if use_abs else grads["mean2d"]).astype(np.float64) * [160.0, 120.0]
+        vis = projs["radius"] > 0
+        want = 3 * np.sqrt((g ** 2).sum(1)) * vis
+        assert np.allclose(acc, want, rtol=1e-6)
+        assert np.array_equal(den, 3 * vis)
+        assert np.array_equal(mr, projs["radius"])
+
+
+def _gaussian(mean, log_scale, opacity):
+    g = np.zeros(59, np.float32)
+    g[0:3] = mean
+    g[3:6] = log_scale
+    g[6] = 1.0
+    g[10] = opacity
+    return g
+
+
+def test_densify_bookkeeping_kat():
+    """src/densify.rs:247-288."""
+    extent = 10.0
+    params = np.array([_gaussian(0.0, np.log(np.float32(0.001)), 2.0),
+                       _gaussian(1.0, np.log(np.float32(0.5)), 2.0),
+                       _gaussian(2.0, np.log(np.float32(0.001)), -10.0),
+                       _gaussian(3.0, np.log(np.float32(0.001)), 2.0)])
+    m = np.ones_like(params)
+    v = np.ones_like(params)
+    acc = np.array([1.0, 1.0, 0.0, 0.0], np.float32)
+    den = np.array([1, 1, 1, 1], np.uint32)
+    mr = np.zeros(4, np.uint32)
+    normals = np.array([0.3, -1.2, 0.7, 1.5, 0.1, -0.4], np.float32)
+    po, mo, vo, rep = O.densify_and_prune(params, m, v, acc, den, mr, rec.densify_config(),
+                                          extent, normals)
+    assert (int(rep["cloned"]), int(rep["split"]), int(rep["pruned"])) == (1, 1, 1)
+    assert len(po) == 5
+    assert np.all(po[0, 0:3] == 0.0) and np.all(po[1, 0:3] == 3.0) and np.all(po[2, 0:3] == 0.0)
+    expect = np.log(np.float32(0.5)) - np.log(np.float32(1.6))
+    for child in po[3:5]:
+        assert np.all(np.abs(child[3:6] - expect) < 1e-5)
+    # child means: parent mean + R(q) (normals * scales), identity rotation here
+    s = np.float32(0.5)
+    assert np.allclose(po[3, 0:3], 1.0 + normals[0:3] * s, atol=1e-6)
+    assert np.allclose(po[4, 0:3], 1.0 + normals[3:6] * s, atol=1e-6)
+    # moments survive for kept originals, zero for new rows
+    assert np.all(mo[0:2] == 1.0) and np.all(mo[2:] == 0.0) and np.all(vo[2:] == 0.0)
+
+
+def test_densify_masks_brute_force():
+    rng = np.random.default_rng(9)
+    n = 300
+    params = np.zeros((n, 59), np.float32)
+    params[:, 0:3] = rng.uniform(-1, 1, (n, 3))
+    params[:, 3:6] = rng.uniform(-6, -1, (n, 3))
+    q = rng.standard_normal((n, 4))
+    params[:, 6:10] = q
+    params[:, 10] = rng.uniform(-7, 3, n)
+    m = rng.standard_normal((n, 59)).astype(np.float32)
+    v = rng.uniform(0, 1, (n, 59)).astype(np.float32)
+    acc = rng.uniform(0, 1e-3, n).astype(np.float32)
+    den = rng.integers(0, 4, n).astype(np.uint32)
+    mr = rng.integers(0, 30, n).astype(np.uint32)
+    extent = np.float32(4.0)
+    for cfg in [rec.densify_config(), rec.densify_config(prune_big_world=True),
+                rec.densify_config(max_screen_size=20)]:
+        normals = rng.standard_normal(3 * 2 * n).astype(np.float32)
+        po, mo, vo, rep = O.densify_and_prune(params, m, v, acc, den, mr, cfg, extent, normals)
+        grads = np.where(den > 0, acc / np.maximum(den, 1).astype(np.float32), 0).astype(np.float32)
+        smax = np.exp(params[:, 3:6]).max(1)
+        lim = np.float32(cfg["percent_dense"]) * extent
+        thr = np.float32(cfg["grad_threshold"])
+        clone = (grads >= thr) & (smax <= lim)
+        split = (grads >= thr) & (smax > lim)
+        alpha = 1 / (1 + np.exp(-params[:, 10].astype(np.float64)))
+
+        def pruned(rows, orig):
+            a = 1 / (1 + np.exp(-rows[:, 10].astype(np.float64)))
+            pr = a < 0.005
+            if cfg["prune_big_world"]:
+                pr |= np.exp(rows[:, 3:6].astype(np.float64)).max(1) > 0.1 * float(extent)
+            if cfg["has_max_screen_size"] and orig is not None:
+                pr |= mr[orig] > cfg["max_screen_size"]
+            return pr
+        keep_orig = ~split & ~pruned(params, np.arange(n))
+        keep_clone = ~pruned(params[clone], None)
+        n_child = 2 * int(split.sum())
+        assert int(rep["cloned"]) == clone.sum() and int(rep["split"]) == split.sum()
+        n_keep = keep_orig.sum() + keep_clone.sum()
+        assert len(po) >= n_keep and len(po) <= n_keep + n_child
+        assert np.array_equal(po[:keep_orig.sum()], params[keep_orig])
+        assert np.array_equal(mo[:keep_orig.sum()], m[keep_orig])
+        assert np.array_equal(po[keep_orig.sum():n_keep], params[clone][keep_clone])
+        assert np.all(mo[keep_orig.sum():] == 0)
+        del alpha
+
+
+def test_reset_opacity():
+    p = np.zeros((3, 59), np.float32)
+    p[:, 10] = [-10.0, 0.0, 3.0]
+    m = np.ones_like(p)
+    v = np.ones_like(p)
+    O.reset_opacity(p, m, v, 0.01)
+    lim = np.float32(np.log(np.float32(0.01) / (np.float32(1) - np.float32(0.01))))
+    assert p[0, 10] == -10.0 and abs(p[1, 10] - lim) < 1e-6 and abs(p[2, 10] - lim) < 1e-6
+    assert np.all(m[:, 10] == 0) and np.all(v[:, 10] == 0) and np.all(m[:, 11] == 1)
+
+
+def test_nn_distances_match_brute_force():
+    """src/model.rs:192-226 with an LCG point set."""
+    s = 12345
+    pts = []
+    for _ in range(200):
+        row = []
+        for _ in range(3):
+            s = (s * 6364136223846793005 + 1442695040888963407) % (1 << 64)
+            row.append(np.float32((s >> 33) / float(1 << 31)))
+        pts.append(row)
+    pts = np.array(pts, np.float64)
+    got = O.mean_nn_dist2(pts)
+    for i in range(len(pts)):
+        d = np.sort(((pts - pts[i]) ** 2).sum(1)[np.arange(len(pts)) != i])
+        want = d[:3].mean()
+        assert abs(got[i] - want) < 1e-5 * max(want, 1e-6)
+
+
+def test_nn_distances_handle_coincident_points():
+    """src/model.rs:228-246."""
+    pts = np.zeros((42, 3))
+    pts[40] = [1, 0, 0]
+    pts[41] = [0, 2, 0]
+    got = O.mean_nn_dist2(pts)
+    assert np.all(got[:40] == 0)
+    assert abs(got[40] - 1.0) < 1e-12 and abs(got[41] - 4.0) < 1e-12
+    two = O.mean_nn_dist2(np.array([[0, 0, 0], [0, 0, 3.0]]))
+    assert abs(two[0] - 9) < 1e-12 and abs(two[1] - 9) < 1e-12
