@@ -1,0 +1,271 @@
+// Internal runtime of libsplat: context, errors, timed kernel launches, device buffers and
+// the HBM layouts shared by the kernel files.
+//
+// HBM layouts (one GPU, FP32):
+//   params / grads / Adam moments: "planes" -- parameter j of Gaussian i at [j * S + i],
+//     j in the reference's flat order (0-2 mean, 3-5 log_scale, 6-9 quat, 10 opacity,
+//     11 + 3k + c SH), S = plane stride (>= N, multiple of 64).  Every kernel then reads
+//     and writes whole 256-byte wave rows.
+//   projection: three float4 records per Gaussian for the rasterizer ({mx, my, A, B},
+//     {C, opacity, sigma_max, half_x}, {half_y, r, g, b}), depth, radius, tile rectangle
+//     (uint2 {x0 | y0 << 16, x1 | y1 << 16}) and the tile-entry count.
+//   binning: entry arrays (tile, slot, gaussian), per-tile counts/starts, and the sorted
+//     per-tile lists (entry id, gaussian id) that the compositing kernels read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/splat.h"
+#include "fmath.hpp"
+
+namespace splat {
+
+constexpr int kTile = 16;
+constexpr int kParams = 59;
+constexpr int kSlots = 11;
+constexpr float kAlphaMin = 1.0f / 255.0f;  // src/rasterize.rs:53
+constexpr float kAlphaMax = 0.99f;          // src/rasterize.rs:55
+constexpr float kTMin = 1e-4f;              // src/rasterize.rs:57
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+#define SPLAT_HIP(call)                                                                  \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      ::splat::fail(e_ == hipErrorOutOfMemory ? SPLAT_ERR_OOM : SPLAT_ERR_HIP,           \
+                    std::string(#call) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+#define SPLAT_REQUIRE(cond, msg)                                                         \
+  do {                                                                                   \
+    if (!(cond)) ::splat::fail(SPLAT_ERR_PRECONDITION, msg);                             \
+  } while (0)
+
+// Device allocation that grows on demand (contents are not preserved on growth).
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (ptr) (void)hipFree(ptr);
+  }
+  void* reserve(size_t b) {
+    if (b > bytes) {
+      if (ptr) SPLAT_HIP(hipFree(ptr));
+      ptr = nullptr;
+      bytes = 0;
+      size_t want = b < 256 ? 256 : b;
+      SPLAT_HIP(hipMalloc(&ptr, want));
+      bytes = want;
+    }
+    return ptr;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr);
+  }
+};
+
+struct KernelStat {
+  uint64_t launches = 0;
+  double ms = 0;
+};
+
+struct PendingTiming {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+}  // namespace splat
+
+struct splat_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool timing = false;
+  std::vector<splat::PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+  std::map<std::string, splat::KernelStat> kstats;
+  std::map<std::string, splat::DevBuf> ws;  // named scratch buffers of the host tier
+
+  template <class T>
+  T* buf(const char* name, size_t count) {
+    return static_cast<T*>(ws[name].reserve(count * sizeof(T)));
+  }
+  hipEvent_t event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    SPLAT_HIP(hipEventCreate(&e));
+    return e;
+  }
+  // Collect finished timing events (synchronises the stream).
+  void drain_timing() {
+    if (pending.empty()) return;
+    SPLAT_HIP(hipStreamSynchronize(stream));
+    for (auto& p : pending) {
+      float ms = 0;
+      SPLAT_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+      auto& s = kstats[p.name];
+      s.launches++;
+      s.ms += ms;
+      event_pool.push_back(p.a);
+      event_pool.push_back(p.b);
+    }
+    pending.clear();
+  }
+};
+
+namespace splat {
+
+// Launch `kernel` on the context stream, bracketed by events when timing is on.
+template <class K, class... Args>
+inline void launch(splat_ctx* ctx, const char* name, K kernel, dim3 grid, dim3 block,
+                   size_t shmem, Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  hipEvent_t a = nullptr;
+  if (ctx->timing) {
+    a = ctx->event();
+    SPLAT_HIP(hipEventRecord(a, ctx->stream));
+  }
+  hipLaunchKernelGGL(kernel, grid, block, shmem, ctx->stream, args...);
+  SPLAT_HIP(hipGetLastError());
+  if (ctx->timing) {
+    hipEvent_t b = ctx->event();
+    SPLAT_HIP(hipEventRecord(b, ctx->stream));
+    ctx->pending.push_back({name, a, b});
+    if (ctx->pending.size() > 4096) ctx->drain_timing();
+  }
+}
+
+inline unsigned blocks_for(size_t n, unsigned threads) {
+  return (unsigned)((n + threads - 1) / threads);
+}
+
+inline size_t plane_stride(size_t n) { return (n + 63) & ~size_t(63); }
+
+// ---- device-side views -------------------------------------------------------------
+
+struct CameraDev {  // src/camera.rs:13-32
+  uint32_t width, height;
+  float fx, fy, cx, cy;
+  float rot[9];
+  float trans[3];
+};
+
+inline CameraDev camera_dev(const splat_camera& c) {
+  CameraDev d;
+  d.width = c.width;
+  d.height = c.height;
+  d.fx = c.fx;
+  d.fy = c.fy;
+  d.cx = c.cx;
+  d.cy = c.cy;
+  for (int i = 0; i < 9; i++) d.rot[i] = c.rot[i];
+  for (int i = 0; i < 3; i++) d.trans[i] = c.trans[i];
+  return d;
+}
+
+// Projection outputs for one view (device pointers, capacity n).
+struct ProjBuffers {
+  float4* rec0;     // {mx, my, A, B}
+  float4* rec1;     // {C, opacity, sigma_max, half_x}; half_x < 0 marks an empty footprint
+  float4* rec2;     // {half_y, r, g, b}
+  float* depth;
+  uint32_t* radius;
+  uint2* rect;      // {x0 | y0 << 16, x1 | y1 << 16}, tile_max exclusive
+  uint32_t* ntiles; // entries of each Gaussian (0 when culled)
+};
+
+// Binning of one view.
+struct BinBuffers {
+  uint32_t* entry_offset;    // n + 1 (exclusive scan of ntiles)
+  uint32_t* entry_gaussian;  // E
+  uint32_t* entry_tile;      // E
+  uint32_t* entry_slot;      // E (position within the tile, from the counting pass)
+  uint32_t* tile_count;      // T
+  uint32_t* tile_start;      // T + 1
+  uint64_t* bucket;          // E: (depth bits << 32 | entry), grouped by tile
+  uint32_t* sorted_entry;    // E: entry ids in (tile, depth, entry) order
+  uint32_t* sorted_gauss;    // E: their Gaussian ids
+  uint32_t tiles_x, tiles_y;
+};
+
+// ---- kernel launchers (one per kernel file) -------------------------------------------
+
+// project.hip
+void project_forward(splat_ctx* ctx, const float* params, size_t n, size_t stride,
+                     const splat_camera& cam, const splat_proj_opts& opts, ProjBuffers pb);
+void proj_pack(splat_ctx* ctx, ProjBuffers pb, size_t n, splat_proj* out);
+void proj_unpack(splat_ctx* ctx, const splat_proj* in, size_t n, uint32_t tiles_x,
+                 uint32_t tiles_y, ProjBuffers pb, uint32_t* bad_flag);
+void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stride,
+                      const splat_camera& cam, const splat_proj_opts& opts,
+                      const splat_proj_grad* dproj, float* grads);
+void aos_to_planes(splat_ctx* ctx, const float* aos, size_t n, float* planes, size_t stride);
+void planes_to_aos(splat_ctx* ctx, const float* planes, size_t n, size_t stride, float* aos);
+
+// scan.hip
+void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
+                        uint32_t* total_dev);
+size_t scan_temp_bytes(size_t n);
+
+// binning.hip
+// Builds the sorted per-tile lists.  Returns E (synchronises once to read it).
+size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uint32_t tiles_y,
+                 BinBuffers& bb, size_t* capacity_io, std::map<std::string, DevBuf>& store,
+                 uint64_t* keys_out);
+
+// raster.hip
+void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                    uint32_t width, uint32_t height, const float bg[3], float* image,
+                    float* final_t, uint32_t* n_contrib);
+void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                     uint32_t width, uint32_t height, const float bg[3], const float* final_t,
+                     const uint32_t* n_contrib, const float* dimage, bool ordered,
+                     float* slots);
+void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
+                  splat_proj_grad* grads, float* abs_grads);
+
+// loss.hip
+struct LossOut {
+  float* dimage;     // width*height*3
+  float* partials;   // per-block (ssim_sum, l1_sum) pairs
+  int num_partials;
+};
+void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* target_f32,
+                           const uint8_t* target_u8, uint32_t width, uint32_t height,
+                           float lambda, bool want_l1, float* dimage, float* scratch,
+                           double* loss_parts_host);
+size_t loss_scratch_floats(uint32_t width, uint32_t height);
+void blur_image(splat_ctx* ctx, const float* src, uint32_t width, uint32_t height,
+                uint32_t channels, float* dst);
+void gaussian_kernel_host(float k[11]);
+
+// optim.hip
+void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, float* v,
+                 size_t n, size_t stride, float c1, float c2, const splat_adam_state& st,
+                 const splat_learning_rates& lr);
+void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius,
+                          const splat_proj_grad* grads, const float* abs_grads, size_t n,
+                          uint32_t width, uint32_t height, float* grad_accum, uint32_t* denom,
+                          uint32_t* max_radii);
+
+}  // namespace splat

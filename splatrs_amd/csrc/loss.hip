@@ -1,0 +1,349 @@
+// Photometric loss (1 - lambda) L1 + lambda (1 - SSIM) and its image gradient
+// (src/loss.rs:14-190), in two fused passes.
+//
+// Pass 1 blurs x, y, x^2, y^2, xy (11-tap separable zero-padded, sigma 1.5) and writes
+// the SSIM partials d_mu, d_exx, d_exy plus per-block SSIM and L1 sums.  Pass 2 blurs the
+// three partials (the blur is its own adjoint) and combines them with the L1 sign term.
+//
+// Each workgroup owns a strip of 256 interleaved row elements (x * 3 + c) and a chunk of
+// rows.  Rows are staged in LDS; each lane keeps the last 11 horizontally blurred rows of
+// every map in registers and emits one vertically blurred row per step.  Taps are summed
+// in the reference's order (horizontal: k = 0..10; vertical: out += kv * line, k =
+// 0..10); out-of-image taps add +0.0, which is exact, so every intermediate equals the
+// reference's value bit for bit.
+#include "internal.hpp"
+
+namespace splat {
+namespace {
+
+constexpr int kStrip = 256;
+constexpr int kHalo = 15;        // 5 pixels x 3 channels
+constexpr int kRowsPerBlock = 32;
+
+struct Taps {
+  float k[11];
+};
+
+template <bool U8>
+__device__ __forceinline__ float load_target(const void* t, size_t idx) {
+  if (U8) return (float)static_cast<const uint8_t*>(t)[idx] / 255.0f;  // View::to_real
+  return static_cast<const float*>(t)[idx];
+}
+
+// Horizontal 11-tap blur at flat row position q (channel stride 3) from an LDS row
+// segment whose element 0 is row position q0 - kHalo.
+__device__ __forceinline__ float hblur(const float* seg, int q, int q0, int width3, const Taps& k) {
+  float acc = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 11; t++) {
+    const int qs = q + (t - 5) * 3;
+    const float v = (qs >= 0 && qs < width3) ? seg[qs - q0 + kHalo] : 0.0f;
+    acc += k.k[t] * v;
+  }
+  return acc;
+}
+
+template <int M>
+__device__ __forceinline__ void shift_in(float (&h)[M][11], const float (&nv)[M]) {
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+#pragma unroll
+    for (int t = 0; t < 10; t++) h[m][t] = h[m][t + 1];
+    h[m][10] = nv[m];
+  }
+}
+
+template <int M>
+__device__ __forceinline__ void vblur(const float (&h)[M][11], const Taps& k, float (&out)[M]) {
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 11; t++) acc += k.k[t] * h[m][t];
+    out[m] = acc;
+  }
+}
+
+__device__ __forceinline__ float block_sum(float v, float* sh) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  return ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// Pass 1.  partial[block] = {sum SSIM, sum |x - y|} over the block's outputs.
+template <bool U8>
+__global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict__ xin,
+                                                         const void* __restrict__ yin,
+                                                         uint32_t W, uint32_t H, Taps taps,
+                                                         float c1, float c2,
+                                                         float* __restrict__ d_mu,
+                                                         float* __restrict__ d_exx,
+                                                         float* __restrict__ d_exy,
+                                                         float2* __restrict__ partial) {
+  __shared__ float sx[kStrip + 2 * kHalo], sy[kStrip + 2 * kHalo];
+  __shared__ float red[4];
+  const int width3 = (int)W * 3;
+  const int q0 = blockIdx.x * kStrip;
+  const int q = q0 + threadIdx.x;
+  const bool col_ok = q < width3;
+  const int ybeg = blockIdx.y * kRowsPerBlock;
+  const int yend = min(ybeg + kRowsPerBlock, (int)H);
+  float h[5][11];
+#pragma unroll
+  for (int m = 0; m < 5; m++)
+#pragma unroll
+    for (int t = 0; t < 11; t++) h[m][t] = 0.0f;
+  float ssum = 0.0f, l1sum = 0.0f;
+  // Row rr enters the window; output row rr - 5 is then complete.
+  for (int rr = ybeg - 5; rr < yend + 5; rr++) {
+    float nv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();
+    if (rr >= 0 && rr < (int)H) {
+      for (int t = threadIdx.x; t < kStrip + 2 * kHalo; t += blockDim.x) {
+        const int qq = q0 - kHalo + t;
+        const bool in = qq >= 0 && qq < width3;
+        const size_t idx = (size_t)rr * width3 + qq;
+        sx[t] = in ? xin[idx] : 0.0f;
+        sy[t] = in ? load_target<U8>(yin, idx) : 0.0f;
+      }
+    }
+    __syncthreads();
+    if (rr >= 0 && rr < (int)H && col_ok) {
+      // x, y, x^2, y^2, xy blurred horizontally (products formed per element first)
+      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 11; t++) {
+        const int qs = q + (t - 5) * 3;
+        const bool in = qs >= 0 && qs < width3;
+        const float xv = in ? sx[qs - q0 + kHalo] : 0.0f;
+        const float yv = in ? sy[qs - q0 + kHalo] : 0.0f;
+        const float kk = taps.k[t];
+        a0 += kk * xv;
+        a1 += kk * yv;
+        a2 += kk * (xv * xv);
+        a3 += kk * (yv * yv);
+        a4 += kk * (xv * yv);
+      }
+      nv[0] = a0;
+      nv[1] = a1;
+      nv[2] = a2;
+      nv[3] = a3;
+      nv[4] = a4;
+    }
+    shift_in<5>(h, nv);
+    const int out_row = rr - 5;
+    if (out_row >= ybeg && out_row < yend && col_ok) {
+      float mu[5];
+      vblur<5>(h, taps, mu);
+      const float mx = mu[0], my = mu[1];
+      const float sx2 = mu[2] - mx * mx;
+      const float sy2 = mu[3] - my * my;
+      const float sxy = mu[4] - mx * my;
+      const float n1 = 2.0f * mx * my + c1;
+      const float n2 = 2.0f * sxy + c2;
+      const float d1 = mx * mx + my * my + c1;
+      const float d2 = sx2 + sy2 + c2;
+      const float num = n1 * n2;
+      const float den = d1 * d2;
+      const float s = num / den;
+      const float dn_dmx = 2.0f * my * n2 - 2.0f * my * n1;
+      const float dd_dmx = 2.0f * mx * d2 - 2.0f * mx * d1;
+      const size_t idx = (size_t)out_row * width3 + q;
+      d_mu[idx] = (dn_dmx * den - num * dd_dmx) / (den * den);
+      d_exx[idx] = -num * d1 / (den * den);
+      d_exy[idx] = 2.0f * n1 / den;
+      ssum += s;
+      l1sum += fabsf(xin[idx] - load_target<U8>(yin, idx));
+    }
+  }
+  const float bs = block_sum(ssum, red);
+  const float bl = block_sum(l1sum, red);
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = make_float2(bs, bl);
+}
+
+// Pass 2.  MODE 0: photometric-loss gradient; MODE 1: d(mean SSIM)/dx only.
+template <bool U8, int MODE>
+__global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict__ xin,
+                                                        const void* __restrict__ yin,
+                                                        uint32_t W, uint32_t H, Taps taps,
+                                                        const float* __restrict__ d_mu,
+                                                        const float* __restrict__ d_exx,
+                                                        const float* __restrict__ d_exy,
+                                                        float inv_n, float w1, float lambda,
+                                                        float* __restrict__ grad) {
+  __shared__ float s0[kStrip + 2 * kHalo], s1[kStrip + 2 * kHalo], s2[kStrip + 2 * kHalo];
+  const int width3 = (int)W * 3;
+  const int q0 = blockIdx.x * kStrip;
+  const int q = q0 + threadIdx.x;
+  const bool col_ok = q < width3;
+  const int ybeg = blockIdx.y * kRowsPerBlock;
+  const int yend = min(ybeg + kRowsPerBlock, (int)H);
+  float h[3][11];
+#pragma unroll
+  for (int m = 0; m < 3; m++)
+#pragma unroll
+    for (int t = 0; t < 11; t++) h[m][t] = 0.0f;
+  for (int rr = ybeg - 5; rr < yend + 5; rr++) {
+    float nv[3] = {0.0f, 0.0f, 0.0f};
+    __syncthreads();
+    if (rr >= 0 && rr < (int)H) {
+      for (int t = threadIdx.x; t < kStrip + 2 * kHalo; t += blockDim.x) {
+        const int qq = q0 - kHalo + t;
+        const bool in = qq >= 0 && qq < width3;
+        const size_t idx = (size_t)rr * width3 + qq;
+        s0[t] = in ? d_mu[idx] : 0.0f;
+        s1[t] = in ? d_exx[idx] : 0.0f;
+        s2[t] = in ? d_exy[idx] : 0.0f;
+      }
+    }
+    __syncthreads();
+    if (rr >= 0 && rr < (int)H && col_ok) {
+      nv[0] = hblur(s0, q, q0, width3, taps);
+      nv[1] = hblur(s1, q, q0, width3, taps);
+      nv[2] = hblur(s2, q, q0, width3, taps);
+    }
+    shift_in<3>(h, nv);
+    const int out_row = rr - 5;
+    if (out_row >= ybeg && out_row < yend && col_ok) {
+      float b[3];
+      vblur<3>(h, taps, b);
+      const size_t idx = (size_t)out_row * width3 + q;
+      const float xv = xin[idx];
+      const float yv = load_target<U8>(yin, idx);
+      const float g = (b[0] + 2.0f * xv * b[1] + yv * b[2]) * inv_n;
+      if (MODE == 1) {
+        grad[idx] = g;
+      } else {
+        const float d = xv - yv;
+        const float sign = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+        grad[idx] = w1 * sign - lambda * g;
+      }
+    }
+  }
+}
+
+// Separable blur for the host-tier blur() (arbitrary channel count).
+__global__ void blur_h_kernel(const float* __restrict__ src, uint32_t W, uint32_t H, uint32_t C,
+                              Taps taps, float* __restrict__ tmp) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t row = (size_t)W * C;
+  if (idx >= row * H) return;
+  const size_t y = idx / row, within = idx % row;
+  const int x = (int)(within / C), c = (int)(within % C);
+  float acc = 0.0f;
+  for (int k = 0; k < 11; k++) {
+    const int xs = x + k - 5;
+    if (xs >= 0 && xs < (int)W) acc += taps.k[k] * src[y * row + (size_t)xs * C + c];
+  }
+  tmp[idx] = acc;
+}
+
+__global__ void blur_v_kernel(const float* __restrict__ tmp, uint32_t W, uint32_t H, uint32_t C,
+                              Taps taps, float* __restrict__ dst) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t row = (size_t)W * C;
+  if (idx >= row * H) return;
+  const int y = (int)(idx / row);
+  const size_t within = idx % row;
+  float acc = 0.0f;
+  for (int k = 0; k < 11; k++) {
+    const int ys = y + k - 5;
+    if (ys >= 0 && ys < (int)H) acc += taps.k[k] * tmp[(size_t)ys * row + within];
+  }
+  dst[idx] = acc;
+}
+
+}  // namespace
+
+// gaussian_kernel, src/loss.rs:14-27 (host, glibc-compatible expf).
+void gaussian_kernel_host(float k[11]) {
+  const float sigma = 1.5f;
+  float sum = 0.0f;
+  for (int i = 0; i < 11; i++) {
+    const float x = (float)i - (float)(11 / 2);
+    k[i] = expf(-(x * x) / (2.0f * sigma * sigma));
+    sum += k[i];
+  }
+  for (int i = 0; i < 11; i++) k[i] /= sum;
+}
+
+size_t loss_scratch_floats(uint32_t width, uint32_t height) {
+  const size_t n = (size_t)width * height * 3;
+  const size_t blocks = (size_t)blocks_for(n / height, kStrip) * blocks_for(height, kRowsPerBlock);
+  return 3 * n + 2 * blocks + 64;
+}
+
+// Runs both passes.  target_u8 (trainer views) or target_f32 (host tier) must be given.
+// loss_parts_host, if non-null, receives {sum SSIM, sum |x - y|} (synchronises).
+// want_l1 = false computes d(mean SSIM)/dx into dimage (ssim_with_grad).
+void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* target_f32,
+                           const uint8_t* target_u8, uint32_t width, uint32_t height,
+                           float lambda, bool want_l1, float* dimage, float* scratch,
+                           double* loss_parts_host) {
+  const size_t n = (size_t)width * height * 3;
+  Taps taps;
+  gaussian_kernel_host(taps.k);
+  const float c1 = 0.01f * 0.01f, c2 = 0.03f * 0.03f;
+  float* d_mu = scratch;
+  float* d_exx = scratch + n;
+  float* d_exy = scratch + 2 * n;
+  float2* partial = reinterpret_cast<float2*>(scratch + 3 * n);
+  const dim3 grid(blocks_for((size_t)width * 3, kStrip), blocks_for(height, kRowsPerBlock));
+  const float inv_n = 1.0f / (float)n;
+  const float w1 = (1.0f - lambda) * inv_n;
+  if (target_u8) {
+    launch(ctx, "loss_ssim_stats", ssim_stats_kernel<true>, grid, dim3(256), 0, pred,
+           (const void*)target_u8, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial);
+    if (want_l1)
+      launch(ctx, "loss_grad", ssim_grad_kernel<true, 0>, grid, dim3(256), 0, pred,
+             (const void*)target_u8, width, height, taps, (const float*)d_mu,
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+    else
+      launch(ctx, "loss_grad", ssim_grad_kernel<true, 1>, grid, dim3(256), 0, pred,
+             (const void*)target_u8, width, height, taps, (const float*)d_mu,
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+  } else {
+    launch(ctx, "loss_ssim_stats", ssim_stats_kernel<false>, grid, dim3(256), 0, pred,
+           (const void*)target_f32, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial);
+    if (want_l1)
+      launch(ctx, "loss_grad", ssim_grad_kernel<false, 0>, grid, dim3(256), 0, pred,
+             (const void*)target_f32, width, height, taps, (const float*)d_mu,
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+    else
+      launch(ctx, "loss_grad", ssim_grad_kernel<false, 1>, grid, dim3(256), 0, pred,
+             (const void*)target_f32, width, height, taps, (const float*)d_mu,
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+  }
+  if (loss_parts_host) {
+    const size_t nb = (size_t)grid.x * grid.y;
+    std::vector<float2> parts(nb);
+    SPLAT_HIP(hipMemcpyAsync(parts.data(), partial, nb * sizeof(float2), hipMemcpyDeviceToHost,
+                             ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    double a = 0, b = 0;
+    for (auto& p : parts) {
+      a += p.x;
+      b += p.y;
+    }
+    loss_parts_host[0] = a;
+    loss_parts_host[1] = b;
+  }
+}
+
+void blur_image(splat_ctx* ctx, const float* src, uint32_t width, uint32_t height,
+                uint32_t channels, float* dst) {
+  Taps taps;
+  gaussian_kernel_host(taps.k);
+  const size_t n = (size_t)width * height * channels;
+  float* tmp = ctx->buf<float>("blur_tmp", n);
+  launch(ctx, "blur_h", blur_h_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, src, width, height,
+         channels, taps, tmp);
+  launch(ctx, "blur_v", blur_v_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (const float*)tmp, width, height, channels, taps, dst);
+}
+
+}  // namespace splat

@@ -1,0 +1,377 @@
+// Tile compositing, forward and backward (src/rasterize.rs:231-505).
+//
+// One 256-lane workgroup per 16x16 tile, one lane per pixel.  The tile's depth-sorted
+// entries are staged through LDS in batches (record gather + per-tile pixel rectangle,
+// TileGauss::new at src/rasterize.rs:149-182); every lane walks the batch with its own
+// compositing state, exactly the per-pixel walk of the reference's entry-outer loops.
+// The forward pass stops a workgroup when all its pixels are done.
+//
+// Backward: each entry's 11 per-pixel terms are reduced over the tile's pixels and
+// written to the entry's slots (src/rasterize.rs:449-454); a second kernel sums each
+// Gaussian's slots in entry order (src/rasterize.rs:480-503).  ORDERED=true sums the
+// pixel terms sequentially in row-major order -- the reference's order, so slots and
+// gradients are bitwise identical to it; ORDERED=false uses wave butterflies.
+#include "internal.hpp"
+
+namespace splat {
+namespace {
+
+constexpr int kBatch = 64;       // entries staged per LDS batch (forward)
+constexpr int kBwdBatch = 32;    // entries per batch (fast backward)
+constexpr int kOrdBatch = 8;     // entries per batch (ordered backward)
+
+struct TileInfo {
+  uint32_t gx0, gy0, w, h;
+};
+
+__device__ __forceinline__ TileInfo tile_info(uint32_t tile, uint32_t tiles_x, uint32_t W,
+                                              uint32_t H) {  // src/rasterize.rs:198-209
+  TileInfo t;
+  t.gx0 = (tile % tiles_x) * kTile;
+  t.gy0 = (tile / tiles_x) * kTile;
+  t.w = min(W - t.gx0, (uint32_t)kTile);
+  t.h = min(H - t.gy0, (uint32_t)kTile);
+  return t;
+}
+
+// Tile-local pixel range of one entry (the clip closure of src/rasterize.rs:158-164).
+__device__ __forceinline__ void clip_range(float m, float half, uint32_t offset, uint32_t n,
+                                           uint32_t* lo_out, uint32_t* hi_out) {
+  const float lo = ceilf((m - half) - 0.5f) - (float)offset;
+  const float hi = floorf((m + half) - 0.5f) - (float)offset + 1.0f;
+  *lo_out = as_u32(rclamp(lo, 0.0f, (float)n));
+  *hi_out = as_u32(rclamp(hi, 0.0f, (float)n));
+}
+
+// Packed tile-local rectangle x0 | x1 << 8 | y0 << 16 | y1 << 24 (all <= 16).
+__device__ __forceinline__ uint32_t entry_rect(float4 r0, float4 r1, float4 r2, const TileInfo& t) {
+  const float hx = r1.w, hy = r2.x;
+  if (!(hx >= 0.0f)) return 0u;  // sigma_max <= 0 or det <= 0: empty
+  uint32_t x0, x1, y0, y1;
+  clip_range(r0.x, hx, t.gx0, t.w, &x0, &x1);
+  clip_range(r0.y, hy, t.gy0, t.h, &y0, &y1);
+  return x0 | (x1 << 8) | (y0 << 16) | (y1 << 24);
+}
+
+__device__ __forceinline__ bool in_rect(uint32_t rc, uint32_t i, uint32_t j) {
+  return i >= (rc & 0xffu) && i < ((rc >> 8) & 0xffu) && j >= ((rc >> 16) & 0xffu) &&
+         j < (rc >> 24);
+}
+
+// evaluate, src/rasterize.rs:231-244.
+__device__ __forceinline__ bool evaluate(float4 r0, float4 r1, float px, float py,
+                                         const uint64_t* tab, float* gauss, float* alpha) {
+  const float dx = px - r0.x;
+  const float dy = py - r0.y;
+  const float sigma = 0.5f * (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;
+  if (sigma < 0.0f || sigma > r1.z) return false;
+  const float G = expf_tab(-sigma, tab);
+  const float og = r1.y * G;
+  if (og < kAlphaMin) return false;
+  *gauss = G;
+  *alpha = rmin(og, kAlphaMax);
+  return true;
+}
+
+__global__ void __launch_bounds__(256) raster_fwd_kernel(
+    const float4* __restrict__ rec0, const float4* __restrict__ rec1,
+    const float4* __restrict__ rec2, const uint32_t* __restrict__ tile_start,
+    const uint32_t* __restrict__ sorted_gauss, uint32_t tiles_x, uint32_t W, uint32_t H,
+    float bg0, float bg1, float bg2, float* __restrict__ image, float* __restrict__ final_t,
+    uint32_t* __restrict__ n_contrib) {
+  __shared__ uint64_t tab[32];
+  __shared__ float4 s0[kBatch], s1[kBatch], s2[kBatch];
+  __shared__ uint32_t srect[kBatch];
+  if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
+  const uint32_t tile = blockIdx.x;
+  const TileInfo ti = tile_info(tile, tiles_x, W, H);
+  const uint32_t i = threadIdx.x % kTile, j = threadIdx.x / kTile;
+  const bool valid = i < ti.w && j < ti.h;
+  const float px = (float)(ti.gx0 + i) + 0.5f;
+  const float py = (float)(ti.gy0 + j) + 0.5f;
+  float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+  uint32_t nc = 0;
+  bool done = !valid;
+  const uint32_t start = tile_start[tile], end = tile_start[tile + 1];
+  for (uint32_t b0 = start; b0 < end; b0 += kBatch) {
+    if (__syncthreads_and(done ? 1 : 0)) break;
+    const uint32_t cnt = min((uint32_t)kBatch, end - b0);
+    if (threadIdx.x < cnt) {
+      const uint32_t g = sorted_gauss[b0 + threadIdx.x];
+      const float4 a = rec0[g], b = rec1[g], c = rec2[g];
+      s0[threadIdx.x] = a;
+      s1[threadIdx.x] = b;
+      s2[threadIdx.x] = c;
+      srect[threadIdx.x] = entry_rect(a, b, c, ti);
+    }
+    __syncthreads();
+    if (!done) {
+      for (uint32_t k = 0; k < cnt; k++) {
+        if (!in_rect(srect[k], i, j)) continue;
+        const float4 a = s0[k], b = s1[k];
+        float G, alpha;
+        if (!evaluate(a, b, px, py, tab, &G, &alpha)) continue;
+        const float test_t = T * (1.0f - alpha);
+        if (test_t < kTMin) {
+          done = true;
+          break;
+        }
+        const float4 c = s2[k];
+        c0 += alpha * T * c.y;
+        c1 += alpha * T * c.z;
+        c2 += alpha * T * c.w;
+        T = test_t;
+        nc = b0 - start + k + 1;
+      }
+    }
+  }
+  if (valid) {
+    const size_t pix = (size_t)(ti.gy0 + j) * W + ti.gx0 + i;
+    image[pix * 3 + 0] = c0 + T * bg0;
+    image[pix * 3 + 1] = c1 + T * bg1;
+    image[pix * 3 + 2] = c2 + T * bg2;
+    final_t[pix] = T;
+    n_contrib[pix] = nc;
+  }
+}
+
+// Per-pixel backward state (PixelBack, src/rasterize.rs:345-350).
+struct PixBack {
+  float t, acc0, acc1, acc2, last_alpha, lc0, lc1, lc2;
+  float d0, d1, d2, bg_dot, t_final;
+  uint32_t last;
+};
+
+// Terms of one (entry, pixel) pair, src/rasterize.rs:404-445.  v[] = slot contributions.
+__device__ __forceinline__ void pair_terms(PixBack& s, float4 r0, float4 r1, float4 r2, float px,
+                                           float py, const uint64_t* tab, float v[kSlots]) {
+  float G, alpha;
+#pragma unroll
+  for (int q = 0; q < kSlots; q++) v[q] = 0.0f;
+  if (!evaluate(r0, r1, px, py, tab, &G, &alpha)) return;
+  s.t /= 1.0f - alpha;
+  const float t = s.t;
+  float dalpha = 0.0f;
+  s.acc0 = s.last_alpha * s.lc0 + (1.0f - s.last_alpha) * s.acc0;
+  dalpha += (r2.y - s.acc0) * s.d0;
+  v[6] = alpha * t * s.d0;
+  s.acc1 = s.last_alpha * s.lc1 + (1.0f - s.last_alpha) * s.acc1;
+  dalpha += (r2.z - s.acc1) * s.d1;
+  v[7] = alpha * t * s.d1;
+  s.acc2 = s.last_alpha * s.lc2 + (1.0f - s.last_alpha) * s.acc2;
+  dalpha += (r2.w - s.acc2) * s.d2;
+  v[8] = alpha * t * s.d2;
+  dalpha *= t;
+  s.lc0 = r2.y;
+  s.lc1 = r2.z;
+  s.lc2 = r2.w;
+  s.last_alpha = alpha;
+  dalpha += -s.t_final / (1.0f - alpha) * s.bg_dot;
+  const float o = r1.y;
+  if (o * G < kAlphaMax) {
+    v[5] = G * dalpha;
+    const float dgauss = o * dalpha;
+    const float dsigma = -dgauss * G;
+    const float dx = px - r0.x;
+    const float dy = py - r0.y;
+    const float ddx = dsigma * (r0.z * dx + r0.w * dy);
+    const float ddy = dsigma * (r0.w * dx + r1.x * dy);
+    v[0] = -ddx;
+    v[1] = -ddy;
+    v[2] = dsigma * 0.5f * dx * dx;
+    v[3] = dsigma * dx * dy;
+    v[4] = dsigma * 0.5f * dy * dy;
+    v[9] = fabsf(ddx);
+    v[10] = fabsf(ddy);
+  }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+template <bool ORDERED>
+__global__ void __launch_bounds__(256) raster_bwd_kernel(
+    const float4* __restrict__ rec0, const float4* __restrict__ rec1,
+    const float4* __restrict__ rec2, const uint32_t* __restrict__ tile_start,
+    const uint32_t* __restrict__ sorted_gauss, const uint32_t* __restrict__ sorted_entry,
+    uint32_t tiles_x, uint32_t W, uint32_t H, float bg0, float bg1, float bg2,
+    const float* __restrict__ final_t, const uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ dimage, float* __restrict__ slots) {
+  constexpr int B = ORDERED ? kOrdBatch : kBwdBatch;
+  __shared__ uint64_t tab[32];
+  __shared__ float4 s0[B], s1[B], s2[B];
+  __shared__ uint32_t srect[B];
+  __shared__ uint32_t smax_last;
+  // ORDERED: per-pixel terms of the batch; otherwise per-wave partial sums.
+  __shared__ float red[ORDERED ? B * kSlots * 256 : B * kSlots * 4];
+  if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
+  if (threadIdx.x == 0) smax_last = 0;
+  const uint32_t tile = blockIdx.x;
+  const TileInfo ti = tile_info(tile, tiles_x, W, H);
+  const uint32_t i = threadIdx.x % kTile, j = threadIdx.x / kTile;
+  const bool valid = i < ti.w && j < ti.h;
+  const float px = (float)(ti.gx0 + i) + 0.5f;
+  const float py = (float)(ti.gy0 + j) + 0.5f;
+  PixBack s;
+  s.acc0 = s.acc1 = s.acc2 = 0.0f;
+  s.lc0 = s.lc1 = s.lc2 = 0.0f;
+  s.last_alpha = 0.0f;
+  s.t = 1.0f;
+  s.t_final = 1.0f;
+  s.d0 = s.d1 = s.d2 = 0.0f;
+  s.bg_dot = 0.0f;
+  s.last = 0;
+  if (valid) {
+    const size_t pix = (size_t)(ti.gy0 + j) * W + ti.gx0 + i;
+    s.d0 = dimage[pix * 3 + 0];
+    s.d1 = dimage[pix * 3 + 1];
+    s.d2 = dimage[pix * 3 + 2];
+    s.bg_dot = bg0 * s.d0 + bg1 * s.d1 + bg2 * s.d2;
+    s.last = n_contrib[pix];
+    s.t_final = final_t[pix];
+    s.t = s.t_final;
+  }
+  const uint32_t start = tile_start[tile], end = tile_start[tile + 1];
+  const uint32_t count = end - start;
+  __syncthreads();
+  if (valid && s.last > 0) atomicMax(&smax_last, s.last);
+  __syncthreads();
+  const uint32_t kmax = min(smax_last, count);
+  // Entries no pixel reaches get zero slots.
+  for (uint32_t q = kmax * kSlots + threadIdx.x; q < count * kSlots; q += blockDim.x) {
+    const uint32_t k = q / kSlots;
+    slots[(size_t)sorted_entry[start + k] * kSlots + q % kSlots] = 0.0f;
+  }
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // Batches walk [0, kmax) from the back.
+  for (int32_t hi = (int32_t)kmax; hi > 0; hi -= B) {
+    const int32_t lo = max(hi - B, 0);
+    const uint32_t cnt = (uint32_t)(hi - lo);
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+      const uint32_t g = sorted_gauss[start + lo + threadIdx.x];
+      const float4 a = rec0[g], b = rec1[g], c = rec2[g];
+      s0[threadIdx.x] = a;
+      s1[threadIdx.x] = b;
+      s2[threadIdx.x] = c;
+      srect[threadIdx.x] = entry_rect(a, b, c, ti);
+    }
+    __syncthreads();
+    for (int32_t kk = (int32_t)cnt - 1; kk >= 0; kk--) {
+      const uint32_t k = lo + kk;
+      float v[kSlots];
+      bool act = valid && k < s.last && in_rect(srect[kk], i, j);
+      if (act) {
+        pair_terms(s, s0[kk], s1[kk], s2[kk], px, py, tab, v);
+      } else {
+#pragma unroll
+        for (int q = 0; q < kSlots; q++) v[q] = 0.0f;
+      }
+      if (ORDERED) {
+#pragma unroll
+        for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + threadIdx.x] = v[q];
+      } else {
+        const bool any = __any(act);
+        if (any) {
+#pragma unroll
+          for (int q = 0; q < kSlots; q++) {
+            const float r = wave_sum(v[q]);
+            if (lane == 0) red[(kk * kSlots + q) * 4 + wave] = r;
+          }
+        } else if (lane == 0) {
+#pragma unroll
+          for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 4 + wave] = 0.0f;
+        }
+      }
+    }
+    __syncthreads();
+    // Reduce the batch and write its slots.
+    for (uint32_t q = threadIdx.x; q < cnt * kSlots; q += blockDim.x) {
+      const uint32_t kk = q / kSlots, slot = q % kSlots;
+      float acc = 0.0f;
+      if (ORDERED) {
+        const float* p = &red[(kk * kSlots + slot) * 256];
+        for (int t = 0; t < 256; t++) acc += p[t];
+      } else {
+        const float* p = &red[(kk * kSlots + slot) * 4];
+        acc = ((p[0] + p[1]) + p[2]) + p[3];
+      }
+      slots[(size_t)sorted_entry[start + lo + kk] * kSlots + slot] = acc;
+    }
+  }
+}
+
+// Sum each Gaussian's slots over its entries in entry order (src/rasterize.rs:480-503).
+__global__ void __launch_bounds__(256) reduce_slots_kernel(const uint32_t* __restrict__ entry_offset,
+                                                           const float* __restrict__ slots,
+                                                           uint32_t n,
+                                                           splat_proj_grad* __restrict__ grads,
+                                                           float* __restrict__ abs_grads) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  float sum[kSlots];
+#pragma unroll
+  for (int k = 0; k < kSlots; k++) sum[k] = 0.0f;
+  const uint32_t s = entry_offset[g], e = entry_offset[g + 1];
+  for (uint32_t entry = s; entry < e; entry++) {
+    const float* p = slots + (size_t)entry * kSlots;
+#pragma unroll
+    for (int k = 0; k < kSlots; k++) sum[k] += p[k];
+  }
+  splat_proj_grad pg;
+  pg.mean2d[0] = sum[0];
+  pg.mean2d[1] = sum[1];
+  pg.conic[0] = sum[2];
+  pg.conic[1] = sum[3];
+  pg.conic[2] = sum[4];
+  pg.opacity = sum[5];
+  pg.color[0] = sum[6];
+  pg.color[1] = sum[7];
+  pg.color[2] = sum[8];
+  grads[g] = pg;
+  if (abs_grads) {
+    abs_grads[2 * (size_t)g] = sum[9];
+    abs_grads[2 * (size_t)g + 1] = sum[10];
+  }
+}
+
+}  // namespace
+
+void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, uint32_t width,
+                    uint32_t height, const float bg[3], float* image, float* final_t,
+                    uint32_t* n_contrib) {
+  const unsigned T = bb.tiles_x * bb.tiles_y;
+  launch(ctx, "raster_fwd", raster_fwd_kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec0,
+         (const float4*)pb.rec1, (const float4*)pb.rec2, (const uint32_t*)bb.tile_start,
+         (const uint32_t*)bb.sorted_gauss, bb.tiles_x, width, height, bg[0], bg[1], bg[2], image,
+         final_t, n_contrib);
+}
+
+void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, uint32_t width,
+                     uint32_t height, const float bg[3], const float* final_t,
+                     const uint32_t* n_contrib, const float* dimage, bool ordered, float* slots) {
+  const unsigned T = bb.tiles_x * bb.tiles_y;
+  if (ordered)
+    launch(ctx, "raster_bwd_ordered", raster_bwd_kernel<true>, dim3(T), dim3(256), 0,
+           (const float4*)pb.rec0, (const float4*)pb.rec1, (const float4*)pb.rec2,
+           (const uint32_t*)bb.tile_start, (const uint32_t*)bb.sorted_gauss,
+           (const uint32_t*)bb.sorted_entry, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
+           final_t, n_contrib, dimage, slots);
+  else
+    launch(ctx, "raster_bwd", raster_bwd_kernel<false>, dim3(T), dim3(256), 0,
+           (const float4*)pb.rec0, (const float4*)pb.rec1, (const float4*)pb.rec2,
+           (const uint32_t*)bb.tile_start, (const uint32_t*)bb.sorted_gauss,
+           (const uint32_t*)bb.sorted_entry, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
+           final_t, n_contrib, dimage, slots);
+}
+
+void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
+                  splat_proj_grad* grads, float* abs_grads) {
+  launch(ctx, "reduce_slots", reduce_slots_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         entry_offset, slots, (uint32_t)n, grads, abs_grads);
+}
+
+}  // namespace splat

@@ -1,0 +1,120 @@
+// Device-wide exclusive prefix sum of uint32 (three passes: block sums, scan of the block
+// sums in one workgroup, block-local rescan plus offset).  out[n] receives the total.
+#include "internal.hpp"
+
+namespace splat {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 2048 items per block
+constexpr int kTopThreads = 1024;
+constexpr int kTopItems = 16;  // one workgroup scans up to 16384 block sums
+
+// Inclusive scan of one value per thread across the workgroup; returns the inclusive
+// prefix and writes the workgroup total to *total.
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  if (lane == 63) sh[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < THREADS / 64; w++) {
+      const uint32_t t = sh[w];
+      sh[w] = run;
+      run += t;
+    }
+    sh[THREADS / 64] = run;
+  }
+  __syncthreads();
+  v += sh[wave];
+  *total = sh[THREADS / 64];
+  __syncthreads();
+  return v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(const uint32_t* __restrict__ in,
+                                                                   size_t n,
+                                                                   uint32_t* __restrict__ block_sums) {
+  __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++)
+    if (base + k < n) s += in[base + k];
+  uint32_t total;
+  block_inclusive_scan<kScanThreads>(s, sh, &total);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kTopThreads) scan_top_kernel(uint32_t* __restrict__ sums, uint32_t nb,
+                                                               uint32_t* __restrict__ total_out) {
+  __shared__ uint32_t sh[kTopThreads / 64 + 1];
+  uint32_t vals[kTopItems];
+  uint32_t s = 0;
+  const uint32_t base = threadIdx.x * kTopItems;
+#pragma unroll
+  for (int k = 0; k < kTopItems; k++) {
+    vals[k] = (base + k < nb) ? sums[base + k] : 0u;
+    s += vals[k];
+  }
+  uint32_t total;
+  const uint32_t incl = block_inclusive_scan<kTopThreads>(s, sh, &total);
+  uint32_t run = incl - s;
+#pragma unroll
+  for (int k = 0; k < kTopItems; k++) {
+    if (base + k < nb) sums[base + k] = run;
+    run += vals[k];
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const uint32_t* __restrict__ in,
+                                                                  size_t n,
+                                                                  const uint32_t* __restrict__ block_offsets,
+                                                                  uint32_t* __restrict__ out,
+                                                                  const uint32_t* __restrict__ total) {
+  __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+  uint32_t vals[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    vals[k] = (base + k < n) ? in[base + k] : 0u;
+    s += vals[k];
+  }
+  uint32_t tot;
+  const uint32_t incl = block_inclusive_scan<kScanThreads>(s, sh, &tot);
+  uint32_t run = block_offsets[blockIdx.x] + incl - s;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += vals[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
+}
+
+}  // namespace
+
+size_t scan_temp_bytes(size_t n) {
+  return (blocks_for(n == 0 ? 1 : n, kScanTile) + 1) * sizeof(uint32_t);
+}
+
+void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
+                        uint32_t* total_dev) {
+  const unsigned nb = blocks_for(n == 0 ? 1 : n, kScanTile);
+  SPLAT_REQUIRE(nb <= (unsigned)(kTopThreads * kTopItems), "scan: input too large");
+  uint32_t* sums = ctx->buf<uint32_t>("scan_block_sums", nb + 1);
+  launch(ctx, "scan_reduce", scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, in, n, sums);
+  launch(ctx, "scan_top", scan_top_kernel, dim3(1), dim3(kTopThreads), 0, sums, nb, total_dev);
+  launch(ctx, "scan_apply", scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, in, n,
+         (const uint32_t*)sums, out, (const uint32_t*)total_dev);
+}
+
+}  // namespace splat

@@ -1,0 +1,293 @@
+"""Parity of the HIP library with the CPU oracle on identical inputs (needs a gfx950 GPU).
+
+Bars: projection records, tile binning (entry lists, sorted keys and entries, tile
+ranges), the photometric-loss image gradient, Adam, the densify masks/outputs and the
+statistics are bit-exact.  Composited pixels are within 1e-5 (absolute) of the oracle
+and in practice bitwise equal except where the per-Gaussian alpha-threshold exponent
+ln(opacity * 255) rounds differently (the GPU rounds a binary64 log; glibc's logf
+differs from correct rounding in ~5e-4 of inputs).  Parameter gradients are within 1e-4
+relative (plus an absolute floor of 1e-4 of the field's largest magnitude).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from helpers import ground_truth, random_gaussian, ref_test_camera, ring_cameras
+from splatrs_amd import abi, scenes
+from splatrs_amd import records as rec
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = abi.Context(0)
+    yield c
+    c.close()
+
+
+def proj_bits(p):
+    return np.frombuffer(p.tobytes(), np.uint32).reshape(len(p), 15)
+
+
+def assert_proj_equal(a, b):
+    ba, bb = proj_bits(a), proj_bits(b)
+    bad = np.nonzero((ba != bb).any(1))[0]
+    assert len(bad) == 0, f"{len(bad)} projection records differ, first {bad[:5]}: {a[bad[:2]]} vs {b[bad[:2]]}"
+
+
+def assert_binning_equal(g, o):
+    assert g.tiles_x == o.tiles_x and g.tiles_y == o.tiles_y
+    assert g.num_entries == o.num_entries
+    assert np.array_equal(g.entry_gaussian, o.entry_gaussian)
+    assert np.array_equal(g.entry_offset, o.entry_offset)
+    assert np.array_equal(g.sorted_keys, o.sorted_keys)
+    assert np.array_equal(g.sorted_entries, o.sorted_entries)
+    assert np.array_equal(g.tile_range, o.tile_range)
+
+
+def grad_close(a, b, what, rtol=1e-4):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    floor = rtol * max(np.abs(b).max(), 1e-30)
+    err = np.abs(a - b) - (rtol * np.maximum(np.abs(a), np.abs(b)) + floor)
+    i = int(np.argmax(err))
+    assert err.max() <= 0, f"{what}: worst index {i}: {a.flat[i]} vs {b.flat[i]}"
+
+
+def scene_small():
+    rng = np.random.default_rng(500)
+    g = ground_truth(rng, 120)
+    g[::10, 3:6] = [-0.7, -0.9, -0.8]
+    g[::7, 10] = -5.0
+    return g, ring_cameras(1, 70, 50, 2.4)[0], rec.proj_opts(1, True), (0.3, 0.6, 0.1)
+
+
+def scene_c0(view=0):
+    return scenes.small_scene(0), scenes.small_scene_cameras()[view], rec.proj_opts(3), (0., 0., 0.)
+
+
+def full_pipeline(ctx, g, cam, opts, bg, ordered=True, check_bwd=True, seed=2):
+    w, h = int(cam["width"]), int(cam["height"])
+    pg = ctx.project_all(g, cam, opts)
+    po = O.project_all(g, cam, opts)
+    assert_proj_equal(pg, po)
+    bgpu = ctx.bin(pg, w, h)
+    bora = O.bin(po, w, h)
+    assert_binning_equal(bgpu, bora)
+    st = rec.raster_settings(w, h, bg)
+    ig, tg, ng = ctx.rasterize_forward(po, bora, st)
+    io, to, no = O.rasterize_forward(po, bora, st)
+    assert np.abs(ig - io).max() <= 1e-5, np.abs(ig - io).max()
+    assert np.abs(tg - to).max() <= 1e-5
+    frac_same = (ng == no).mean()
+    assert frac_same >= 0.9999, frac_same
+    say(f"forward identical pixels {np.mean((ig == io).all(-1)):.6f}")
+    if not check_bwd:
+        return
+    dimg = np.random.default_rng(seed).standard_normal(io.shape).astype(np.float32)
+    gg, ag = ctx.rasterize_backward(po, bora, st, to, no, dimg, ordered=ordered)
+    go, ao = O.rasterize_backward(po, bora, st, to, no, dimg)
+    for f in ("mean2d", "conic", "opacity", "color"):
+        if ordered:
+            same = np.mean(gg[f] == go[f])
+            say(f"backward {f}: bitwise-equal fraction {same:.6f}")
+        grad_close(gg[f], go[f], f)
+    grad_close(ag, ao, "abs")
+    pbg = ctx.project_backward_all(g, cam, opts, go)
+    pbo = O.project_backward_all(g, cam, opts, go)
+    assert np.array_equal(pbg.view(np.uint32), pbo.view(np.uint32))
+
+
+def test_small_scene_pipeline(ctx):
+    say("small ring scene 70x50, antialias, SH1")
+    full_pipeline(ctx, *scene_small())
+
+
+def test_small_scene_pipeline_fast_backward(ctx):
+    say("small ring scene, tree-reduced backward")
+    full_pipeline(ctx, *scene_small(), ordered=False)
+
+
+@pytest.mark.parametrize("view", [0, 3])
+def test_config0_scene_pipeline(ctx, view):
+    say(f"10k Gaussians, 256x256, view {view}")
+    full_pipeline(ctx, *scene_c0(view))
+
+
+def test_random_gaussians_all_degrees(ctx):
+    rng = np.random.default_rng(11)
+    g = np.array([random_gaussian(rng) for _ in range(300)], np.float32)
+    cam = ref_test_camera(96, 64)
+    for deg in range(4):
+        for aa in (False, True):
+            say(f"random gaussians degree {deg} antialias {aa}")
+            full_pipeline(ctx, g, cam, rec.proj_opts(deg, aa), (0.2, 0.5, 0.8))
+
+
+def test_empty_and_culled(ctx):
+    say("empty model and all-culled model")
+    cam = ref_test_camera(48, 32)
+    empty = np.zeros((0, 59), np.float32)
+    assert len(ctx.project_all(empty, cam, rec.proj_opts())) == 0
+    img = ctx.render(empty, cam, rec.proj_opts(), (0.1, 0.2, 0.3))
+    assert np.all(img == np.array([0.1, 0.2, 0.3], np.float32))
+    g = np.zeros((5, 59), np.float32)
+    g[:, 6] = 1.0
+    g[:, 2] = -10.0  # behind the camera
+    full_pipeline(ctx, g, cam, rec.proj_opts(), (0.0, 0.0, 0.0))
+
+
+def test_duplicates_break_ties_by_entry(ctx):
+    say("cloned Gaussians: equal depth keys")
+    rng = np.random.default_rng(1)
+    g = ground_truth(rng, 40)
+    g = np.concatenate([g, g[:20], g[:5]])
+    full_pipeline(ctx, g, ring_cameras(1, 48, 40, 2.4)[0], rec.proj_opts(1), (0.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("n", [3000, 20000])
+def test_crowded_tiles_use_large_sorts(ctx, n):
+    # n Gaussians around one point: every tile of a 64x64 image gets n entries, which
+    # exercises the 1024-lane LDS sort (3000) and the global-memory sort (20000).
+    say(f"crowded tiles, {n} entries per tile")
+    rng = np.random.default_rng(n)
+    g = np.zeros((n, 59), np.float32)
+    g[:, 0:3] = rng.normal(0, 0.02, (n, 3))
+    g[:, 3:6] = -0.5
+    g[:, 6:10] = rng.standard_normal((n, 4))
+    g[:, 10] = -3.0
+    g[:, 11:14] = rng.normal(0.5, 0.2, (n, 3))
+    cam = scenes.look_at(64, 64, 60.0, 60.0, (0.0, 0.0, -4.0), (0.0, 0.0, 0.0))
+    pg = ctx.project_all(g, cam, rec.proj_opts(0))
+    bgpu = ctx.bin(pg, 64, 64)
+    bora = O.bin(O.project_all(g, cam, rec.proj_opts(0)), 64, 64)
+    assert_binning_equal(bgpu, bora)
+    assert (bora.tile_range[:, 1] - bora.tile_range[:, 0]).max() >= n
+    full_pipeline(ctx, g[: min(n, 3000)], cam, rec.proj_opts(0), (0.0, 0.0, 0.0))
+
+
+def test_loss_gradient_bit_exact(ctx):
+    rng = np.random.default_rng(3)
+    for (w, h) in [(16, 12), (70, 50), (300, 173)]:
+        say(f"photometric loss {w}x{h}")
+        x = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+        y = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+        lg, gg = ctx.photometric_loss(x, y, w, h, 0.2)
+        lo, go = O.photometric_loss(x, y, w, h, 0.2)
+        assert np.array_equal(gg.view(np.uint32), go.view(np.uint32))
+        assert abs(lg - lo) <= 1e-5 * abs(lo)
+        sg, dg = ctx.ssim_with_grad(x, y, w, h)
+        so, do = O.ssim_with_grad(x, y, w, h)
+        assert np.array_equal(dg.view(np.uint32), do.view(np.uint32))
+        assert abs(sg - so) <= 1e-5
+        for ch in (1, 3, 4):
+            src = rng.uniform(-1, 1, h * w * ch).astype(np.float32)
+            assert np.array_equal(ctx.blur(src, w, h, ch).view(np.uint32),
+                                  O.blur(src, w, h, ch).view(np.uint32))
+
+
+def test_adam_bit_exact(ctx):
+    say("adam 20 steps on 1000 records")
+    rng = np.random.default_rng(4)
+    n = 1000
+    p = rng.standard_normal((n, 59)).astype(np.float32)
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    p2, m2, v2 = p.copy(), m.copy(), v.copy()
+    s1, s2 = rec.adam_state(), rec.adam_state()
+    lr = rec.learning_rates(4.8e-4)
+    for step in range(20):
+        g = rng.standard_normal((n, 59)).astype(np.float32)
+        g[::4] = 0
+        ctx.adam_step(p, g, m, v, s1, lr)
+        O.adam_step(p2, g, m2, v2, s2, lr)
+    assert int(s1["step"]) == int(s2["step"]) == 20
+    for a, b in ((p, p2), (m, m2), (v, v2)):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_stats_and_densify_bit_exact(ctx):
+    rng = np.random.default_rng(9)
+    n = 5000
+    params = np.zeros((n, 59), np.float32)
+    params[:, 0:3] = rng.uniform(-1, 1, (n, 3))
+    params[:, 3:6] = rng.uniform(-6, -1, (n, 3))
+    params[:, 6:10] = rng.standard_normal((n, 4))
+    params[:, 10] = rng.uniform(-7, 3, n)
+    params[:, 11:] = rng.standard_normal((n, 48))
+    m = rng.standard_normal((n, 59)).astype(np.float32)
+    v = rng.uniform(0, 1, (n, 59)).astype(np.float32)
+    projs = np.zeros(n, rec.PROJ)
+    projs["radius"] = rng.integers(0, 30, n)
+    grads = np.zeros(n, rec.PROJ_GRAD)
+    grads["mean2d"] = rng.standard_normal((n, 2)).astype(np.float32) * 3e-6
+    absg = np.abs(rng.standard_normal((n, 2))).astype(np.float32) * 3e-6
+    for use_abs in (False, True):
+        say(f"stats accumulate abs={use_abs}")
+        acc = [np.zeros(n, np.float32) for _ in range(2)]
+        den = [np.zeros(n, np.uint32) for _ in range(2)]
+        mr = [np.zeros(n, np.uint32) for _ in range(2)]
+        for _ in range(3):
+            ctx.stats_accumulate(projs, grads, absg if use_abs else None, 640, 480, acc[0],
+                                 den[0], mr[0])
+            O.stats_accumulate(projs, grads, absg if use_abs else None, 640, 480, acc[1],
+                               den[1], mr[1])
+        assert np.array_equal(acc[0].view(np.uint32), acc[1].view(np.uint32))
+        assert np.array_equal(den[0], den[1]) and np.array_equal(mr[0], mr[1])
+    extent = np.float32(4.0)
+    for cfg in (rec.densify_config(), rec.densify_config(prune_big_world=True),
+                rec.densify_config(max_screen_size=20), rec.densify_config(split_children=3)):
+        say(f"densify cfg {cfg}")
+        normals = rng.standard_normal(3 * int(cfg["split_children"]) * n).astype(np.float32)
+        outg = ctx.densify_and_prune(params, m, v, acc[0], den[0], mr[0], cfg, extent, normals)
+        outo = O.densify_and_prune(params, m, v, acc[1], den[1], mr[1], cfg, extent, normals)
+        assert tuple(outg[3].tolist()) == tuple(outo[3].tolist())
+        for a, b in zip(outg[:3], outo[:3]):
+            assert a.shape == b.shape
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    say("reset opacity")
+    a = [params.copy(), m.copy(), v.copy()]
+    b = [params.copy(), m.copy(), v.copy()]
+    ctx.reset_opacity(*a, 0.01)
+    O.reset_opacity(*b, 0.01)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+
+
+def test_render_matches_oracle_pipeline(ctx):
+    say("render (project -> bin -> forward)")
+    g, cam, opts, bg = scene_c0(1)
+    img = ctx.render(g, cam, opts, bg)
+    po = O.project_all(g, cam, opts)
+    bo = O.bin(po, 256, 256)
+    io, _, _ = O.rasterize_forward(po, bo, rec.raster_settings(256, 256, bg))
+    assert np.abs(img - io).max() <= 1e-5
+
+
+def test_malformed_binning_is_a_precondition_error(ctx):
+    say("malformed binning rejected on the host")
+    g, cam, opts, bg = scene_small()
+    po = O.project_all(g, cam, opts)
+    b = O.bin(po, 70, 50)
+    b.sorted_entries = b.sorted_entries.copy()
+    b.sorted_entries[0] = 10 ** 9
+    with pytest.raises(abi.PreconditionError):
+        ctx.rasterize_forward(po, b, rec.raster_settings(70, 50))
+    with pytest.raises(abi.PreconditionError):
+        ctx.bin(po, 32, 32)  # rectangles made for a larger image
+
+
+def test_full_size_1m_1080p(ctx):
+    say("1M Gaussians, 1920x1080, forward + ordered backward + loss")
+    g = scenes.large_scene(0)
+    cam = scenes.hd_camera()
+    opts = rec.proj_opts(3)
+    full_pipeline(ctx, g, cam, opts, (0.0, 0.0, 0.0))
