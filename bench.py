@@ -1,0 +1,168 @@
+"""Training-iteration throughput of the MI355X splat trainer at 1M Gaussians, 1920x1080.
+
+    python bench.py [--steps K] [--warmup W] [--ordered] [--cpu-steps C]
+
+Workload (one GPU): 1M Gaussians drawn as means U[-5,5]^3, log-scales N(-5,0.7), random
+unit quaternions, opacity logits N(0,1), SH degree 3; one 1920x1080 pinhole camera with
+fx = fy = 1600 at distance 8; 16x16 tiles.  The target image is rendered from a second
+scene of the same generator (seed 1) and stored as RGB8.  One step = project -> bin ->
+composite -> L1 + 0.2 D-SSIM loss -> backward -> Adam (src/train.rs:211-247), all
+resident in HBM.  `value` is iterations/s over K timed steps with inputs resident.
+
+A second pass of K steps runs with HIP events around every kernel (on the trainer's
+stream) to attribute time per kernel; the roofline entry uses the dominant kernel's
+algorithmic bytes per launch over its mean duration.  The CPU baseline runs the oracle's
+train step (a scalar C++ restatement of the reference) on the host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "train iters/s @1M Gaussians 1080p (plain vs batched+learned) at PSNR parity"
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(name, N, V, E, P):
+    """Bytes a kernel must move per launch (reads + writes of its inputs/outputs)."""
+    table = {
+        "project_fwd": 304 * N,              # 236 params read; 68 record/depth/radius/rect/count
+        "project_bwd": 508 * N,              # 236 params + 36 ProjGrad read; 236 grads written
+        "adam": 1652 * N,                    # p, g, m, v read; p, m, v written (59 x 4 B each)
+        "raster_fwd": 52 * E + 20 * P,       # 48 B record + 4 B id per entry; image, T, n
+        "raster_bwd": 100 * E + 20 * P,      # record + 2 ids per entry, 44 B slots; dL, T, n
+        "raster_bwd_ordered": 100 * E + 20 * P,
+        "reduce_slots": 44 * E + 44 * N,     # slots read; ProjGrad + abs written
+        "loss_ssim_stats": 51 * P,           # pred 12 + target 3 read; 3 partial maps written
+        "loss_grad": 63 * P,                 # 3 maps + pred + target read; dL/dimage written
+        "bin_keygen": 12 * E + 16 * N,
+        "bin_scatter": 28 * E,
+        "bin_sort_small": 24 * E,
+        "stats": 24 * N,
+    }
+    return table.get(name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--ordered", action="store_true",
+                    help="reference summation order in the backward pass")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="oracle train steps timed for the CPU baseline (0 = skip)")
+    ap.add_argument("--gaussians", type=int, default=1_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from splatrs_amd import abi, scenes
+    from splatrs_amd import records as rec
+
+    ctx = abi.Context(0)
+    g = scenes.large_scene(0, args.gaussians)
+    cam = scenes.hd_camera()
+    W, H = int(cam["width"]), int(cam["height"])
+    gt = scenes.large_scene(1, args.gaussians)
+    target = abi.to_rgb8(ctx.render(gt, cam, rec.proj_opts(3)))
+    del gt
+    trainer = abi.Trainer(ctx, g)
+    view = trainer.add_view(cam, target)
+    extent = scenes.scene_extent([cam])
+
+    def params(it, want_loss=False):
+        lr = rec.learning_rates(rec.position_lr(it, extent))
+        return rec.step_params(3, lr, ordered_backward=args.ordered, want_loss=want_loss)
+
+    it = 0
+    info = None
+    for _ in range(max(args.warmup, 1)):
+        it += 1
+        r = trainer.step(view, params(it, want_loss=(info is None)))
+        if info is None:
+            info = dict(loss0=float(r["loss"]), entries=int(r["num_entries"]),
+                        visible=int(r["num_visible"]))
+    ctx.sync()
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        it += 1
+        trainer.step(view, params(it))
+    ctx.sync()
+    t1 = time.perf_counter()
+    sec = t1 - t0
+    value = args.steps / sec
+
+    # per-kernel attribution (separate pass: events add a little launch overhead)
+    ctx.set_timing(True)
+    ctx.reset_timing()
+    for _ in range(args.steps):
+        it += 1
+        trainer.step(view, params(it))
+    times = ctx.kernel_times()
+    ctx.set_timing(False)
+    r = trainer.step(view, params(it + 1, want_loss=True))
+    N, V, E, P = args.gaussians, int(r["num_visible"]), int(r["num_entries"]), W * H
+
+    kernels = {}
+    for name, (launches, total_ms) in times.items():
+        per_step = total_ms / args.steps
+        kernels[name] = {"ms_per_step": round(per_step, 4),
+                         "launches_per_step": launches / args.steps,
+                         "mean_ms": round(total_ms / max(launches, 1), 4)}
+        b = algorithmic_bytes(name, N, V, E, P)
+        if b is not None and launches:
+            kernels[name]["gbs"] = round(b / (total_ms / launches * 1e-3) / 1e9, 1)
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    roof_name = dom if algorithmic_bytes(dom, N, V, E, P) is not None else max(
+        (k for k in kernels if algorithmic_bytes(k, N, V, E, P) is not None),
+        key=lambda k: kernels[k]["ms_per_step"])
+    b = algorithmic_bytes(roof_name, N, V, E, P)
+    mean_s = kernels[roof_name]["mean_ms"] * 1e-3
+    achieved = b / mean_s / 1e9
+    roofline = {"kernel": roof_name, "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "algorithmic_bytes": int(b), "mean_ms": kernels[roof_name]["mean_ms"]}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        st = O.TrainState(g)
+        lr = rec.learning_rates(rec.position_lr(1, extent))
+        t0 = time.perf_counter()
+        for k in range(args.cpu_steps):
+            st.train_step(cam, target, 3, lr)
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(args.cpu_steps / cs, 4), "unit": "it/s", "cores": 1,
+               "kind": "port",
+               "sample": f"{args.cpu_steps} full train step(s) of the same 1M/1080p workload, "
+                         f"scalar C++ oracle, {cs:.1f} s"}
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "it/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sec / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "1M Gaussians, 1920x1080 single-view train step (plain method)",
+                   "gaussians": N, "width": W, "height": H, "tile": 16, "sh_degree": 3,
+                   "visible": V, "tile_entries": E,
+                   "backward": "ordered" if args.ordered else "tree-reduced",
+                   "loss_at_step1": round(info["loss0"], 6)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -260,6 +260,61 @@ int splat_densify_and_prune(splat_ctx* ctx, const float* params, const float* m,
 int splat_reset_opacity(splat_ctx* ctx, float* params, float* m, float* v, uint64_t n,
                         float max_alpha);
 
+/* ---- trainer: device-resident training (src/train.rs) ---------------------------- */
+
+typedef struct splat_trainer splat_trainer;
+
+/* Per-iteration inputs of one training step (the values train.rs computes on the host
+ * each iteration, src/train.rs:196-247). */
+typedef struct {
+  int32_t sh_degree;            /* model.sh_degree (train.rs:197-202) */
+  int32_t accumulate_stats;     /* densify && it < densify_until_iter (train.rs:228) */
+  int32_t absgrad;              /* TrainConfig::absgrad */
+  int32_t ordered_backward;     /* 1: the reference's summation order (bitwise), 0: fast */
+  int32_t want_loss;            /* 1: compute the loss value (synchronises the stream) */
+  int32_t antialias;            /* ProjOpts::antialias */
+  float near_plane;             /* ProjOpts::near (0.2) */
+  float lambda_dssim;           /* 0.2 */
+  float background[3];
+  splat_learning_rates lr;      /* with lr.mean = position_lr(it, extent) */
+} splat_step_params;
+
+typedef struct {
+  float loss;              /* (1 - lambda) L1 + lambda (1 - SSIM), if want_loss */
+  uint64_t num_entries;    /* tile entries of this view */
+  uint64_t num_visible;    /* Gaussians with radius > 0 (if want_loss) */
+} splat_step_result;
+
+/* Create a trainer holding n Gaussians (AoS, 59 floats each) with room for `capacity`
+ * (grown on demand by densification).  Adam state starts at zero, step 0
+ * (Adam::new, src/adam.rs:65-74); statistics at zero (DensifyStats::new). */
+int splat_trainer_create(splat_ctx* ctx, const float* gaussians, uint64_t n, uint64_t capacity,
+                         splat_trainer** out);
+int splat_trainer_destroy(splat_trainer* t);
+/* Add a training view: camera and its RGB8 image (height x width x 3), kept in HBM as
+ * bytes and converted in the loss kernel (View::to_real, src/dataset.rs:23-25). */
+int splat_trainer_add_view(splat_trainer* t, const splat_camera* cam, const uint8_t* rgb,
+                           uint32_t* index);
+/* One iteration on view `view`: project -> bin -> composite -> loss -> backward ->
+ * statistics -> Adam (src/train.rs:211-247). */
+int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params* p,
+                       splat_step_result* r);
+/* densify_and_prune on the resident model (src/densify.rs:125-221); the normal draws
+ * for split children come from a counter-based generator keyed by `seed`. */
+int splat_trainer_densify(splat_trainer* t, const splat_densify_config* cfg, float extent,
+                          uint64_t seed, splat_densify_report* report);
+/* reset_opacity (src/densify.rs:224-230). */
+int splat_trainer_reset_opacity(splat_trainer* t, float max_alpha);
+/* Forward render of the resident model through view `view`'s camera (render.rs:14). */
+int splat_trainer_render(splat_trainer* t, uint32_t view, int32_t sh_degree, int32_t antialias,
+                         const float background[3], float* image);
+/* Current Gaussian count, Adam step count. */
+int splat_trainer_info(splat_trainer* t, uint64_t* n, uint64_t* adam_step);
+/* Copy the model (and optionally the Adam moments and statistics) to the host. */
+int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* v,
+                           float* grad_accum, uint32_t* denom, uint32_t* max_radii,
+                           uint64_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

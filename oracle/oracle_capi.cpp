@@ -263,6 +263,47 @@ float oracle_powi_f32(float a, int b) { return powi<float>(a, b); }
 
 void oracle_mean_nn_dist2(const double* pts, uint64_t n, double* out) { mean_nn_dist2(pts, n, out); }
 
+// One training iteration (src/train.rs:211-247): project, bin, composite, loss against an
+// RGB8 target (View::to_real), backward, optional statistics, Adam.  In place on params,
+// m, v, the statistics and *step.  Returns the loss; *num_entries receives E.
+float oracle_train_step_f32(float* params, float* m, float* v, uint64_t n, uint64_t* step,
+                            const splat_camera* cam, const uint8_t* rgb, int sh_degree,
+                            int antialias, float lambda, const float bg[3],
+                            const splat_learning_rates* lr, int accumulate_stats, int absgrad,
+                            float* grad_accum, uint32_t* denom, uint32_t* max_radii,
+                            uint64_t* num_entries) {
+  Camera<float> c = camera_from<float>(cam);
+  ProjOpts<float> o;
+  o.near_plane = 0.2f;
+  o.eps2d = 0.3f;
+  o.antialias = antialias != 0;
+  o.sh_degree = sh_degree;
+  const uint32_t w = cam->width, h = cam->height;
+  std::vector<Proj<float>> p(n);
+  for (uint64_t i = 0; i < n; i++) p[i] = project_one<float>(params + i * 59, c, o);
+  Binning b;
+  if (!bin<float>(p.data(), n, w, h, b)) return NAN;
+  *num_entries = b.sorted.size();
+  const size_t npx = (size_t)w * h;
+  std::vector<float> image(npx * 3), final_t(npx), target(npx * 3), dimage(npx * 3);
+  std::vector<uint32_t> nc(npx);
+  rasterize_forward<float>(p.data(), b, w, h, bg, image.data(), final_t.data(), nc.data());
+  for (size_t i = 0; i < npx * 3; i++) target[i] = (float)rgb[i] / 255.0f;
+  const float loss = photometric_loss<float>(image.data(), target.data(), w, h, lambda,
+                                             dimage.data());
+  std::vector<ProjGrad<float>> pg(n);
+  std::vector<float> ab(2 * n), grads(n * 59);
+  rasterize_backward<float>(p.data(), n, b, w, h, bg, final_t.data(), nc.data(), dimage.data(),
+                            pg.data(), ab.data());
+  for (uint64_t i = 0; i < n; i++) project_backward_one<float>(params + i * 59, c, o, pg[i], &grads[i * 59]);
+  if (accumulate_stats)
+    stats_accumulate<float>(p.data(), pg.data(), absgrad ? ab.data() : nullptr, n, w, h,
+                            grad_accum, denom, max_radii);
+  float l[6] = {lr->mean, lr->log_scale, lr->quat, lr->opacity, lr->sh_dc, lr->sh_rest};
+  adam_step<float>(params, grads.data(), m, v, n, step, 0.9f, 0.999f, 1e-15f, l);
+  return loss;
+}
+
 // ---------------------------------------------------------------- f64 --------------
 // Layouts: gaussians 59 doubles each; camera 18 doubles [w, h, fx, fy, cx, cy, rot 9,
 // trans 3]; opts [near, eps2d, antialias, sh_degree]; Proj 15 doubles (proj_to64);

@@ -113,6 +113,15 @@ _SIGS = {
     "splat_densify_and_prune": (i32, [P, P, P, P, P, P, P, u64, P, f32, P, u64, P, P, P, u64,
                                       P, P]),
     "splat_reset_opacity": (i32, [P, P, P, P, u64, f32]),
+    "splat_trainer_create": (i32, [P, P, u64, u64, P]),
+    "splat_trainer_destroy": (i32, [P]),
+    "splat_trainer_add_view": (i32, [P, P, P, P]),
+    "splat_trainer_step": (i32, [P, u32, P, P]),
+    "splat_trainer_densify": (i32, [P, P, f32, u64, P]),
+    "splat_trainer_reset_opacity": (i32, [P, f32]),
+    "splat_trainer_render": (i32, [P, u32, i32, i32, P, P]),
+    "splat_trainer_info": (i32, [P, P, P]),
+    "splat_trainer_download": (i32, [P, P, P, P, P, P, P, u64]),
 }
 
 _lib = None
@@ -147,7 +156,9 @@ class Context:
     """A libsplat context (one HIP stream on one device)."""
 
     def __init__(self, device=0):
+        import weakref
         self._h = C.c_void_p()
+        self._trainers = weakref.WeakSet()
         st = lib().splat_ctx_create(device, C.byref(self._h))
         if st != OK:
             raise SplatError(st, f"splat_ctx_create(device={device}) failed")
@@ -157,6 +168,8 @@ class Context:
         return self._h
 
     def close(self):
+        for t in list(self._trainers):  # trainers run on this context's stream
+            t.close()
         if self._h:
             lib().splat_ctx_destroy(self._h)
             self._h = C.c_void_p()
@@ -354,3 +367,83 @@ def psnr(pred, target):
     if st != OK:
         raise SplatError(st, "psnr")
     return float(out[0])
+
+
+class Trainer:
+    """Device-resident training state (splat_trainer_*): model, Adam moments, statistics
+    and views in HBM; `step` runs one iteration of src/train.rs:211-247 on the GPU."""
+
+    def __init__(self, ctx, gaussians, capacity=0):
+        self.ctx = ctx
+        g = np.ascontiguousarray(gaussians, np.float32)
+        self._h = C.c_void_p()
+        ctx.check(lib().splat_trainer_create(ctx.handle, _p(g), len(g), max(capacity, len(g)),
+                                             C.byref(self._h)), "trainer_create")
+        ctx._trainers.add(self)
+        self.views = []
+
+    def close(self):
+        if self._h:
+            lib().splat_trainer_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_view(self, cam, rgb):
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        assert rgb.shape == (int(cam["height"]), int(cam["width"]), 3)
+        idx = C.c_uint32()
+        self.ctx.check(lib().splat_trainer_add_view(self._h, _p(cam), _p(rgb), C.byref(idx)),
+                       "trainer_add_view")
+        self.views.append(cam)
+        return idx.value
+
+    def step(self, view, params):
+        r = np.zeros((), rec.STEP_RESULT)
+        self.ctx.check(lib().splat_trainer_step(self._h, view, _p(params), _p(r)), "trainer_step")
+        return r
+
+    def densify(self, cfg, extent, seed):
+        rep = np.zeros((), rec.DENSIFY_REPORT)
+        self.ctx.check(lib().splat_trainer_densify(self._h, _p(cfg), extent, seed, _p(rep)),
+                       "trainer_densify")
+        return rep
+
+    def reset_opacity(self, max_alpha=0.01):
+        self.ctx.check(lib().splat_trainer_reset_opacity(self._h, max_alpha),
+                       "trainer_reset_opacity")
+
+    def render(self, view, sh_degree, antialias=False, background=(0.0, 0.0, 0.0)):
+        cam = self.views[view]
+        img = np.zeros((int(cam["height"]), int(cam["width"]), 3), np.float32)
+        bg = np.asarray(background, np.float32)
+        self.ctx.check(lib().splat_trainer_render(self._h, view, sh_degree, int(antialias),
+                                                  _p(bg), _p(img)), "trainer_render")
+        return img
+
+    def info(self):
+        n = C.c_uint64()
+        s = C.c_uint64()
+        self.ctx.check(lib().splat_trainer_info(self._h, C.byref(n), C.byref(s)), "trainer_info")
+        return n.value, s.value
+
+    def download(self, moments=False, stats=False):
+        n, _ = self.info()
+        g = np.zeros((n, rec.NUM_PARAMS), np.float32)
+        m = np.zeros_like(g) if moments else None
+        v = np.zeros_like(g) if moments else None
+        acc = np.zeros(n, np.float32) if stats else None
+        den = np.zeros(n, np.uint32) if stats else None
+        mr = np.zeros(n, np.uint32) if stats else None
+        self.ctx.check(lib().splat_trainer_download(self._h, _p(g), _p(m), _p(v), _p(acc),
+                                                    _p(den), _p(mr), n), "trainer_download")
+        out = {"params": g}
+        if moments:
+            out.update(m=m, v=v)
+        if stats:
+            out.update(grad_accum=acc, denom=den, max_radii=mr)
+        return out

@@ -12,16 +12,6 @@
 
 using namespace splat;
 
-namespace splat {
-size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
-                      size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
-                      const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
-                      const float* normals_dev, size_t normals_len, float* po, float* mo,
-                      float* vo, size_t out_capacity, size_t out_stride,
-                      splat_densify_report* report);
-void reset_opacity_planes(splat_ctx* ctx, float* params, float* m, float* v, size_t n,
-                          size_t stride, float limit);
-}  // namespace splat
 
 namespace {
 
@@ -351,7 +341,8 @@ int splat_rasterize_backward(splat_ctx* ctx, const splat_proj* projs, uint64_t n
     h2d(ctx, dft, final_t, npx * 4);
     h2d(ctx, dnc, n_contrib, npx * 4);
     h2d(ctx, ddi, dimage, npx * 12);
-    raster_backward(ctx, pb, bb, w, h, settings->background, dft, dnc, ddi, ordered != 0, slots);
+    raster_backward(ctx, pb, bb, w, h, settings->background, dft, dnc, ddi, ordered != 0, slots,
+                    true);
     splat_proj_grad* dg = ctx->buf<splat_proj_grad>("pgrad_out", n + 1);
     float* dab = ctx->buf<float>("abs_out", 2 * n + 1);
     reduce_slots(ctx, bb.entry_offset, slots, n, dg, dab);

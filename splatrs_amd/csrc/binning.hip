@@ -31,11 +31,6 @@ __global__ void __launch_bounds__(256) sum_u64_kernel(const uint32_t* __restrict
   if (threadIdx.x == 0) atomicAdd(total, sh[0] + sh[1] + sh[2] + sh[3]);
 }
 
-__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict__ entry_offset,
                                                      uint32_t n, uint32_t e_total,
                                                      const uint2* __restrict__ rect,
@@ -195,10 +190,6 @@ __global__ void __launch_bounds__(1024) sort_big_kernel(const uint32_t* __restri
 }
 
 }  // namespace
-
-static void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
-  launch(ctx, "fill_u32", fill_u32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, p, n, v);
-}
 
 size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uint32_t tiles_y,
                  BinBuffers& bb, size_t* capacity_io, std::map<std::string, DevBuf>& store,

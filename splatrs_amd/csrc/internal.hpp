@@ -226,6 +226,8 @@ void planes_to_aos(splat_ctx* ctx, const float* planes, size_t n, size_t stride,
 void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
                         uint32_t* total_dev);
 size_t scan_temp_bytes(size_t n);
+// Fill kernel (used instead of hipMemset so the same code can sit inside a HIP graph).
+void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v);
 
 // binning.hip
 // Builds the sorted per-tile lists.  Returns E (synchronises once to read it).
@@ -240,7 +242,7 @@ void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
 void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
                      uint32_t width, uint32_t height, const float bg[3], const float* final_t,
                      const uint32_t* n_contrib, const float* dimage, bool ordered,
-                     float* slots);
+                     float* slots, bool need_abs);
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
                   splat_proj_grad* grads, float* abs_grads);
 
@@ -263,6 +265,15 @@ void gaussian_kernel_host(float k[11]);
 void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, float* v,
                  size_t n, size_t stride, float c1, float c2, const splat_adam_state& st,
                  const splat_learning_rates& lr);
+size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
+                      size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
+                      const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
+                      const float* normals_dev, size_t normals_len, float* po, float* mo,
+                      float* vo, size_t out_capacity, size_t out_stride,
+                      splat_densify_report* report);
+void reset_opacity_planes(splat_ctx* ctx, float* params, float* m, float* v, size_t n,
+                          size_t stride, float limit);
+void normals_fill(splat_ctx* ctx, float* out, size_t count, uint64_t seed);
 void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius,
                           const splat_proj_grad* grads, const float* abs_grads, size_t n,
                           uint32_t width, uint32_t height, float* grad_accum, uint32_t* denom,

@@ -186,13 +186,20 @@ __device__ __forceinline__ void pair_terms(PixBack& s, float4 r0, float4 r1, flo
   }
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+// Sum over each 16-lane row of the wave with DPP row shifts (ALU-rate cross-lane moves);
+// lane 16 r + 15 ends with the sum of row r.
+__device__ __forceinline__ float row_sum16(float v) {
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xf, 0xf, false));
+  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xf, 0xf, false));
   return v;
 }
 
-template <bool ORDERED>
+// ORDERED: per-pixel terms staged in LDS and summed sequentially in row-major order.
+// Otherwise: 16-lane row sums by DPP, 16 row partials per entry summed in a fixed order.
+// ABS: also accumulate the AbsGS slots (9, 10); otherwise they are written as zero.
+template <bool ORDERED, bool ABS>
 __global__ void __launch_bounds__(256) raster_bwd_kernel(
     const float4* __restrict__ rec0, const float4* __restrict__ rec1,
     const float4* __restrict__ rec2, const uint32_t* __restrict__ tile_start,
@@ -205,8 +212,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   __shared__ float4 s0[B], s1[B], s2[B];
   __shared__ uint32_t srect[B];
   __shared__ uint32_t smax_last;
-  // ORDERED: per-pixel terms of the batch; otherwise per-wave partial sums.
-  __shared__ float red[ORDERED ? B * kSlots * 256 : B * kSlots * 4];
+  // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums.
+  __shared__ float red[ORDERED ? B * kSlots * 256 : B * kSlots * 16];
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
   if (threadIdx.x == 0) smax_last = 0;
   const uint32_t tile = blockIdx.x;
@@ -274,16 +281,19 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
 #pragma unroll
         for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + threadIdx.x] = v[q];
       } else {
+        constexpr int NS = ABS ? kSlots : 9;
         const bool any = __any(act);
+        const bool row_end = (lane & 15) == 15;
+        const int part = wave * 4 + (lane >> 4);
         if (any) {
 #pragma unroll
-          for (int q = 0; q < kSlots; q++) {
-            const float r = wave_sum(v[q]);
-            if (lane == 0) red[(kk * kSlots + q) * 4 + wave] = r;
+          for (int q = 0; q < NS; q++) {
+            const float r = row_sum16(v[q]);
+            if (row_end) red[(kk * kSlots + q) * 16 + part] = r;
           }
-        } else if (lane == 0) {
+        } else if (row_end) {
 #pragma unroll
-          for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 4 + wave] = 0.0f;
+          for (int q = 0; q < NS; q++) red[(kk * kSlots + q) * 16 + part] = 0.0f;
         }
       }
     }
@@ -295,9 +305,10 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       if (ORDERED) {
         const float* p = &red[(kk * kSlots + slot) * 256];
         for (int t = 0; t < 256; t++) acc += p[t];
-      } else {
-        const float* p = &red[(kk * kSlots + slot) * 4];
-        acc = ((p[0] + p[1]) + p[2]) + p[3];
+      } else if (ABS || slot < 9) {
+        const float* p = &red[(kk * kSlots + slot) * 16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) acc += p[t];
       }
       slots[(size_t)sorted_entry[start + lo + kk] * kSlots + slot] = acc;
     }
@@ -352,20 +363,21 @@ void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
 
 void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, uint32_t width,
                      uint32_t height, const float bg[3], const float* final_t,
-                     const uint32_t* n_contrib, const float* dimage, bool ordered, float* slots) {
+                     const uint32_t* n_contrib, const float* dimage, bool ordered, float* slots,
+                     bool need_abs) {
   const unsigned T = bb.tiles_x * bb.tiles_y;
+  auto go = [&](const char* name, auto kernel) {
+    launch(ctx, name, kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec0,
+           (const float4*)pb.rec1, (const float4*)pb.rec2, (const uint32_t*)bb.tile_start,
+           (const uint32_t*)bb.sorted_gauss, (const uint32_t*)bb.sorted_entry, bb.tiles_x, width,
+           height, bg[0], bg[1], bg[2], final_t, n_contrib, dimage, slots);
+  };
   if (ordered)
-    launch(ctx, "raster_bwd_ordered", raster_bwd_kernel<true>, dim3(T), dim3(256), 0,
-           (const float4*)pb.rec0, (const float4*)pb.rec1, (const float4*)pb.rec2,
-           (const uint32_t*)bb.tile_start, (const uint32_t*)bb.sorted_gauss,
-           (const uint32_t*)bb.sorted_entry, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
-           final_t, n_contrib, dimage, slots);
+    go("raster_bwd_ordered", raster_bwd_kernel<true, true>);
+  else if (need_abs)
+    go("raster_bwd", raster_bwd_kernel<false, true>);
   else
-    launch(ctx, "raster_bwd", raster_bwd_kernel<false>, dim3(T), dim3(256), 0,
-           (const float4*)pb.rec0, (const float4*)pb.rec1, (const float4*)pb.rec2,
-           (const uint32_t*)bb.tile_start, (const uint32_t*)bb.sorted_gauss,
-           (const uint32_t*)bb.sorted_entry, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
-           final_t, n_contrib, dimage, slots);
+    go("raster_bwd", raster_bwd_kernel<false, false>);
 }
 
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,

@@ -100,7 +100,16 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const uint32_t
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
 }
 
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 }  // namespace
+
+void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
+  launch(ctx, "fill", fill_u32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, p, n, v);
+}
 
 size_t scan_temp_bytes(size_t n) {
   return (blocks_for(n == 0 ? 1 : n, kScanTile) + 1) * sizeof(uint32_t);

@@ -1,0 +1,382 @@
+// Device-resident training (src/train.rs:196-303): the model, its gradient, both Adam
+// moments, the densification statistics and every view's image stay in HBM; a step
+// launches project -> bin -> composite -> loss -> backward -> statistics -> Adam on the
+// context stream.  The only host synchronisations are reading the tile-entry count (to
+// size the entry arrays) and, on request, the loss value.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "internal.hpp"
+
+using namespace splat;
+
+struct TrainerView {
+  splat_camera cam;
+  DevBuf rgb;
+};
+
+struct splat_trainer {
+  splat_ctx* ctx = nullptr;
+  int device = 0;
+  size_t n = 0, cap = 0, stride = 0;
+  uint64_t adam_step = 0;
+  DevBuf params, m, v, grads;           // planes [59][stride]
+  DevBuf grad_accum, denom, max_radii;  // statistics, capacity cap
+  std::vector<std::unique_ptr<TrainerView>> views;
+  std::map<std::string, DevBuf> store;  // per-view buffers (projection, binning, images)
+  size_t entry_cap = 0;
+};
+
+namespace {
+
+template <class F>
+int tguard(splat_trainer* t, F&& f) {
+  if (!t || !t->ctx) return SPLAT_ERR_INVALID;
+  splat_ctx* ctx = t->ctx;
+  try {
+    SPLAT_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return SPLAT_OK;
+  } catch (const Error& e) {
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "host allocation failed";
+    return SPLAT_ERR_OOM;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return SPLAT_ERR_INVALID;
+  }
+}
+
+void alloc_model(splat_trainer* t, size_t cap) {
+  t->cap = cap;
+  t->stride = plane_stride(cap);
+  const size_t bytes = t->stride * kParams * 4;
+  t->params.reserve(bytes);
+  t->m.reserve(bytes);
+  t->v.reserve(bytes);
+  t->grads.reserve(bytes);
+  t->grad_accum.reserve((cap + 1) * 4);
+  t->denom.reserve((cap + 1) * 4);
+  t->max_radii.reserve((cap + 1) * 4);
+}
+
+void zero_stats(splat_trainer* t) {
+  splat_ctx* ctx = t->ctx;
+  fill_u32(ctx, t->grad_accum.as<uint32_t>(), t->n, 0u);  // +0.0f has all bits clear
+  fill_u32(ctx, t->denom.as<uint32_t>(), t->n, 0u);
+  fill_u32(ctx, t->max_radii.as<uint32_t>(), t->n, 0u);
+}
+
+ProjBuffers proj_buffers(splat_trainer* t) {
+  ProjBuffers pb;
+  const size_t c = t->cap + 1;
+  pb.rec0 = (float4*)t->store["rec0"].reserve(c * 16);
+  pb.rec1 = (float4*)t->store["rec1"].reserve(c * 16);
+  pb.rec2 = (float4*)t->store["rec2"].reserve(c * 16);
+  pb.depth = (float*)t->store["depth"].reserve(c * 4);
+  pb.radius = (uint32_t*)t->store["radius"].reserve(c * 4);
+  pb.rect = (uint2*)t->store["rect"].reserve(c * 8);
+  pb.ntiles = (uint32_t*)t->store["ntiles"].reserve(c * 4);
+  return pb;
+}
+
+// Counter-based standard normals: SplitMix64 of (seed, index) gives two uniform doubles,
+// Box-Muller as the reference's normal_sample (src/densify.rs:116-121, u1 kept >= 1e-12).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+__global__ void normals_kernel(float* out, size_t count, uint64_t seed) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t a = splitmix64(seed ^ splitmix64(2 * i));
+  const uint64_t b = splitmix64(seed ^ splitmix64(2 * i + 1));
+  double u1 = (double)(a >> 11) * 0x1.0p-53;
+  const double u2 = (double)(b >> 11) * 0x1.0p-53;
+  u1 = u1 > 1e-12 ? u1 : 1e-12;
+  out[i] = (float)(sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2));
+}
+
+__global__ void __launch_bounds__(256) count_visible_kernel(const uint32_t* __restrict__ radius,
+                                                            uint32_t n, uint32_t* count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool vis = i < n && radius[i] > 0;
+  const uint64_t mask = __ballot(vis);
+  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(count, (uint32_t)__popcll(mask));
+}
+
+}  // namespace
+
+namespace splat {
+void normals_fill(splat_ctx* ctx, float* out, size_t count, uint64_t seed) {
+  launch(ctx, "normals", normals_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, out, count,
+         seed);
+}
+}  // namespace splat
+
+extern "C" {
+
+int splat_trainer_create(splat_ctx* ctx, const float* gaussians, uint64_t n, uint64_t capacity,
+                         splat_trainer** out) {
+  if (!ctx || !out) return SPLAT_ERR_INVALID;
+  *out = nullptr;
+  splat_trainer* t = new (std::nothrow) splat_trainer();
+  if (!t) return SPLAT_ERR_OOM;
+  t->ctx = ctx;
+  t->device = ctx->device;
+  const int st = tguard(t, [&] {
+    SPLAT_REQUIRE(n == 0 || gaussians, "null argument");
+    t->n = n;
+    alloc_model(t, std::max<size_t>(std::max<size_t>(capacity, n), 64));
+    if (n) {
+      float* staging = (float*)t->store["aos"].reserve(n * kParams * 4);
+      SPLAT_HIP(hipMemcpyAsync(staging, gaussians, n * kParams * 4, hipMemcpyHostToDevice,
+                               ctx->stream));
+      aos_to_planes(ctx, staging, n, t->params.as<float>(), t->stride);
+    }
+    fill_u32(ctx, t->m.as<uint32_t>(), t->stride * kParams, 0u);
+    fill_u32(ctx, t->v.as<uint32_t>(), t->stride * kParams, 0u);
+    zero_stats(t);
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    t->store.erase("aos");
+  });
+  if (st != SPLAT_OK) {
+    delete t;
+    return st;
+  }
+  *out = t;
+  return SPLAT_OK;
+}
+
+int splat_trainer_destroy(splat_trainer* t) {
+  if (!t) return SPLAT_OK;
+  // The context may already be gone: use only the device id captured at creation.
+  (void)hipSetDevice(t->device);
+  (void)hipDeviceSynchronize();
+  delete t;
+  return SPLAT_OK;
+}
+
+int splat_trainer_add_view(splat_trainer* t, const splat_camera* cam, const uint8_t* rgb,
+                           uint32_t* index) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(cam && rgb, "null argument");
+    SPLAT_REQUIRE(cam->width > 0 && cam->height > 0 && cam->width <= 65535 * kTile &&
+                      cam->height <= 65535 * kTile,
+                  "bad image size");
+    auto v = std::make_unique<TrainerView>();
+    v->cam = *cam;
+    const size_t bytes = (size_t)cam->width * cam->height * 3;
+    v->rgb.reserve(bytes);
+    SPLAT_HIP(hipMemcpyAsync(v->rgb.ptr, rgb, bytes, hipMemcpyHostToDevice, t->ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(t->ctx->stream));
+    t->views.push_back(std::move(v));
+    if (index) *index = (uint32_t)(t->views.size() - 1);
+  });
+}
+
+int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params* p,
+                       splat_step_result* r) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(p && view < t->views.size(), "bad view index");
+    SPLAT_REQUIRE(p->sh_degree >= 0 && p->sh_degree <= 3, "sh_degree must be 0..3");
+    splat_ctx* ctx = t->ctx;
+    const TrainerView& tv = *t->views[view];
+    const uint32_t W = tv.cam.width, H = tv.cam.height;
+    const size_t npx = (size_t)W * H;
+    const uint32_t tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile;
+    const size_t n = t->n;
+    splat_proj_opts opts;
+    opts.near_plane = p->near_plane;
+    opts.eps2d = 0.3f;
+    opts.antialias = p->antialias;
+    opts.sh_degree = p->sh_degree;
+
+    // forward (src/train.rs:211-221)
+    ProjBuffers pb = proj_buffers(t);
+    project_forward(ctx, t->params.as<float>(), n, t->stride, tv.cam, opts, pb);
+    BinBuffers bb{};
+    const size_t E = bin_build(ctx, pb, n, tx, ty, bb, &t->entry_cap, t->store, nullptr);
+    float* image = (float*)t->store["image"].reserve(npx * 12 + 4);
+    float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
+    uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
+    raster_forward(ctx, pb, bb, W, H, p->background, image, final_t, n_contrib);
+
+    // loss (src/train.rs:221-222)
+    float* dimage = (float*)t->store["dimage"].reserve(npx * 12 + 4);
+    float* scratch = (float*)t->store["loss_scratch"].reserve(loss_scratch_floats(W, H) * 4);
+    double parts[2] = {0, 0};
+    loss_forward_backward(ctx, image, nullptr, tv.rgb.as<uint8_t>(), W, H, p->lambda_dssim, true,
+                          dimage, scratch, p->want_loss ? parts : nullptr);
+
+    // backward (src/train.rs:225-235)
+    float* slots = (float*)t->store["slots"].reserve((E + 1) * kSlots * 4);
+    raster_backward(ctx, pb, bb, W, H, p->background, final_t, n_contrib, dimage,
+                    p->ordered_backward != 0, slots, p->accumulate_stats && p->absgrad);
+    splat_proj_grad* pgrad =
+        (splat_proj_grad*)t->store["pgrad"].reserve((t->cap + 1) * sizeof(splat_proj_grad));
+    float* absg = (float*)t->store["absg"].reserve((t->cap + 1) * 8);
+    reduce_slots(ctx, bb.entry_offset, slots, n, pgrad, absg);
+    project_backward(ctx, t->params.as<float>(), n, t->stride, tv.cam, opts, pgrad,
+                     t->grads.as<float>());
+    if (p->accumulate_stats)
+      stats_accumulate_dev(ctx, pb.radius, pgrad, p->absgrad ? absg : nullptr, n, W, H,
+                           t->grad_accum.as<float>(), t->denom.as<uint32_t>(),
+                           t->max_radii.as<uint32_t>());
+
+    // optimiser (src/train.rs:238-247, src/adam.rs:77-90)
+    t->adam_step += 1;
+    splat_adam_state st;
+    st.beta1 = 0.9f;
+    st.beta2 = 0.999f;
+    st.eps = 1e-15f;
+    st.step = t->adam_step;
+    const float c1 = 1.0f - powi(st.beta1, (int32_t)st.step);
+    const float c2 = 1.0f - powi(st.beta2, (int32_t)st.step);
+    adam_planes(ctx, t->params.as<float>(), t->grads.as<float>(), t->m.as<float>(),
+                t->v.as<float>(), n, t->stride, c1, c2, st, p->lr);
+
+    if (r) {
+      r->num_entries = E;
+      r->num_visible = 0;
+      r->loss = 0.0f;
+      if (p->want_loss) {
+        uint32_t* cnt = (uint32_t*)t->store["visible_count"].reserve(4);
+        fill_u32(ctx, cnt, 1, 0u);
+        launch(ctx, "count_visible", count_visible_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+               (const uint32_t*)pb.radius, (uint32_t)n, cnt);
+        uint32_t vis = 0;
+        SPLAT_HIP(hipMemcpyAsync(&vis, cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+        r->num_visible = vis;
+        const float inv_n = 1.0f / (float)(npx * 3);
+        const float l1 = (float)parts[1] * inv_n;
+        const float ssim = (float)parts[0] / (float)(npx * 3);
+        r->loss = (1.0f - p->lambda_dssim) * l1 + p->lambda_dssim * (1.0f - ssim);
+      }
+    }
+  });
+}
+
+int splat_trainer_densify(splat_trainer* t, const splat_densify_config* cfg, float extent,
+                          uint64_t seed, splat_densify_report* report) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(cfg && report, "null argument");
+    splat_ctx* ctx = t->ctx;
+    const size_t n = t->n;
+    *report = splat_densify_report{0, 0, 0};
+    if (n == 0) return;
+    const size_t children = cfg->split_children;
+    const size_t nnorm = 3 * children * n;
+    float* normals = (float*)t->store["normals"].reserve((nnorm + 1) * 4);
+    normals_fill(ctx, normals, nnorm, seed);
+    const size_t cap2 = std::max(t->cap, n * (1 + std::max<size_t>(children, 1)));
+    const size_t stride2 = plane_stride(cap2);
+    DevBuf po, mo, vo;
+    po.reserve(stride2 * kParams * 4);
+    mo.reserve(stride2 * kParams * 4);
+    vo.reserve(stride2 * kParams * 4);
+    const size_t k = densify_planes(ctx, t->params.as<float>(), t->m.as<float>(),
+                                    t->v.as<float>(), n, t->stride, t->grad_accum.as<float>(),
+                                    t->denom.as<uint32_t>(), t->max_radii.as<uint32_t>(), *cfg,
+                                    extent, normals, nnorm, po.as<float>(), mo.as<float>(),
+                                    vo.as<float>(), cap2, stride2, report);
+    SPLAT_REQUIRE(k != (size_t)-1 && k <= cap2, "densify output exceeds its buffer");
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    std::swap(t->params.ptr, po.ptr);
+    std::swap(t->params.bytes, po.bytes);
+    std::swap(t->m.ptr, mo.ptr);
+    std::swap(t->m.bytes, mo.bytes);
+    std::swap(t->v.ptr, vo.ptr);
+    std::swap(t->v.bytes, vo.bytes);
+    t->cap = cap2;
+    t->stride = stride2;
+    t->grads.reserve(stride2 * kParams * 4);
+    t->grad_accum.reserve((cap2 + 1) * 4);
+    t->denom.reserve((cap2 + 1) * 4);
+    t->max_radii.reserve((cap2 + 1) * 4);
+    t->n = k;
+    zero_stats(t);  // src/densify.rs:219
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int splat_trainer_reset_opacity(splat_trainer* t, float max_alpha) {
+  return tguard(t, [&] {
+    const float limit = logf(max_alpha / (1.0f - max_alpha));  // inverse_sigmoid
+    reset_opacity_planes(t->ctx, t->params.as<float>(), t->m.as<float>(), t->v.as<float>(), t->n,
+                         t->stride, limit);
+  });
+}
+
+int splat_trainer_render(splat_trainer* t, uint32_t view, int32_t sh_degree, int32_t antialias,
+                         const float background[3], float* image) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(view < t->views.size() && background && image, "bad argument");
+    SPLAT_REQUIRE(sh_degree >= 0 && sh_degree <= 3, "sh_degree must be 0..3");
+    splat_ctx* ctx = t->ctx;
+    const splat_camera& cam = t->views[view]->cam;
+    const uint32_t W = cam.width, H = cam.height;
+    const size_t npx = (size_t)W * H;
+    splat_proj_opts opts{0.2f, 0.3f, antialias, sh_degree};
+    ProjBuffers pb = proj_buffers(t);
+    project_forward(ctx, t->params.as<float>(), t->n, t->stride, cam, opts, pb);
+    BinBuffers bb{};
+    bin_build(ctx, pb, t->n, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile, bb, &t->entry_cap,
+              t->store, nullptr);
+    float* img = (float*)t->store["image"].reserve(npx * 12 + 4);
+    float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
+    uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
+    raster_forward(ctx, pb, bb, W, H, background, img, final_t, n_contrib);
+    SPLAT_HIP(hipMemcpyAsync(image, img, npx * 12, hipMemcpyDeviceToHost, ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int splat_trainer_info(splat_trainer* t, uint64_t* n, uint64_t* adam_step) {
+  if (!t) return SPLAT_ERR_INVALID;
+  if (n) *n = t->n;
+  if (adam_step) *adam_step = t->adam_step;
+  return SPLAT_OK;
+}
+
+int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* v,
+                           float* grad_accum, uint32_t* denom, uint32_t* max_radii,
+                           uint64_t capacity) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(capacity >= t->n, "output capacity too small");
+    splat_ctx* ctx = t->ctx;
+    const size_t n = t->n;
+    if (n == 0) return;
+    float* staging = (float*)t->store["aos_out"].reserve(n * kParams * 4);
+    const std::pair<float*, const DevBuf*> arrays[3] = {{gaussians, &t->params}, {m, &t->m},
+                                                        {v, &t->v}};
+    for (auto& a : arrays) {
+      if (!a.first) continue;
+      planes_to_aos(ctx, a.second->as<float>(), n, t->stride, staging);
+      SPLAT_HIP(hipMemcpyAsync(a.first, staging, n * kParams * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+      SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    if (grad_accum)
+      SPLAT_HIP(hipMemcpyAsync(grad_accum, t->grad_accum.ptr, n * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    if (denom)
+      SPLAT_HIP(hipMemcpyAsync(denom, t->denom.ptr, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (max_radii)
+      SPLAT_HIP(hipMemcpyAsync(max_radii, t->max_radii.ptr, n * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    t->store.erase("aos_out");
+  });
+}
+
+}  // extern "C"

@@ -36,7 +36,17 @@ DENSIFY_CONFIG = np.dtype([
 ])
 DENSIFY_REPORT = np.dtype([("cloned", "<u8"), ("split", "<u8"), ("pruned", "<u8")])
 
+STEP_PARAMS = np.dtype([
+    ("sh_degree", "<i4"), ("accumulate_stats", "<i4"), ("absgrad", "<i4"),
+    ("ordered_backward", "<i4"), ("want_loss", "<i4"), ("antialias", "<i4"),
+    ("near_plane", "<f4"), ("lambda_dssim", "<f4"), ("background", "<f4", (3,)),
+    ("lr", LEARNING_RATES),
+])
+STEP_RESULT = np.dtype([("loss", "<f4"), ("pad_", "<u4"), ("num_entries", "<u8"),
+                        ("num_visible", "<u8")])
+
 assert CAMERA.itemsize == 72 and PROJ.itemsize == 60 and PROJ_GRAD.itemsize == 36
+assert STEP_PARAMS.itemsize == 68 and STEP_RESULT.itemsize == 24
 assert ADAM_STATE.itemsize == 24 and DENSIFY_CONFIG.itemsize == 32
 
 
@@ -69,6 +79,33 @@ def learning_rates(mean, log_scale=0.005, quat=0.001, opacity=0.025, sh_dc=0.002
     lr["sh_dc"] = sh_dc
     lr["sh_rest"] = np.float32(sh_dc) / np.float32(20.0) if sh_rest is None else sh_rest
     return lr
+
+
+def step_params(sh_degree, lr, accumulate_stats=False, absgrad=False, ordered_backward=False,
+                want_loss=False, antialias=False, near_plane=0.2, lambda_dssim=0.2,
+                background=(0.0, 0.0, 0.0)):
+    p = np.zeros((), STEP_PARAMS)
+    p["sh_degree"] = sh_degree
+    p["accumulate_stats"] = int(bool(accumulate_stats))
+    p["absgrad"] = int(bool(absgrad))
+    p["ordered_backward"] = int(bool(ordered_backward))
+    p["want_loss"] = int(bool(want_loss))
+    p["antialias"] = int(bool(antialias))
+    p["near_plane"] = near_plane
+    p["lambda_dssim"] = lambda_dssim
+    p["background"] = np.asarray(background, np.float32)
+    p["lr"] = lr
+    return p
+
+
+def position_lr(iteration, extent, init=0.00016, final=0.0000016, max_steps=30000):
+    """TrainConfig::position_lr, src/train.rs:132-137, in binary32."""
+    f = np.float32
+    t = f(f(iteration) / f(max(max_steps, 1)))
+    t = min(max(t, f(0.0)), f(1.0))
+    a = f(np.log(f(f(init) * f(extent))))
+    b = f(np.log(f(f(final) * f(extent))))
+    return f(np.exp(f(f(a * f(f(1.0) - t)) + f(b * t))))
 
 
 def adam_state(step=0, beta1=0.9, beta2=0.999, eps=1e-15):

@@ -55,6 +55,8 @@ def lib():
             "oracle_ssim_with_grad_f64": (f64, [P, P, u32, u32, P]),
             "oracle_photometric_loss_f64": (f64, [P, P, u32, u32, f64, P]),
             "oracle_chain_loss_f64": (f64, [P, u64, P, P, P, P, f64, P]),
+            "oracle_train_step_f32": (f32, [P, P, P, u64, P, P, P, i32, i32, f32, P, P, i32,
+                                            i32, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -199,6 +201,33 @@ def densify_and_prune(params, m, v, grad_accum, denom, max_radii, cfg, extent, n
         ptr(n_out), ptr(report)), "densify")
     k = int(n_out[0])
     return po[:k].copy(), mo[:k].copy(), vo[:k].copy(), report
+
+
+class TrainState:
+    """Model, Adam moments and statistics of the oracle trainer (AoS, host)."""
+
+    def __init__(self, gaussians):
+        self.params = np.ascontiguousarray(gaussians, np.float32).copy()
+        self.m = np.zeros_like(self.params)
+        self.v = np.zeros_like(self.params)
+        n = len(self.params)
+        self.grad_accum = np.zeros(n, np.float32)
+        self.denom = np.zeros(n, np.uint32)
+        self.max_radii = np.zeros(n, np.uint32)
+        self.step = np.zeros(1, np.uint64)
+
+    def train_step(self, cam, rgb, sh_degree, lr, antialias=False, lam=0.2, bg=(0, 0, 0),
+                   accumulate_stats=False, absgrad=False):
+        """One iteration of src/train.rs:211-247; returns (loss, num_entries)."""
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        bgv = np.asarray(bg, np.float32)
+        ne = np.zeros(1, np.uint64)
+        loss = lib().oracle_train_step_f32(
+            ptr(self.params), ptr(self.m), ptr(self.v), len(self.params), ptr(self.step),
+            ptr(cam), ptr(rgb), sh_degree, int(antialias), lam, ptr(bgv), ptr(lr),
+            int(accumulate_stats), int(absgrad), ptr(self.grad_accum), ptr(self.denom),
+            ptr(self.max_radii), ptr(ne))
+        return float(loss), int(ne[0])
 
 
 def reset_opacity(params, m, v, max_alpha):
