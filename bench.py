@@ -57,7 +57,20 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="oracle train steps timed for the CPU baseline (0 = skip)")
     ap.add_argument("--gaussians", type=int, default=1_000_000)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (reference kernel of "
+                         "known traffic for the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, device="cuda:0", dtype=torch.float32)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 4, "copies": 5}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the sharded design is described in DESIGN.md)", file=sys.stderr)
@@ -128,9 +141,18 @@ def main():
     b = algorithmic_bytes(roof_name, N, V, E, P)
     mean_s = kernels[roof_name]["mean_ms"] * 1e-3
     achieved = b / mean_s / 1e9
+    # HBM bytes per launch measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE passes on this
+    # workload (profiles/), when they cover this kernel.
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic_1M_1080p.json")
+    if args.gaussians == 1_000_000 and os.path.exists(tpath):
+        t = json.load(open(tpath))["kernels"].get(roof_name)
+        if t:
+            traffic = int(t["read_bytes"] + t["write_bytes"])
     roofline = {"kernel": roof_name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "algorithmic_bytes": int(b), "mean_ms": kernels[roof_name]["mean_ms"]}
+                "traffic": traffic, "algorithmic_bytes": int(b),
+                "mean_ms": kernels[roof_name]["mean_ms"]}
 
     cpu = None
     if args.cpu_steps > 0:
