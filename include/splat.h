@@ -260,6 +260,35 @@ int splat_densify_and_prune(splat_ctx* ctx, const float* params, const float* m,
 int splat_reset_opacity(splat_ctx* ctx, float* params, float* m, float* v, uint64_t n,
                         float max_alpha);
 
+/* ---- initialisation from points ----------------------------------------------------- */
+
+/* mean_nn_dist2, src/model.rs:128-186: for n points (xyz binary32 triples) the mean
+ * squared distance (binary64) to the three nearest other points; coincident points count
+ * as zero-distance neighbours, fewer than three others use what exists, a lone point 0. */
+int splat_mean_nn_dist2(splat_ctx* ctx, const float* xyz, uint64_t n, double* out);
+
+/* Model::from_points, src/model.rs:77-104: n records (59 floats) with log-scale
+ * ln(sqrt(max(d2, 1e-7))) on all axes, identity quaternion, opacity logit(0.1) and the
+ * RGB8 colour in the DC coefficients (the model's SH degree starts at 0). */
+int splat_from_points(splat_ctx* ctx, const float* xyz, const uint8_t* rgb, uint64_t n,
+                      float* gaussians);
+
+/* ---- PLY codec (host only, no context) ------------------------------------------- */
+
+/* ply::write, src/ply.rs:19-69: the 3DGS binary little-endian float32 layout.  Writes
+ * n records of SPLAT_NUM_PARAMS floats; SH coefficients above max_sh_degree (clamped to
+ * 3) are not written.  On failure returns SPLAT_ERR_IO and copies a message into err. */
+int splat_ply_write(const char* path, const float* gaussians, uint64_t n, int32_t max_sh_degree,
+                    char* err, size_t err_len);
+
+/* ply::read, src/ply.rs:73-181.  Property order is taken from the header.  *n and
+ * *sh_degree receive the vertex count and the degree implied by the f_rest count; the
+ * records are written only if capacity >= *n (otherwise SPLAT_ERR_CAPACITY), so a call
+ * with gaussians = NULL and capacity = 0 sizes the output.  Malformed files give
+ * SPLAT_ERR_FORMAT, unreadable ones SPLAT_ERR_IO, with the message in err. */
+int splat_ply_read(const char* path, float* gaussians, uint64_t capacity, uint64_t* n,
+                   int32_t* sh_degree, char* err, size_t err_len);
+
 /* ---- trainer: device-resident training (src/train.rs) ---------------------------- */
 
 typedef struct splat_trainer splat_trainer;
@@ -308,6 +337,57 @@ int splat_trainer_reset_opacity(splat_trainer* t, float max_alpha);
 /* Forward render of the resident model through view `view`'s camera (render.rs:14). */
 int splat_trainer_render(splat_trainer* t, uint32_t view, int32_t sh_degree, int32_t antialias,
                          const float background[3], float* image);
+/* TrainConfig, src/train.rs:20-109 (defaults from splat_train_config_default). */
+typedef struct {
+  uint32_t iterations;
+  float position_lr_init, position_lr_final;
+  uint32_t position_lr_max_steps;
+  float feature_lr, opacity_lr, scaling_lr, rotation_lr;
+  float lambda_dssim;
+  int32_t densify;
+  uint32_t densify_from_iter, densify_until_iter, densification_interval;
+  uint32_t opacity_reset_interval;
+  float densify_grad_threshold, percent_dense, min_opacity;
+  int32_t has_max_screen_size;
+  uint32_t max_screen_size;
+  int32_t absgrad, antialias, white_background;
+  int32_t sh_degree;
+  uint32_t sh_degree_interval;
+  float near_plane;
+  uint64_t seed;
+  const uint32_t* save_iterations; /* may be NULL */
+  uint32_t num_save_iterations;
+  uint32_t log_interval;
+  int32_t ordered_backward; /* 1: reference summation order in the backward pass */
+} splat_train_config;
+
+/* Progress, src/train.rs:140-157. */
+typedef struct {
+  uint32_t iteration;
+  float loss, ema_loss, psnr;
+  uint64_t num_gaussians;
+  double ms_per_iter;
+  uint64_t cloned, split, pruned;
+} splat_progress;
+
+typedef void (*splat_progress_fn)(const splat_progress* p, void* user);
+/* Return 0 to continue; non-zero aborts training with SPLAT_ERR_IO (on_save's Err). */
+typedef int (*splat_save_fn)(uint32_t iteration, splat_trainer* t, void* user);
+
+void splat_train_config_default(splat_train_config* cfg);
+
+/* train, src/train.rs:161-306, on the trainer's views (all of them are training views).
+ * The view order is a per-epoch Fisher-Yates shuffle drawn from SplitMix64(seed) and
+ * popped from the back; split-child normals are drawn as in splat_trainer_densify with a
+ * seed derived from (seed, iteration).  Returns SPLAT_ERR_INVALID for no views, zero
+ * iterations or sh_degree > 3 (before any save). */
+int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
+                splat_progress_fn on_progress, splat_save_fn on_save, void* user);
+
+/* The view index splat_train uses at (1-based) iteration `it` for `num_views` views and
+ * `seed` (the order is independent of everything else). */
+uint32_t splat_train_view_at(uint64_t seed, uint32_t num_views, uint32_t it);
+
 /* Current Gaussian count, Adam step count. */
 int splat_trainer_info(splat_trainer* t, uint64_t* n, uint64_t* adam_step);
 /* Copy the model (and optionally the Adam moments and statistics) to the host. */

@@ -86,6 +86,46 @@ def binning_struct(b):
     return s
 
 
+class TrainConfig(C.Structure):
+    """splat_train_config (TrainConfig, src/train.rs:20-109)."""
+    _fields_ = [("iterations", u32), ("position_lr_init", f32), ("position_lr_final", f32),
+                ("position_lr_max_steps", u32), ("feature_lr", f32), ("opacity_lr", f32),
+                ("scaling_lr", f32), ("rotation_lr", f32), ("lambda_dssim", f32),
+                ("densify", i32), ("densify_from_iter", u32), ("densify_until_iter", u32),
+                ("densification_interval", u32), ("opacity_reset_interval", u32),
+                ("densify_grad_threshold", f32), ("percent_dense", f32), ("min_opacity", f32),
+                ("has_max_screen_size", i32), ("max_screen_size", u32), ("absgrad", i32),
+                ("antialias", i32), ("white_background", i32), ("sh_degree", i32),
+                ("sh_degree_interval", u32), ("near_plane", f32), ("seed", u64),
+                ("save_iterations", P), ("num_save_iterations", u32), ("log_interval", u32),
+                ("ordered_backward", i32)]
+
+    @classmethod
+    def default(cls, **overrides):
+        c = cls()
+        lib().splat_train_config_default(C.byref(c))
+        for k, v in overrides.items():
+            if k == "save_iterations":
+                arr = (u32 * len(v))(*v)
+                c._save = arr  # keep alive
+                c.save_iterations = C.cast(arr, P)
+                c.num_save_iterations = len(v)
+            else:
+                setattr(c, k, v)
+        return c
+
+
+class Progress(C.Structure):
+    """splat_progress (Progress, src/train.rs:140-157)."""
+    _fields_ = [("iteration", u32), ("loss", f32), ("ema_loss", f32), ("psnr", f32),
+                ("num_gaussians", u64), ("ms_per_iter", f64), ("cloned", u64), ("split", u64),
+                ("pruned", u64)]
+
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.POINTER(Progress), P)
+SAVE_FN = C.CFUNCTYPE(C.c_int, u32, P, P)
+
+
 # ---- library loading -----------------------------------------------------------------
 
 _SIGS = {
@@ -113,6 +153,13 @@ _SIGS = {
     "splat_densify_and_prune": (i32, [P, P, P, P, P, P, P, u64, P, f32, P, u64, P, P, P, u64,
                                       P, P]),
     "splat_reset_opacity": (i32, [P, P, P, P, u64, f32]),
+    "splat_mean_nn_dist2": (i32, [P, P, u64, P]),
+    "splat_from_points": (i32, [P, P, P, u64, P]),
+    "splat_train_config_default": (None, [P]),
+    "splat_train": (i32, [P, f32, P, P, P, P]),
+    "splat_train_view_at": (u32, [u64, u32, u32]),
+    "splat_ply_write": (i32, [C.c_char_p, P, u64, i32, C.c_char_p, C.c_size_t]),
+    "splat_ply_read": (i32, [C.c_char_p, P, u64, P, P, C.c_char_p, C.c_size_t]),
     "splat_trainer_create": (i32, [P, P, u64, u64, P]),
     "splat_trainer_destroy": (i32, [P]),
     "splat_trainer_add_view": (i32, [P, P, P, P]),
@@ -342,10 +389,55 @@ class Context:
         k = int(n_out[0])
         return po[:k].copy(), mo[:k].copy(), vo[:k].copy(), report
 
+    def mean_nn_dist2(self, xyz):
+        """mean_nn_dist2, src/model.rs:128-186 (binary64)."""
+        x = np.ascontiguousarray(xyz, np.float32)
+        out = np.zeros(len(x), np.float64)
+        self.check(lib().splat_mean_nn_dist2(self._h, _p(x), len(x), _p(out)), "mean_nn_dist2")
+        return out
+
+    def from_points(self, xyz, rgb):
+        """Model::from_points, src/model.rs:77-104 -> n x 59 records."""
+        x = np.ascontiguousarray(xyz, np.float32)
+        c = np.ascontiguousarray(rgb, np.uint8)
+        out = np.zeros((len(x), rec.NUM_PARAMS), np.float32)
+        self.check(lib().splat_from_points(self._h, _p(x), _p(c), len(x), _p(out)), "from_points")
+        return out
+
     def reset_opacity(self, params, m, v, max_alpha):
         """reset_opacity, src/densify.rs:224-230 (+ Adam::reset_opacity_state)."""
         self.check(lib().splat_reset_opacity(self._h, _p(params), _p(m), _p(v), len(params),
                                              max_alpha), "reset_opacity")
+
+
+class PlyError(SplatError):
+    pass
+
+
+def ply_write(path, gaussians, max_sh_degree):
+    """ply::write, src/ply.rs:19-69."""
+    g = np.ascontiguousarray(gaussians, np.float32)
+    err = C.create_string_buffer(512)
+    st = lib().splat_ply_write(os.fsencode(path), _p(g), len(g), max_sh_degree, err, len(err))
+    if st != OK:
+        raise PlyError(st, err.value.decode())
+
+
+def ply_read(path):
+    """ply::read, src/ply.rs:73-181 -> (gaussians n x 59, sh_degree)."""
+    err = C.create_string_buffer(512)
+    n = C.c_uint64()
+    deg = C.c_int32()
+    st = lib().splat_ply_read(os.fsencode(path), None, 0, C.byref(n), C.byref(deg), err,
+                              len(err))
+    if st not in (OK, ERR_CAPACITY):
+        raise PlyError(st, err.value.decode())
+    g = np.zeros((n.value, rec.NUM_PARAMS), np.float32)
+    st = lib().splat_ply_read(os.fsencode(path), _p(g), n.value, C.byref(n), C.byref(deg), err,
+                              len(err))
+    if st != OK:
+        raise PlyError(st, err.value.decode())
+    return g, deg.value
 
 
 def to_rgb8(image):
@@ -424,6 +516,21 @@ class Trainer:
         self.ctx.check(lib().splat_trainer_render(self._h, view, sh_degree, int(antialias),
                                                   _p(bg), _p(img)), "trainer_render")
         return img
+
+    def train(self, extent, cfg, on_progress=None, on_save=None):
+        """train, src/train.rs:161-306 on the resident model; callbacks receive a
+        Progress record and (iteration, trainer)."""
+        def prog(p, _user):
+            if on_progress:
+                on_progress(p.contents)
+
+        def save(it, _h, _user):
+            try:
+                return 0 if on_save is None or on_save(it, self) is not False else 1
+            except Exception:
+                return 1
+        pf, sf = PROGRESS_FN(prog), SAVE_FN(save)
+        self.ctx.check(lib().splat_train(self._h, extent, C.byref(cfg), pf, sf, None), "train")
 
     def info(self):
         n = C.c_uint64()

@@ -257,6 +257,9 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
                            float lambda, bool want_l1, float* dimage, float* scratch,
                            double* loss_parts_host);
 size_t loss_scratch_floats(uint32_t width, uint32_t height);
+// The loss value of the last loss_forward_backward on `scratch`, into device memory.
+void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
+                   float lambda, float* out_dev);
 void blur_image(splat_ctx* ctx, const float* src, uint32_t width, uint32_t height,
                 uint32_t channels, float* dst);
 void gaussian_kernel_host(float k[11]);

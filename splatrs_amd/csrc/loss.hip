@@ -257,7 +257,31 @@ __global__ void blur_v_kernel(const float* __restrict__ tmp, uint32_t W, uint32_
   dst[idx] = acc;
 }
 
+// loss = (1 - lambda) * (sum |x - y| * inv_n) + lambda * (1 - sum SSIM / n), from the
+// per-block partial sums (summed in binary64), written to *out without a host round trip.
+__global__ void loss_value_kernel(const float2* __restrict__ partial, uint32_t nb, float lambda,
+                                  float n, float* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double a = 0, b = 0;
+  for (uint32_t i = 0; i < nb; i++) {
+    a += partial[i].x;
+    b += partial[i].y;
+  }
+  const float inv_n = 1.0f / n;
+  const float l1 = (float)b * inv_n;
+  const float ssim = (float)a / n;
+  *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
+}
+
 }  // namespace
+
+void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
+                   float lambda, float* out_dev) {
+  const size_t n = (size_t)width * height * 3;
+  const unsigned nb = blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
+  launch(ctx, "loss_value", loss_value_kernel, dim3(1), dim3(64), 0,
+         reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, out_dev);
+}
 
 // gaussian_kernel, src/loss.rs:14-27 (host, glibc-compatible expf).
 void gaussian_kernel_host(float k[11]) {

@@ -54,8 +54,9 @@ __device__ __forceinline__ uint32_t entry_rect(float4 r0, float4 r1, float4 r2, 
 }
 
 __device__ __forceinline__ bool in_rect(uint32_t rc, uint32_t i, uint32_t j) {
-  return i >= (rc & 0xffu) && i < ((rc >> 8) & 0xffu) && j >= ((rc >> 16) & 0xffu) &&
-         j < (rc >> 24);
+  // bitwise, not short-circuit: no divergent branches in the compositing loops
+  return (i >= (rc & 0xffu)) & (i < ((rc >> 8) & 0xffu)) & (j >= ((rc >> 16) & 0xffu)) &
+         (j < (rc >> 24));
 }
 
 // evaluate, src/rasterize.rs:231-244.
@@ -105,24 +106,31 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
       srect[threadIdx.x] = entry_rect(a, b, c, ti);
     }
     __syncthreads();
-    if (!done) {
-      for (uint32_t k = 0; k < cnt; k++) {
-        if (!in_rect(srect[k], i, j)) continue;
-        const float4 a = s0[k], b = s1[k];
-        float G, alpha;
-        if (!evaluate(a, b, px, py, tab, &G, &alpha)) continue;
-        const float test_t = T * (1.0f - alpha);
-        if (test_t < kTMin) {
-          done = true;
-          break;
-        }
-        const float4 c = s2[k];
-        c0 += alpha * T * c.y;
-        c1 += alpha * T * c.z;
-        c2 += alpha * T * c.w;
-        T = test_t;
-        nc = b0 - start + k + 1;
-      }
+    // Predicated form of the per-pixel walk (src/rasterize.rs:262-290): every decision of
+    // evaluate() and the T_MIN test is a lane predicate; only the exp is skipped when no
+    // lane of the wave needs it.
+    for (uint32_t k = 0; k < cnt; k++) {
+      const float4 a = s0[k], b = s1[k];
+      const float dx = px - a.x;
+      const float dy = py - a.y;
+      const float sigma = 0.5f * (a.z * dx * dx + b.x * dy * dy) + a.w * dx * dy;
+      const bool pass1 = (!done) & in_rect(srect[k], i, j) & !((sigma < 0.0f) | (sigma > b.z));
+      if (!__any(pass1)) continue;
+      const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
+      const float og = b.y * G;
+      const bool pass2 = pass1 & !(og < kAlphaMin);
+      const float alpha = rmin(og, kAlphaMax);
+      const float test_t = T * (1.0f - alpha);
+      const bool stop = test_t < kTMin;
+      done = done | (pass2 & stop);
+      const bool comp = pass2 & !stop;
+      const float4 c = s2[k];
+      const float w = alpha * T;
+      c0 = comp ? c0 + w * c.y : c0;
+      c1 = comp ? c1 + w * c.z : c1;
+      c2 = comp ? c2 + w * c.w : c2;
+      T = comp ? test_t : T;
+      nc = comp ? b0 - start + k + 1 : nc;
     }
   }
   if (valid) {
@@ -142,48 +150,62 @@ struct PixBack {
   uint32_t last;
 };
 
-// Terms of one (entry, pixel) pair, src/rasterize.rs:404-445.  v[] = slot contributions.
-__device__ __forceinline__ void pair_terms(PixBack& s, float4 r0, float4 r1, float4 r2, float px,
-                                           float py, const uint64_t* tab, float v[kSlots]) {
-  float G, alpha;
+// Terms of one (entry, pixel) pair, src/rasterize.rs:404-445, in predicated form: `act`
+// says the pixel is inside the entry's rectangle and before its last contributor.
+// Returns whether any lane of the wave contributes; v[] holds the slot terms (zero for
+// lanes that do not), and the pixel state advances only on contributing lanes.
+__device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, float4 r1, float4 r2,
+                                           float px, float py, const uint64_t* tab,
+                                           float v[kSlots]) {
 #pragma unroll
   for (int q = 0; q < kSlots; q++) v[q] = 0.0f;
-  if (!evaluate(r0, r1, px, py, tab, &G, &alpha)) return;
-  s.t /= 1.0f - alpha;
-  const float t = s.t;
-  float dalpha = 0.0f;
-  s.acc0 = s.last_alpha * s.lc0 + (1.0f - s.last_alpha) * s.acc0;
-  dalpha += (r2.y - s.acc0) * s.d0;
-  v[6] = alpha * t * s.d0;
-  s.acc1 = s.last_alpha * s.lc1 + (1.0f - s.last_alpha) * s.acc1;
-  dalpha += (r2.z - s.acc1) * s.d1;
-  v[7] = alpha * t * s.d1;
-  s.acc2 = s.last_alpha * s.lc2 + (1.0f - s.last_alpha) * s.acc2;
-  dalpha += (r2.w - s.acc2) * s.d2;
-  v[8] = alpha * t * s.d2;
-  dalpha *= t;
-  s.lc0 = r2.y;
-  s.lc1 = r2.z;
-  s.lc2 = r2.w;
-  s.last_alpha = alpha;
-  dalpha += -s.t_final / (1.0f - alpha) * s.bg_dot;
+  const float dx = px - r0.x;
+  const float dy = py - r0.y;
+  const float sigma = 0.5f * (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;
+  const bool p1 = act & !((sigma < 0.0f) | (sigma > r1.z));
+  if (!__any(p1)) return false;
+  const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
   const float o = r1.y;
-  if (o * G < kAlphaMax) {
-    v[5] = G * dalpha;
-    const float dgauss = o * dalpha;
-    const float dsigma = -dgauss * G;
-    const float dx = px - r0.x;
-    const float dy = py - r0.y;
-    const float ddx = dsigma * (r0.z * dx + r0.w * dy);
-    const float ddy = dsigma * (r0.w * dx + r1.x * dy);
-    v[0] = -ddx;
-    v[1] = -ddy;
-    v[2] = dsigma * 0.5f * dx * dx;
-    v[3] = dsigma * dx * dy;
-    v[4] = dsigma * 0.5f * dy * dy;
-    v[9] = fabsf(ddx);
-    v[10] = fabsf(ddy);
+  const float og = o * G;
+  const bool p2 = p1 & !(og < kAlphaMin);
+  const float alpha = rmin(og, kAlphaMax);
+  const float t = s.t / (1.0f - alpha);
+  const float a0 = s.last_alpha * s.lc0 + (1.0f - s.last_alpha) * s.acc0;
+  const float a1 = s.last_alpha * s.lc1 + (1.0f - s.last_alpha) * s.acc1;
+  const float a2 = s.last_alpha * s.lc2 + (1.0f - s.last_alpha) * s.acc2;
+  float dalpha = 0.0f;
+  dalpha += (r2.y - a0) * s.d0;
+  dalpha += (r2.z - a1) * s.d1;
+  dalpha += (r2.w - a2) * s.d2;
+  dalpha *= t;
+  dalpha += -s.t_final / (1.0f - alpha) * s.bg_dot;
+  const bool p3 = p2 & (og < kAlphaMax);
+  const float dgauss = o * dalpha;
+  const float dsigma = -dgauss * G;
+  const float ddx = dsigma * (r0.z * dx + r0.w * dy);
+  const float ddy = dsigma * (r0.w * dx + r1.x * dy);
+  v[6] = p2 ? alpha * t * s.d0 : 0.0f;
+  v[7] = p2 ? alpha * t * s.d1 : 0.0f;
+  v[8] = p2 ? alpha * t * s.d2 : 0.0f;
+  v[5] = p3 ? G * dalpha : 0.0f;
+  v[0] = p3 ? -ddx : 0.0f;
+  v[1] = p3 ? -ddy : 0.0f;
+  v[2] = p3 ? dsigma * 0.5f * dx * dx : 0.0f;
+  v[3] = p3 ? dsigma * dx * dy : 0.0f;
+  v[4] = p3 ? dsigma * 0.5f * dy * dy : 0.0f;
+  v[9] = p3 ? fabsf(ddx) : 0.0f;
+  v[10] = p3 ? fabsf(ddy) : 0.0f;
+  if (p2) {
+    s.t = t;
+    s.acc0 = a0;
+    s.acc1 = a1;
+    s.acc2 = a2;
+    s.lc0 = r2.y;
+    s.lc1 = r2.z;
+    s.lc2 = r2.w;
+    s.last_alpha = alpha;
   }
+  return true;
 }
 
 // Sum over each 16-lane row of the wave with DPP row shifts (ALU-rate cross-lane moves);
@@ -270,19 +292,13 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     for (int32_t kk = (int32_t)cnt - 1; kk >= 0; kk--) {
       const uint32_t k = lo + kk;
       float v[kSlots];
-      bool act = valid && k < s.last && in_rect(srect[kk], i, j);
-      if (act) {
-        pair_terms(s, s0[kk], s1[kk], s2[kk], px, py, tab, v);
-      } else {
-#pragma unroll
-        for (int q = 0; q < kSlots; q++) v[q] = 0.0f;
-      }
+      const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
+      const bool any = pair_terms(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v);
       if (ORDERED) {
 #pragma unroll
         for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + threadIdx.x] = v[q];
       } else {
         constexpr int NS = ABS ? kSlots : 9;
-        const bool any = __any(act);
         const bool row_end = (lane & 15) == 15;
         const int part = wave * 4 + (lane >> 4);
         if (any) {

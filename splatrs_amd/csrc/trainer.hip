@@ -3,6 +3,7 @@
 // launches project -> bin -> composite -> loss -> backward -> statistics -> Adam on the
 // context stream.  The only host synchronisations are reading the tile-entry count (to
 // size the entry arrays) and, on request, the loss value.
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -27,6 +28,7 @@ struct splat_trainer {
   std::vector<std::unique_ptr<TrainerView>> views;
   std::map<std::string, DevBuf> store;  // per-view buffers (projection, binning, images)
   size_t entry_cap = 0;
+  float* loss_target = nullptr;  // when set, each step writes its loss value here (device)
 };
 
 namespace {
@@ -216,6 +218,7 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     double parts[2] = {0, 0};
     loss_forward_backward(ctx, image, nullptr, tv.rgb.as<uint8_t>(), W, H, p->lambda_dssim, true,
                           dimage, scratch, p->want_loss ? parts : nullptr);
+    if (t->loss_target) loss_value_to(ctx, scratch, W, H, p->lambda_dssim, t->loss_target);
 
     // backward (src/train.rs:225-235)
     float* slots = (float*)t->store["slots"].reserve((E + 1) * kSlots * 4);
@@ -377,6 +380,244 @@ int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* 
     SPLAT_HIP(hipStreamSynchronize(ctx->stream));
     t->store.erase("aos_out");
   });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- train loop -------
+
+namespace {
+
+uint64_t sm64_next(uint64_t& state) {  // SplitMix64
+  uint64_t z = (state += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Per-epoch shuffled view order popped from the back (src/train.rs:203-207).  The
+// reference shuffles with ChaCha8; this draws a Fisher-Yates shuffle from SplitMix64.
+struct ViewOrder {
+  uint64_t state;
+  uint32_t n;
+  std::vector<uint32_t> order;
+  ViewOrder(uint64_t seed, uint32_t num) : state(seed), n(num) {}
+  uint32_t next() {
+    if (order.empty()) {
+      order.resize(n);
+      for (uint32_t i = 0; i < n; i++) order[i] = i;
+      for (uint32_t i = n; i-- > 1;) {
+        const uint32_t j = (uint32_t)(sm64_next(state) % (uint64_t)(i + 1));
+        std::swap(order[i], order[j]);
+      }
+    }
+    const uint32_t v = order.back();
+    order.pop_back();
+    return v;
+  }
+};
+
+// TrainConfig::position_lr, src/train.rs:132-137 (glibc logf/expf, as the reference).
+float position_lr(const splat_train_config& c, uint32_t it, float extent) {
+  float t = (float)it / (float)std::max<uint32_t>(c.position_lr_max_steps, 1u);
+  t = rclamp(t, 0.0f, 1.0f);
+  const float init = c.position_lr_init * extent;
+  const float fin = c.position_lr_final * extent;
+  return expf(logf(init) * (1.0f - t) + logf(fin) * t);
+}
+
+// psnr, src/loss.rs:175-190, against RGB8 (View::to_real).
+float psnr_u8(const std::vector<float>& img, const std::vector<uint8_t>& gt) {
+  double mse = 0;
+  for (size_t i = 0; i < img.size(); i++) {
+    const double d = (double)rclamp(img[i], 0.0f, 1.0f) -
+                     (double)rclamp((float)gt[i] / 255.0f, 0.0f, 1.0f);
+    mse += d * d;
+  }
+  mse /= (double)std::max<size_t>(img.size(), 1);
+  return mse <= 0 ? INFINITY : (float)(-10.0 * std::log10(mse));
+}
+
+}  // namespace
+
+extern "C" {
+
+void splat_train_config_default(splat_train_config* c) {  // src/train.rs:76-109
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->iterations = 30000;
+  c->position_lr_init = 0.00016f;
+  c->position_lr_final = 0.0000016f;
+  c->position_lr_max_steps = 30000;
+  c->feature_lr = 0.0025f;
+  c->opacity_lr = 0.025f;
+  c->scaling_lr = 0.005f;
+  c->rotation_lr = 0.001f;
+  c->lambda_dssim = 0.2f;
+  c->densify = 1;
+  c->densify_from_iter = 500;
+  c->densify_until_iter = 15000;
+  c->densification_interval = 100;
+  c->opacity_reset_interval = 3000;
+  c->densify_grad_threshold = 0.0002f;
+  c->percent_dense = 0.01f;
+  c->min_opacity = 0.005f;
+  c->sh_degree = 3;
+  c->sh_degree_interval = 1000;
+  c->near_plane = 0.2f;
+  c->seed = 0;
+  c->log_interval = 100;
+}
+
+uint32_t splat_train_view_at(uint64_t seed, uint32_t num_views, uint32_t it) {
+  if (num_views == 0 || it == 0) return 0;
+  ViewOrder o(seed, num_views);
+  uint32_t v = 0;
+  for (uint32_t i = 1; i <= it; i++) v = o.next();
+  return v;
+}
+
+int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
+                splat_progress_fn on_progress, splat_save_fn on_save, void* user) {
+  if (!t || !cfg) return SPLAT_ERR_INVALID;
+  splat_ctx* ctx = t->ctx;
+  // src/train.rs:169-183: validation before anything runs or saves
+  if (t->views.empty()) {
+    ctx->err = "no training views";
+    return SPLAT_ERR_INVALID;
+  }
+  if (cfg->iterations == 0) {
+    ctx->err = "iterations must be at least 1";
+    return SPLAT_ERR_INVALID;
+  }
+  if (cfg->sh_degree < 0 || cfg->sh_degree > 3) {
+    ctx->err = "sh_degree exceeds the maximum of 3";
+    return SPLAT_ERR_INVALID;
+  }
+  int status = SPLAT_OK;
+  const int st = tguard(t, [&] {
+    const uint32_t log_iv = cfg->log_interval;
+    DevBuf loss_hist;
+    float* hist = (float*)loss_hist.reserve((size_t)(log_iv ? log_iv : 1) * 4 + 4);
+    ViewOrder order(cfg->seed, (uint32_t)t->views.size());
+    const float bg = cfg->white_background ? 1.0f : 0.0f;
+    int32_t sh = 0;  // model.sh_degree starts at 0 (src/train.rs:186-187)
+    float ema = 0.0f;
+    uint64_t dc = 0, ds = 0, dp = 0;
+    uint32_t since_log = 0;
+    auto timer = std::chrono::steady_clock::now();
+    std::vector<float> losses;
+    for (uint32_t it = 1; it <= cfg->iterations; it++) {
+      if (cfg->sh_degree_interval > 0 && it % cfg->sh_degree_interval == 0 && sh < cfg->sh_degree)
+        sh++;
+      const uint32_t view = order.next();
+      const bool in_window = cfg->densify && it < cfg->densify_until_iter;
+      splat_step_params p;
+      std::memset(&p, 0, sizeof(p));
+      p.sh_degree = sh;
+      p.accumulate_stats = in_window;
+      p.absgrad = cfg->absgrad;
+      p.ordered_backward = cfg->ordered_backward;
+      p.antialias = cfg->antialias;
+      p.near_plane = cfg->near_plane;
+      p.lambda_dssim = cfg->lambda_dssim;
+      p.background[0] = p.background[1] = p.background[2] = bg;
+      p.lr.mean = position_lr(*cfg, it, extent);
+      p.lr.log_scale = cfg->scaling_lr;
+      p.lr.quat = cfg->rotation_lr;
+      p.lr.opacity = cfg->opacity_lr;
+      p.lr.sh_dc = cfg->feature_lr;
+      p.lr.sh_rest = cfg->feature_lr / 20.0f;
+      t->loss_target = log_iv ? hist + since_log % log_iv : nullptr;
+      const int s1 = splat_trainer_step(t, view, &p, nullptr);
+      t->loss_target = nullptr;
+      if (s1 != SPLAT_OK) fail(s1, ctx->err);
+      since_log++;
+      if (in_window) {  // density control, src/train.rs:249-271
+        if (it > cfg->densify_from_iter && cfg->densification_interval > 0 &&
+            it % cfg->densification_interval == 0) {
+          splat_densify_config dcfg;
+          dcfg.grad_threshold = cfg->densify_grad_threshold;
+          dcfg.percent_dense = cfg->percent_dense;
+          dcfg.min_opacity = cfg->min_opacity;
+          dcfg.prune_big_world = cfg->opacity_reset_interval > 0 && it > cfg->opacity_reset_interval;
+          dcfg.max_world_size_frac = 0.1f;
+          dcfg.has_max_screen_size = cfg->has_max_screen_size;
+          dcfg.max_screen_size = cfg->max_screen_size;
+          dcfg.split_children = 2;
+          uint64_t s = cfg->seed ^ (0x5851f42d4c957f2dull * (uint64_t)it);
+          splat_densify_report rep;
+          const int s2 = splat_trainer_densify(t, &dcfg, extent, sm64_next(s), &rep);
+          if (s2 != SPLAT_OK) fail(s2, ctx->err);
+          dc += rep.cloned;
+          ds += rep.split;
+          dp += rep.pruned;
+        }
+        const bool reset = cfg->opacity_reset_interval > 0 &&
+                           (it % cfg->opacity_reset_interval == 0 ||
+                            (cfg->white_background && it == cfg->densify_from_iter));
+        if (reset) {
+          const int s3 = splat_trainer_reset_opacity(t, 0.01f);
+          if (s3 != SPLAT_OK) fail(s3, ctx->err);
+        }
+      }
+      const bool log_now = log_iv > 0 && (it % log_iv == 0 || it == cfg->iterations);
+      if (log_now) {
+        // EMA over every iteration's loss (src/train.rs:274-278), read back in one copy.
+        losses.resize(since_log);
+        SPLAT_HIP(hipMemcpyAsync(losses.data(), hist, since_log * 4, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+        const uint32_t first = it - since_log + 1;
+        for (uint32_t k = 0; k < since_log; k++)
+          ema = (first + k == 1) ? losses[k] : 0.4f * losses[k] + 0.6f * ema;
+        if (log_now && on_progress) {
+          const auto now = std::chrono::steady_clock::now();
+          const double ms = std::chrono::duration<double, std::milli>(now - timer).count();
+          const TrainerView& tv = *t->views[view];
+          const size_t npx = (size_t)tv.cam.width * tv.cam.height;
+          std::vector<float> img(npx * 3);
+          std::vector<uint8_t> gt(npx * 3);
+          SPLAT_HIP(hipMemcpyAsync(img.data(), t->store["image"].ptr, npx * 12,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+          SPLAT_HIP(hipMemcpyAsync(gt.data(), tv.rgb.ptr, npx * 3, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+          SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+          splat_progress pr;
+          pr.iteration = it;
+          pr.loss = losses[since_log - 1];
+          pr.ema_loss = ema;
+          pr.psnr = psnr_u8(img, gt);
+          pr.num_gaussians = t->n;
+          const uint32_t span = (it % log_iv == 0) ? log_iv : it % log_iv;
+          pr.ms_per_iter = ms / std::max<uint32_t>(span, 1);
+          pr.cloned = dc;
+          pr.split = ds;
+          pr.pruned = dp;
+          on_progress(&pr, user);
+          dc = ds = dp = 0;
+          timer = std::chrono::steady_clock::now();
+        }
+        since_log = 0;
+      }
+      bool save = false;
+      for (uint32_t k = 0; k < cfg->num_save_iterations; k++)
+        if (cfg->save_iterations && cfg->save_iterations[k] == it) save = true;
+      if (save && it != cfg->iterations && on_save) {
+        SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+        if (on_save(it, t, user) != 0) {
+          status = SPLAT_ERR_IO;
+          return;
+        }
+      }
+    }
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    if (on_save && on_save(cfg->iterations, t, user) != 0) status = SPLAT_ERR_IO;
+  });
+  t->loss_target = nullptr;
+  if (st != SPLAT_OK) return st;
+  if (status != SPLAT_OK) ctx->err = "on_save failed";
+  return status;
 }
 
 }  // extern "C"

@@ -79,6 +79,20 @@ def ground_truth(rng, n):
     return out
 
 
+def init_points(gt, rng):
+    """tests/common/scene.rs:111-127: the true means jittered by U(-0.03, 0.03), with the
+    DC colour as RGB8."""
+    sh_c0 = np.float32(0.28209479177387814)
+    xyz = np.zeros((len(gt), 3), np.float32)
+    rgb = np.zeros((len(gt), 3), np.uint8)
+    for i, g in enumerate(gt):
+        xyz[i] = [np.float32(g[k]) + np.float32(rnd(rng, -0.03, 0.03)) for k in range(3)]
+        for c in range(3):
+            v = np.clip(np.float32(g[11 + c]) * sh_c0 + np.float32(0.5), 0, 1) * np.float32(255)
+            rgb[i, c] = int(np.floor(v + 0.5))
+    return xyz, rgb
+
+
 def naive_render(projs, width, height, bg):
     """tests/pipeline.rs:134-186: per-pixel compositor over every visible Gaussian whose
     tile rectangle covers the pixel's tile, in (depth bits, index) order."""
