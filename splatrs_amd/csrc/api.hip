@@ -63,9 +63,7 @@ void download_planes(splat_ctx* ctx, const float* planes, size_t n, size_t strid
 ProjBuffers alloc_proj(splat_ctx* ctx, size_t n) {
   ProjBuffers pb;
   const size_t c = n + 1;
-  pb.rec0 = ctx->buf<float4>("proj_rec0", c);
-  pb.rec1 = ctx->buf<float4>("proj_rec1", c);
-  pb.rec2 = ctx->buf<float4>("proj_rec2", c);
+  pb.rec = ctx->buf<float4>("proj_rec", 4 * c);
   pb.depth = ctx->buf<float>("proj_depth", c);
   pb.radius = ctx->buf<uint32_t>("proj_radius", c);
   pb.rect = ctx->buf<uint2>("proj_rect", c);

@@ -185,9 +185,10 @@ inline CameraDev camera_dev(const splat_camera& c) {
 
 // Projection outputs for one view (device pointers, capacity n).
 struct ProjBuffers {
-  float4* rec0;     // {mx, my, A, B}
-  float4* rec1;     // {C, opacity, sigma_max, half_x}; half_x < 0 marks an empty footprint
-  float4* rec2;     // {half_y, r, g, b}
+  // One 64-byte rasterizer record per Gaussian (a single cache line per gather):
+  // rec[4i] = {mx, my, A, B}, rec[4i+1] = {C, opacity, sigma_max, half_x} (half_x < 0
+  // marks an empty footprint), rec[4i+2] = {half_y, r, g, b}, rec[4i+3] unused.
+  float4* rec;
   float* depth;
   uint32_t* radius;
   uint2* rect;      // {x0 | y0 << 16, x1 | y1 << 16}, tile_max exclusive

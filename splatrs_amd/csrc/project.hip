@@ -316,10 +316,9 @@ __global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restric
     if (tx0 >= tx1 || ty0 >= ty1) ok = false;
   }
   if (!ok) {
-    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    pb.rec0[i] = z;
-    pb.rec1[i] = make_float4(0.f, 0.f, 0.f, -1.f);
-    pb.rec2[i] = make_float4(-1.f, 0.f, 0.f, 0.f);
+    pb.rec[4 * (size_t)i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    pb.rec[4 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, -1.f);
+    pb.rec[4 * (size_t)i + 2] = make_float4(-1.f, 0.f, 0.f, 0.f);
     pb.depth[i] = 0.0f;
     pb.radius[i] = 0u;
     pb.rect[i] = make_uint2(0u, 0u);
@@ -330,9 +329,9 @@ __global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restric
   raster_record(k.mean2d[0], k.mean2d[1], k.conic[0], k.conic[1], k.conic[2],
                 k.o_raw * k.comp, rmax(k.color_raw[0], 0.0f), rmax(k.color_raw[1], 0.0f),
                 rmax(k.color_raw[2], 0.0f), &a, &b, &c);
-  pb.rec0[i] = a;
-  pb.rec1[i] = b;
-  pb.rec2[i] = c;
+  pb.rec[4 * (size_t)i] = a;
+  pb.rec[4 * (size_t)i + 1] = b;
+  pb.rec[4 * (size_t)i + 2] = c;
   pb.depth[i] = k.t.z;
   pb.radius[i] = as_u32(r);
   pb.rect[i] = make_uint2(tx0 | (ty0 << 16), tx1 | (ty1 << 16));
@@ -344,7 +343,8 @@ __global__ void proj_pack_kernel(ProjBuffers pb, uint32_t n, splat_proj* out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   splat_proj p;
-  const float4 a = pb.rec0[i], b = pb.rec1[i], c = pb.rec2[i];
+  const float4 a = pb.rec[4 * (size_t)i], b = pb.rec[4 * (size_t)i + 1],
+               c = pb.rec[4 * (size_t)i + 2];
   const uint32_t rad = pb.radius[i];
   if (rad == 0) {
     memset(&p, 0, sizeof(p));
@@ -378,9 +378,9 @@ __global__ void proj_unpack_kernel(const splat_proj* __restrict__ in, uint32_t n
   if (i >= n) return;
   const splat_proj p = in[i];
   if (p.radius == 0) {
-    pb.rec0[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    pb.rec1[i] = make_float4(0.f, 0.f, 0.f, -1.f);
-    pb.rec2[i] = make_float4(-1.f, 0.f, 0.f, 0.f);
+    pb.rec[4 * (size_t)i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    pb.rec[4 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, -1.f);
+    pb.rec[4 * (size_t)i + 2] = make_float4(-1.f, 0.f, 0.f, 0.f);
     pb.depth[i] = p.depth;
     pb.radius[i] = 0u;
     pb.rect[i] = make_uint2(0u, 0u);
@@ -395,9 +395,9 @@ __global__ void proj_unpack_kernel(const splat_proj* __restrict__ in, uint32_t n
   float4 a, b, c;
   raster_record(p.mean2d[0], p.mean2d[1], p.conic[0], p.conic[1], p.conic[2], p.opacity,
                 p.color[0], p.color[1], p.color[2], &a, &b, &c);
-  pb.rec0[i] = a;
-  pb.rec1[i] = b;
-  pb.rec2[i] = c;
+  pb.rec[4 * (size_t)i] = a;
+  pb.rec[4 * (size_t)i + 1] = b;
+  pb.rec[4 * (size_t)i + 2] = c;
   pb.depth[i] = p.depth;
   pb.radius[i] = p.radius;
   const uint32_t x0 = p.tile_min[0], y0 = p.tile_min[1], x1 = p.tile_max[0], y1 = p.tile_max[1];

@@ -75,8 +75,7 @@ __device__ __forceinline__ bool evaluate(float4 r0, float4 r1, float px, float p
 }
 
 __global__ void __launch_bounds__(256) raster_fwd_kernel(
-    const float4* __restrict__ rec0, const float4* __restrict__ rec1,
-    const float4* __restrict__ rec2, const uint32_t* __restrict__ tile_start,
+    const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
     const uint32_t* __restrict__ sorted_gauss, uint32_t tiles_x, uint32_t W, uint32_t H,
     float bg0, float bg1, float bg2, float* __restrict__ image, float* __restrict__ final_t,
     uint32_t* __restrict__ n_contrib) {
@@ -99,7 +98,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
     const uint32_t cnt = min((uint32_t)kBatch, end - b0);
     if (threadIdx.x < cnt) {
       const uint32_t g = sorted_gauss[b0 + threadIdx.x];
-      const float4 a = rec0[g], b = rec1[g], c = rec2[g];
+      const float4 a = rec[4 * g], b = rec[4 * g + 1], c = rec[4 * g + 2];
       s0[threadIdx.x] = a;
       s1[threadIdx.x] = b;
       s2[threadIdx.x] = c;
@@ -223,8 +222,7 @@ __device__ __forceinline__ float row_sum16(float v) {
 // ABS: also accumulate the AbsGS slots (9, 10); otherwise they are written as zero.
 template <bool ORDERED, bool ABS>
 __global__ void __launch_bounds__(256) raster_bwd_kernel(
-    const float4* __restrict__ rec0, const float4* __restrict__ rec1,
-    const float4* __restrict__ rec2, const uint32_t* __restrict__ tile_start,
+    const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
     const uint32_t* __restrict__ sorted_gauss, const uint32_t* __restrict__ sorted_entry,
     uint32_t tiles_x, uint32_t W, uint32_t H, float bg0, float bg1, float bg2,
     const float* __restrict__ final_t, const uint32_t* __restrict__ n_contrib,
@@ -282,7 +280,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     __syncthreads();
     if (threadIdx.x < cnt) {
       const uint32_t g = sorted_gauss[start + lo + threadIdx.x];
-      const float4 a = rec0[g], b = rec1[g], c = rec2[g];
+      const float4 a = rec[4 * g], b = rec[4 * g + 1], c = rec[4 * g + 2];
       s0[threadIdx.x] = a;
       s1[threadIdx.x] = b;
       s2[threadIdx.x] = c;
@@ -371,8 +369,8 @@ void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
                     uint32_t height, const float bg[3], float* image, float* final_t,
                     uint32_t* n_contrib) {
   const unsigned T = bb.tiles_x * bb.tiles_y;
-  launch(ctx, "raster_fwd", raster_fwd_kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec0,
-         (const float4*)pb.rec1, (const float4*)pb.rec2, (const uint32_t*)bb.tile_start,
+  launch(ctx, "raster_fwd", raster_fwd_kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec,
+         (const uint32_t*)bb.tile_start,
          (const uint32_t*)bb.sorted_gauss, bb.tiles_x, width, height, bg[0], bg[1], bg[2], image,
          final_t, n_contrib);
 }
@@ -383,8 +381,8 @@ void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb
                      bool need_abs) {
   const unsigned T = bb.tiles_x * bb.tiles_y;
   auto go = [&](const char* name, auto kernel) {
-    launch(ctx, name, kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec0,
-           (const float4*)pb.rec1, (const float4*)pb.rec2, (const uint32_t*)bb.tile_start,
+    launch(ctx, name, kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec,
+           (const uint32_t*)bb.tile_start,
            (const uint32_t*)bb.sorted_gauss, (const uint32_t*)bb.sorted_entry, bb.tiles_x, width,
            height, bg[0], bg[1], bg[2], final_t, n_contrib, dimage, slots);
   };

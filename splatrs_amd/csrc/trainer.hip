@@ -77,9 +77,7 @@ void zero_stats(splat_trainer* t) {
 ProjBuffers proj_buffers(splat_trainer* t) {
   ProjBuffers pb;
   const size_t c = t->cap + 1;
-  pb.rec0 = (float4*)t->store["rec0"].reserve(c * 16);
-  pb.rec1 = (float4*)t->store["rec1"].reserve(c * 16);
-  pb.rec2 = (float4*)t->store["rec2"].reserve(c * 16);
+  pb.rec = (float4*)t->store["rec"].reserve(c * 64);
   pb.depth = (float*)t->store["depth"].reserve(c * 4);
   pb.radius = (uint32_t*)t->store["radius"].reserve(c * 4);
   pb.rect = (uint2*)t->store["rect"].reserve(c * 8);
