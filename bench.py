@@ -105,23 +105,33 @@ def main():
                         visible=int(r["num_visible"]))
     ctx.sync()
 
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        it += 1
-        trainer.step(view, params(it))
-    ctx.sync()
-    t1 = time.perf_counter()
-    sec = t1 - t0
-    value = args.steps / sec
+    def timed_pass():
+        nonlocal it
+        trainer.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            it += 1
+            trainer.step(view, params(it), result=False)
+        trainer.sync()
+        return time.perf_counter() - t0
 
-    # per-kernel attribution (separate pass: events add a little launch overhead)
+    # graph-captured steps (the default mode): the measured value
+    sec = timed_pass()
+    value = args.steps / sec
+    graph_stats = trainer.graph_stats()
+    # the same kernels launched one by one (for comparison)
+    trainer.set_graph(False)
+    eager_sec = timed_pass()
+
+    # per-kernel attribution (separate eager pass: events add a little launch overhead)
     ctx.set_timing(True)
     ctx.reset_timing()
     for _ in range(args.steps):
         it += 1
-        trainer.step(view, params(it))
+        trainer.step(view, params(it), result=False)
     times = ctx.kernel_times()
     ctx.set_timing(False)
+    trainer.set_graph(True)
     r = trainer.step(view, params(it + 1, want_loss=True))
     N, V, E, P = args.gaussians, int(r["num_visible"]), int(r["num_entries"]), W * H
 
@@ -178,7 +188,10 @@ def main():
                    "gaussians": N, "width": W, "height": H, "tile": 16, "sh_degree": 3,
                    "visible": V, "tile_entries": E,
                    "backward": "ordered" if args.ordered else "tree-reduced",
-                   "loss_at_step1": round(info["loss0"], 6)},
+                   "loss_at_step1": round(info["loss0"], 6),
+                   "step_mode": "hipGraph replay",
+                   "eager_it_per_s": round(args.steps / eager_sec, 3),
+                   "graph_captures_replays_aborts": list(graph_stats)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -325,9 +325,27 @@ int splat_trainer_destroy(splat_trainer* t);
 int splat_trainer_add_view(splat_trainer* t, const splat_camera* cam, const uint8_t* rgb,
                            uint32_t* index);
 /* One iteration on view `view`: project -> bin -> composite -> loss -> backward ->
- * statistics -> Adam (src/train.rs:211-247). */
+ * statistics -> Adam (src/train.rs:211-247).  In graph mode (the default) the step is a
+ * replay of a captured HIP graph and returns without waiting for the GPU when r is NULL
+ * (at most two steps stay unchecked in flight); with r it waits and fills r.  Every other
+ * trainer call waits for the steps in flight first. */
 int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params* p,
                        splat_step_result* r);
+/* No counterpart in the reference (its steps are plain function calls).  enabled = 1:
+ * graph-captured steps (default); 0: kernels launched one by one, with a host read of the
+ * entry count inside each step (the mode kernel timing uses).  Results are identical. */
+int splat_trainer_set_graph(splat_trainer* t, int32_t enabled);
+/* No counterpart in the reference.  Wait for the steps in flight (replaying any that
+ * outgrew the entry arrays). */
+int splat_trainer_sync(splat_trainer* t);
+/* No counterpart in the reference.  Capacity of the tile-entry arrays that graph-captured
+ * steps preallocate (default max(4 n, 65536), grown by 1.25x on demand); a caller that
+ * knows its scenes can size them up front. */
+int splat_trainer_reserve_entries(splat_trainer* t, uint64_t entries);
+/* No counterpart in the reference.  Graph captures, graph replays, and steps replayed
+ * because their tile entries outgrew the preallocated arrays. */
+int splat_trainer_graph_stats(splat_trainer* t, uint64_t* captures, uint64_t* replays,
+                              uint64_t* aborts);
 /* densify_and_prune on the resident model (src/densify.rs:125-221); the normal draws
  * for split children come from a counter-based generator keyed by `seed`. */
 int splat_trainer_densify(splat_trainer* t, const splat_densify_config* cfg, float extent,

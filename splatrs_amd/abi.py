@@ -168,6 +168,10 @@ _SIGS = {
     "splat_trainer_reset_opacity": (i32, [P, f32]),
     "splat_trainer_render": (i32, [P, u32, i32, i32, P, P]),
     "splat_trainer_info": (i32, [P, P, P]),
+    "splat_trainer_set_graph": (i32, [P, i32]),
+    "splat_trainer_graph_stats": (i32, [P, P, P, P]),
+    "splat_trainer_reserve_entries": (i32, [P, u64]),
+    "splat_trainer_sync": (i32, [P]),
     "splat_trainer_download": (i32, [P, P, P, P, P, P, P, u64]),
 }
 
@@ -494,10 +498,29 @@ class Trainer:
         self.views.append(cam)
         return idx.value
 
-    def step(self, view, params):
-        r = np.zeros((), rec.STEP_RESULT)
-        self.ctx.check(lib().splat_trainer_step(self._h, view, _p(params), _p(r)), "trainer_step")
+    def step(self, view, params, result=True):
+        """One iteration; with result=False the call returns without waiting (graph mode)."""
+        r = np.zeros((), rec.STEP_RESULT) if result else None
+        self.ctx.check(lib().splat_trainer_step(self._h, view, _p(params),
+                                                _p(r) if result else None), "trainer_step")
         return r
+
+    def set_graph(self, enabled):
+        self.ctx.check(lib().splat_trainer_set_graph(self._h, int(enabled)), "trainer_set_graph")
+
+    def sync(self):
+        self.ctx.check(lib().splat_trainer_sync(self._h), "trainer_sync")
+
+    def reserve_entries(self, entries):
+        self.ctx.check(lib().splat_trainer_reserve_entries(self._h, entries),
+                       "trainer_reserve_entries")
+
+    def graph_stats(self):
+        """(captures, replays, aborted steps replayed after growing the entry arrays)."""
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.ctx.check(lib().splat_trainer_graph_stats(self._h, C.byref(a), C.byref(b),
+                                                       C.byref(c)), "trainer_graph_stats")
+        return a.value, b.value, c.value
 
     def densify(self, cfg, extent, seed):
         rep = np.zeros((), rec.DENSIFY_REPORT)

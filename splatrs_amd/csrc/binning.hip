@@ -38,7 +38,12 @@ __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ entry_gaussian,
                                                      uint32_t* __restrict__ entry_tile,
                                                      uint32_t* __restrict__ entry_slot,
-                                                     uint32_t* __restrict__ tile_count) {
+                                                     uint32_t* __restrict__ tile_count,
+                                                     const StepStatus* __restrict__ status) {
+  if (status) {
+    if (status->abort) return;
+    e_total = status->entries;
+  }
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= e_total) return;
   // last g with entry_offset[g] <= e (Gaussians with zero entries are skipped)
@@ -68,11 +73,26 @@ __global__ void __launch_bounds__(256) scatter_kernel(uint32_t e_total,
                                                       const uint32_t* __restrict__ entry_slot,
                                                       const uint32_t* __restrict__ tile_start,
                                                       const float* __restrict__ depth,
-                                                      uint64_t* __restrict__ bucket) {
+                                                      uint64_t* __restrict__ bucket,
+                                                      const StepStatus* __restrict__ status) {
+  if (status) {
+    if (status->abort) return;
+    e_total = status->entries;
+  }
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= e_total) return;
   const uint32_t pos = tile_start[entry_tile[e]] + entry_slot[e];
   bucket[pos] = ((uint64_t)f2bits(depth[entry_gaussian[e]]) << 32) | e;
+}
+
+// Entry count of the step and the capacity check of a graph-captured step.
+__global__ void guard_kernel(const unsigned long long* __restrict__ total64,
+                             unsigned long long capacity, StepStatus* __restrict__ status) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const unsigned long long t = *total64;
+  status->total = t;
+  status->entries = t > 0xffffffffull ? 0xffffffffu : (uint32_t)t;
+  if (t > capacity) status->abort = 1u;
 }
 
 // One bitonic step over `n` keys (virtual +inf padding beyond n).  Pairs p in [0, half).
@@ -205,7 +225,7 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   uint32_t* big_list = (uint32_t*)store["bin_big_list"].reserve(T * 4);
 
   // 1. entry offsets; the 64-bit total guards the u32 offsets (src/rasterize.rs:74)
-  SPLAT_HIP(hipMemsetAsync(total64, 0, 8, ctx->stream));
+  fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
   launch(ctx, "bin_total", sum_u64_kernel, dim3(std::min<unsigned>(blocks_for(n, 256), 1024u)),
          dim3(256), 0, (const uint32_t*)pb.ntiles, n, total64);
   exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small);
@@ -237,14 +257,15 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   fill_u32(ctx, small + 1, 1, 0);
   launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(E, 256)), dim3(256), 0,
          (const uint32_t*)bb.entry_offset, (uint32_t)n, (uint32_t)E, (const uint2*)pb.rect,
-         tiles_x, bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count);
+         tiles_x, bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
+         (const StepStatus*)nullptr);
   // 3. tile starts
   exclusive_scan_u32(ctx, bb.tile_count, bb.tile_start, T, small + 2);
   // 4. scatter into buckets
   launch(ctx, "bin_scatter", scatter_kernel, dim3(blocks_for(E, 256)), dim3(256), 0,
          (uint32_t)E, (const uint32_t*)bb.entry_gaussian, (const uint32_t*)bb.entry_tile,
          (const uint32_t*)bb.entry_slot, (const uint32_t*)bb.tile_start,
-         (const float*)pb.depth, bb.bucket);
+         (const float*)pb.depth, bb.bucket, (const StepStatus*)nullptr);
   // 5. per-tile sorts
   launch(ctx, "bin_sort_small", sort_small_kernel, dim3((unsigned)T), dim3(256), 0, (uint32_t)T,
          (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
@@ -254,6 +275,42 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
          bb.sorted_entry, bb.sorted_gauss, keys_out, (const uint32_t*)big_list,
          (const uint32_t*)(small + 1));
   return E;
+}
+
+void bin_guard(splat_ctx* ctx, const unsigned long long* total64, size_t capacity,
+               StepStatus* status) {
+  launch(ctx, "bin_guard", guard_kernel, dim3(1), dim3(64), 0, total64,
+         (unsigned long long)capacity, status);
+}
+
+void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
+                      size_t capacity, uint32_t* small, unsigned long long* total64,
+                      uint32_t* big_list, uint32_t* scan_sums, StepStatus* status) {
+  const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
+  fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
+  launch(ctx, "bin_total", sum_u64_kernel, dim3(std::min<unsigned>(blocks_for(n, 256), 1024u)),
+         dim3(256), 0, (const uint32_t*)pb.ntiles, n, total64);
+  exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums);
+  bin_guard(ctx, total64, capacity, status);
+  fill_u32(ctx, bb.tile_count, T, 0);
+  fill_u32(ctx, small + 1, 1, 0);
+  // The entry kernels cover the capacity and read the step's count from `status`.
+  launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0,
+         (const uint32_t*)bb.entry_offset, (uint32_t)n, 0u, (const uint2*)pb.rect, bb.tiles_x,
+         bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
+         (const StepStatus*)status);
+  exclusive_scan_u32(ctx, bb.tile_count, bb.tile_start, T, small + 2, scan_sums);
+  launch(ctx, "bin_scatter", scatter_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0, 0u,
+         (const uint32_t*)bb.entry_gaussian, (const uint32_t*)bb.entry_tile,
+         (const uint32_t*)bb.entry_slot, (const uint32_t*)bb.tile_start, (const float*)pb.depth,
+         bb.bucket, (const StepStatus*)status);
+  launch(ctx, "bin_sort_small", sort_small_kernel, dim3((unsigned)T), dim3(256), 0, (uint32_t)T,
+         (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
+         bb.sorted_entry, bb.sorted_gauss, (uint64_t*)nullptr, big_list, small + 1);
+  launch(ctx, "bin_sort_big", sort_big_kernel, dim3(256), dim3(1024), 0,
+         (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
+         bb.sorted_entry, bb.sorted_gauss, (uint64_t*)nullptr, (const uint32_t*)big_list,
+         (const uint32_t*)(small + 1));
 }
 
 }  // namespace splat

@@ -6,9 +6,9 @@
 //     j in the reference's flat order (0-2 mean, 3-5 log_scale, 6-9 quat, 10 opacity,
 //     11 + 3k + c SH), S = plane stride (>= N, multiple of 64).  Every kernel then reads
 //     and writes whole 256-byte wave rows.
-//   projection: three float4 records per Gaussian for the rasterizer ({mx, my, A, B},
-//     {C, opacity, sigma_max, half_x}, {half_y, r, g, b}), depth, radius, tile rectangle
-//     (uint2 {x0 | y0 << 16, x1 | y1 << 16}) and the tile-entry count.
+//   projection: one 64-byte rasterizer record per Gaussian ({mx, my, A, B},
+//     {C, opacity, sigma_max, half_x}, {half_y, r, g, b}, unused), depth, radius, tile
+//     rectangle (uint2 {x0 | y0 << 16, x1 | y1 << 16}) and the tile-entry count.
 //   binning: entry arrays (tile, slot, gaussian), per-tile counts/starts, and the sorted
 //     per-tile lists (entry id, gaussian id) that the compositing kernels read.
 #pragma once
@@ -65,7 +65,10 @@ struct DevBuf {
   }
   void* reserve(size_t b) {
     if (b > bytes) {
-      if (ptr) SPLAT_HIP(hipFree(ptr));
+      if (ptr) {
+        SPLAT_HIP(hipDeviceSynchronize());  // work in flight may still use the old buffer
+        SPLAT_HIP(hipFree(ptr));
+      }
       ptr = nullptr;
       bytes = 0;
       size_t want = b < 256 ? 256 : b;
@@ -183,6 +186,37 @@ inline CameraDev camera_dev(const splat_camera& c) {
   return d;
 }
 
+// Per-step arguments of a graph-captured train step.  Kernel arguments are frozen when a
+// graph is instantiated, so the values that change from one iteration to the next (view,
+// target image, learning rates, Adam bias corrections, loss destination) are written into
+// this device record by a small kernel launched before every replay of the graph.
+struct StepDyn {
+  CameraDev cam;
+  const uint8_t* target;  // the view's RGB8 image
+  float* loss_out;        // loss value destination (device), or null
+  splat_learning_rates lr;
+  float c1, c2;           // Adam bias corrections 1 - beta^step
+  uint32_t seq;           // step sequence number (host bookkeeping)
+  uint32_t report_slot;   // slot of the host-visible step report
+};
+
+// Device status of a graph-captured step.  The tile-entry arrays are preallocated with a
+// capacity; when a step needs more entries, `abort` is set and stays set, so that step and
+// every later one leave the model untouched until the host grows the arrays, clears the
+// flag and replays them in order.
+struct StepStatus {
+  unsigned long long total;  // tile entries of the current step (64-bit)
+  uint32_t entries;          // the same, clamped to 32 bits
+  uint32_t abort;            // sticky
+};
+
+// Host-visible record a step writes at its end (pinned, mapped memory).
+struct StepReport {
+  uint32_t seq;
+  uint32_t abort;
+  unsigned long long total;
+};
+
 // Projection outputs for one view (device pointers, capacity n).
 struct ProjBuffers {
   // One 64-byte rasterizer record per Gaussian (a single cache line per gather):
@@ -212,20 +246,26 @@ struct BinBuffers {
 // ---- kernel launchers (one per kernel file) -------------------------------------------
 
 // project.hip
+// Kernels that take `const StepDyn* dyn` read the per-step values from it when it is not
+// null (graph-captured steps) and from their other arguments otherwise.  Kernels that take
+// `const StepStatus* status` do nothing once its abort flag is set.
 void project_forward(splat_ctx* ctx, const float* params, size_t n, size_t stride,
-                     const splat_camera& cam, const splat_proj_opts& opts, ProjBuffers pb);
+                     const splat_camera& cam, const splat_proj_opts& opts, ProjBuffers pb,
+                     const StepDyn* dyn = nullptr);
 void proj_pack(splat_ctx* ctx, ProjBuffers pb, size_t n, splat_proj* out);
 void proj_unpack(splat_ctx* ctx, const splat_proj* in, size_t n, uint32_t tiles_x,
                  uint32_t tiles_y, ProjBuffers pb, uint32_t* bad_flag);
 void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stride,
                       const splat_camera& cam, const splat_proj_opts& opts,
-                      const splat_proj_grad* dproj, float* grads);
+                      const splat_proj_grad* dproj, float* grads, const StepDyn* dyn = nullptr);
 void aos_to_planes(splat_ctx* ctx, const float* aos, size_t n, float* planes, size_t stride);
 void planes_to_aos(splat_ctx* ctx, const float* planes, size_t n, size_t stride, float* aos);
 
 // scan.hip
+// `sums` (scan_temp_bytes(n) bytes) defaults to a context scratch buffer; graph-captured
+// code passes its own so the captured pointer stays valid.
 void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
-                        uint32_t* total_dev);
+                        uint32_t* total_dev, uint32_t* sums = nullptr);
 size_t scan_temp_bytes(size_t n);
 // Fill kernel (used instead of hipMemset so the same code can sit inside a HIP graph).
 void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v);
@@ -235,6 +275,14 @@ void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v);
 size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uint32_t tiles_y,
                  BinBuffers& bb, size_t* capacity_io, std::map<std::string, DevBuf>& store,
                  uint64_t* keys_out);
+// The same without a host synchronisation: the entry arrays of `bb` (capacity `capacity`)
+// and its per-Gaussian / per-tile arrays must already be allocated.  The entry count goes
+// to `status`; a count above the capacity sets the abort flag and the entry kernels exit.
+void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
+                      size_t capacity, uint32_t* small, unsigned long long* total64,
+                      uint32_t* big_list, uint32_t* scan_sums, StepStatus* status);
+void bin_guard(splat_ctx* ctx, const unsigned long long* total64, size_t capacity,
+               StepStatus* status);
 
 // raster.hip
 void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
@@ -245,7 +293,7 @@ void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb
                      const uint32_t* n_contrib, const float* dimage, bool ordered,
                      float* slots, bool need_abs);
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
-                  splat_proj_grad* grads, float* abs_grads);
+                  splat_proj_grad* grads, float* abs_grads, const StepStatus* status = nullptr);
 
 // loss.hip
 struct LossOut {
@@ -256,11 +304,12 @@ struct LossOut {
 void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* target_f32,
                            const uint8_t* target_u8, uint32_t width, uint32_t height,
                            float lambda, bool want_l1, float* dimage, float* scratch,
-                           double* loss_parts_host);
+                           double* loss_parts_host, const StepDyn* dyn = nullptr);
 size_t loss_scratch_floats(uint32_t width, uint32_t height);
-// The loss value of the last loss_forward_backward on `scratch`, into device memory.
+// The loss value of the last loss_forward_backward on `scratch`, into device memory
+// (`out_dev`, or dyn->loss_out when `dyn` is given).
 void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
-                   float lambda, float* out_dev);
+                   float lambda, float* out_dev, const StepDyn* dyn = nullptr);
 void blur_image(splat_ctx* ctx, const float* src, uint32_t width, uint32_t height,
                 uint32_t channels, float* dst);
 void gaussian_kernel_host(float k[11]);
@@ -268,7 +317,8 @@ void gaussian_kernel_host(float k[11]);
 // optim.hip
 void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, float* v,
                  size_t n, size_t stride, float c1, float c2, const splat_adam_state& st,
-                 const splat_learning_rates& lr);
+                 const splat_learning_rates& lr, const StepDyn* dyn = nullptr,
+                 const StepStatus* status = nullptr);
 size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
                       size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
                       const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
@@ -281,6 +331,6 @@ void normals_fill(splat_ctx* ctx, float* out, size_t count, uint64_t seed);
 void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius,
                           const splat_proj_grad* grads, const float* abs_grads, size_t n,
                           uint32_t width, uint32_t height, float* grad_accum, uint32_t* denom,
-                          uint32_t* max_radii);
+                          uint32_t* max_radii, const StepStatus* status = nullptr);
 
 }  // namespace splat

@@ -83,7 +83,9 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
                                                          float* __restrict__ d_mu,
                                                          float* __restrict__ d_exx,
                                                          float* __restrict__ d_exy,
-                                                         float2* __restrict__ partial) {
+                                                         float2* __restrict__ partial,
+                                                         const StepDyn* __restrict__ dyn) {
+  if (dyn) yin = dyn->target;
   __shared__ float sx[kStrip + 2 * kHalo], sy[kStrip + 2 * kHalo];
   __shared__ float red[4];
   const int width3 = (int)W * 3;
@@ -174,7 +176,9 @@ __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict_
                                                         const float* __restrict__ d_exx,
                                                         const float* __restrict__ d_exy,
                                                         float inv_n, float w1, float lambda,
-                                                        float* __restrict__ grad) {
+                                                        float* __restrict__ grad,
+                                                        const StepDyn* __restrict__ dyn) {
+  if (dyn) yin = dyn->target;
   __shared__ float s0[kStrip + 2 * kHalo], s1[kStrip + 2 * kHalo], s2[kStrip + 2 * kHalo];
   const int width3 = (int)W * 3;
   const int q0 = blockIdx.x * kStrip;
@@ -260,8 +264,11 @@ __global__ void blur_v_kernel(const float* __restrict__ tmp, uint32_t W, uint32_
 // loss = (1 - lambda) * (sum |x - y| * inv_n) + lambda * (1 - sum SSIM / n), from the
 // per-block partial sums (summed in binary64), written to *out without a host round trip.
 __global__ void loss_value_kernel(const float2* __restrict__ partial, uint32_t nb, float lambda,
-                                  float n, float* __restrict__ out) {
+                                  float n, float* __restrict__ out,
+                                  const StepDyn* __restrict__ dyn) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (dyn) out = dyn->loss_out;
+  if (!out) return;
   double a = 0, b = 0;
   for (uint32_t i = 0; i < nb; i++) {
     a += partial[i].x;
@@ -276,11 +283,11 @@ __global__ void loss_value_kernel(const float2* __restrict__ partial, uint32_t n
 }  // namespace
 
 void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
-                   float lambda, float* out_dev) {
+                   float lambda, float* out_dev, const StepDyn* dyn) {
   const size_t n = (size_t)width * height * 3;
   const unsigned nb = blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
   launch(ctx, "loss_value", loss_value_kernel, dim3(1), dim3(64), 0,
-         reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, out_dev);
+         reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, out_dev, dyn);
 }
 
 // gaussian_kernel, src/loss.rs:14-27 (host, glibc-compatible expf).
@@ -307,7 +314,7 @@ size_t loss_scratch_floats(uint32_t width, uint32_t height) {
 void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* target_f32,
                            const uint8_t* target_u8, uint32_t width, uint32_t height,
                            float lambda, bool want_l1, float* dimage, float* scratch,
-                           double* loss_parts_host) {
+                           double* loss_parts_host, const StepDyn* dyn) {
   const size_t n = (size_t)width * height * 3;
   Taps taps;
   gaussian_kernel_host(taps.k);
@@ -321,26 +328,26 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
   const float w1 = (1.0f - lambda) * inv_n;
   if (target_u8) {
     launch(ctx, "loss_ssim_stats", ssim_stats_kernel<true>, grid, dim3(256), 0, pred,
-           (const void*)target_u8, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial);
+           (const void*)target_u8, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn);
     if (want_l1)
       launch(ctx, "loss_grad", ssim_grad_kernel<true, 0>, grid, dim3(256), 0, pred,
              (const void*)target_u8, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
     else
       launch(ctx, "loss_grad", ssim_grad_kernel<true, 1>, grid, dim3(256), 0, pred,
              (const void*)target_u8, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
   } else {
     launch(ctx, "loss_ssim_stats", ssim_stats_kernel<false>, grid, dim3(256), 0, pred,
-           (const void*)target_f32, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial);
+           (const void*)target_f32, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn);
     if (want_l1)
       launch(ctx, "loss_grad", ssim_grad_kernel<false, 0>, grid, dim3(256), 0, pred,
              (const void*)target_f32, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
     else
       launch(ctx, "loss_grad", ssim_grad_kernel<false, 1>, grid, dim3(256), 0, pred,
              (const void*)target_f32, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
   }
   if (loss_parts_host) {
     const size_t nb = (size_t)grid.x * grid.y;

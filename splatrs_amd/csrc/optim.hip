@@ -31,10 +31,18 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ params,
                                                    float* __restrict__ m, float* __restrict__ v,
                                                    uint32_t n, uint32_t stride, float c1,
                                                    float c2, float b1, float b2, float eps,
-                                                   splat_learning_rates lr) {
+                                                   splat_learning_rates lr,
+                                                   const StepDyn* __restrict__ dyn,
+                                                   const StepStatus* __restrict__ status) {
   const int j = blockIdx.y;
   const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (i0 >= n) return;
+  if (status && status->abort) return;
+  if (dyn) {
+    c1 = dyn->c1;
+    c2 = dyn->c2;
+    lr = dyn->lr;
+  }
   const float rate = plane_lr(j, lr);
   const size_t base = (size_t)j * stride + i0;
   if (i0 + 4 <= n) {
@@ -62,9 +70,11 @@ __global__ void __launch_bounds__(256) stats_kernel(const uint32_t* __restrict__
                                                     uint32_t n, float sx, float sy,
                                                     float* __restrict__ grad_accum,
                                                     uint32_t* __restrict__ denom,
-                                                    uint32_t* __restrict__ max_radii) {
+                                                    uint32_t* __restrict__ max_radii,
+                                                    const StepStatus* __restrict__ status) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (status && status->abort) return;
   const uint32_t r = radius[i];
   if (r == 0) return;
   float g0, g1;
@@ -214,18 +224,19 @@ __global__ void reset_opacity_kernel(float* __restrict__ params, float* __restri
 
 void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, float* v, size_t n,
                  size_t stride, float c1, float c2, const splat_adam_state& st,
-                 const splat_learning_rates& lr) {
+                 const splat_learning_rates& lr, const StepDyn* dyn, const StepStatus* status) {
   launch(ctx, "adam", adam_kernel, dim3(blocks_for((n + 3) / 4, 256), kParams), dim3(256), 0,
          params, grads, m, v, (uint32_t)n, (uint32_t)stride, c1, c2, st.beta1, st.beta2, st.eps,
-         lr);
+         lr, dyn, status);
 }
 
 void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius, const splat_proj_grad* grads,
                           const float* abs_grads, size_t n, uint32_t width, uint32_t height,
-                          float* grad_accum, uint32_t* denom, uint32_t* max_radii) {
+                          float* grad_accum, uint32_t* denom, uint32_t* max_radii,
+                          const StepStatus* status) {
   const float sx = (float)width * 0.5f, sy = (float)height * 0.5f;
   launch(ctx, "stats", stats_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, radius, grads,
-         abs_grads, (uint32_t)n, sx, sy, grad_accum, denom, max_radii);
+         abs_grads, (uint32_t)n, sx, sy, grad_accum, denom, max_radii, status);
 }
 
 // Densify on planes.  Returns the new count; fills report.  Output planes have stride

@@ -289,9 +289,11 @@ __global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restric
                                                           uint32_t n, uint32_t stride,
                                                           CameraDev cam, float near_plane,
                                                           float eps2d, int antialias, int degree,
-                                                          ProjBuffers pb) {
+                                                          ProjBuffers pb,
+                                                          const StepDyn* __restrict__ dyn) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (dyn) cam = dyn->cam;
   float g[kParams];
   const int ncoef = (degree + 1) * (degree + 1);
 #pragma unroll
@@ -412,9 +414,11 @@ __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restric
                                                           CameraDev cam, float near_plane,
                                                           float eps2d, int antialias, int degree,
                                                           const splat_proj_grad* __restrict__ dproj,
-                                                          float* __restrict__ grads) {
+                                                          float* __restrict__ grads,
+                                                          const StepDyn* __restrict__ dyn) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (dyn) cam = dyn->cam;
   float g[kParams];
 #pragma unroll
   for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
@@ -602,10 +606,11 @@ __global__ void __launch_bounds__(256) planes_to_aos_kernel(const float* __restr
 }  // namespace
 
 void project_forward(splat_ctx* ctx, const float* params, size_t n, size_t stride,
-                     const splat_camera& cam, const splat_proj_opts& opts, ProjBuffers pb) {
+                     const splat_camera& cam, const splat_proj_opts& opts, ProjBuffers pb,
+                     const StepDyn* dyn) {
   launch(ctx, "project_fwd", project_fwd_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params,
          (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
-         opts.antialias, opts.sh_degree, pb);
+         opts.antialias, opts.sh_degree, pb, dyn);
 }
 
 void proj_pack(splat_ctx* ctx, ProjBuffers pb, size_t n, splat_proj* out) {
@@ -621,10 +626,10 @@ void proj_unpack(splat_ctx* ctx, const splat_proj* in, size_t n, uint32_t tiles_
 
 void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stride,
                       const splat_camera& cam, const splat_proj_opts& opts,
-                      const splat_proj_grad* dproj, float* grads) {
+                      const splat_proj_grad* dproj, float* grads, const StepDyn* dyn) {
   launch(ctx, "project_bwd", project_bwd_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params,
          (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
-         opts.antialias, opts.sh_degree, dproj, grads);
+         opts.antialias, opts.sh_degree, dproj, grads, dyn);
 }
 
 void aos_to_planes(splat_ctx* ctx, const float* aos, size_t n, float* planes, size_t stride) {

@@ -334,9 +334,11 @@ __global__ void __launch_bounds__(256) reduce_slots_kernel(const uint32_t* __res
                                                            const float* __restrict__ slots,
                                                            uint32_t n,
                                                            splat_proj_grad* __restrict__ grads,
-                                                           float* __restrict__ abs_grads) {
+                                                           float* __restrict__ abs_grads,
+                                                           const StepStatus* __restrict__ status) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
+  if (status && status->abort) return;
   float sum[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; k++) sum[k] = 0.0f;
@@ -395,9 +397,9 @@ void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb
 }
 
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
-                  splat_proj_grad* grads, float* abs_grads) {
+                  splat_proj_grad* grads, float* abs_grads, const StepStatus* status) {
   launch(ctx, "reduce_slots", reduce_slots_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         entry_offset, slots, (uint32_t)n, grads, abs_grads);
+         entry_offset, slots, (uint32_t)n, grads, abs_grads, status);
 }
 
 }  // namespace splat

@@ -116,10 +116,10 @@ size_t scan_temp_bytes(size_t n) {
 }
 
 void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
-                        uint32_t* total_dev) {
+                        uint32_t* total_dev, uint32_t* sums) {
   const unsigned nb = blocks_for(n == 0 ? 1 : n, kScanTile);
   SPLAT_REQUIRE(nb <= (unsigned)(kTopThreads * kTopItems), "scan: input too large");
-  uint32_t* sums = ctx->buf<uint32_t>("scan_block_sums", nb + 1);
+  if (!sums) sums = ctx->buf<uint32_t>("scan_block_sums", nb + 1);
   launch(ctx, "scan_reduce", scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, in, n, sums);
   launch(ctx, "scan_top", scan_top_kernel, dim3(1), dim3(kTopThreads), 0, sums, nb, total_dev);
   launch(ctx, "scan_apply", scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, in, n,

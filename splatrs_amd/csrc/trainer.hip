@@ -1,11 +1,22 @@
 // Device-resident training (src/train.rs:196-303): the model, its gradient, both Adam
-// moments, the densification statistics and every view's image stay in HBM; a step
-// launches project -> bin -> composite -> loss -> backward -> statistics -> Adam on the
-// context stream.  The only host synchronisations are reading the tile-entry count (to
-// size the entry arrays) and, on request, the loss value.
+// moments, the densification statistics and every view's image stay in HBM; a step runs
+// project -> bin -> composite -> loss -> backward -> statistics -> Adam on the context
+// stream.
+//
+// Two ways to run a step:
+//   * graph (default): the step's launches are captured once into a HIP graph (keyed by
+//     every buffer and size it bakes in) and replayed each iteration.  The per-iteration
+//     values go through a small device record written just before the replay.  The entry
+//     arrays have a preallocated capacity; a step that needs more sets a sticky abort flag
+//     (nothing of the model changes), and the host, which checks each step's report two
+//     steps behind, grows the arrays and replays the aborted steps in order.  No host
+//     synchronisation sits inside a step.
+//   * eager: the same kernels launched one by one, reading the entry count back to size
+//     the entry arrays (used for per-kernel event timing).
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <new>
 
@@ -16,6 +27,36 @@ using namespace splat;
 struct TrainerView {
   splat_camera cam;
   DevBuf rgb;
+};
+
+// Everything a captured step bakes in: sizes, flags and device buffers.  Two steps with
+// byte-identical plans can replay the same graph (every buffer in the plan is reserved to
+// at least the sizes in the plan before the replay).
+struct StepPlan {
+  uint64_t n, stride, entry_cap;
+  uint32_t width, height;
+  int32_t sh_degree, antialias, ordered, accumulate, absgrad, pad0;
+  float near_plane, lambda, bg[3], pad1;
+  float *params, *m, *v, *grads, *grad_accum;
+  uint32_t *denom, *max_radii;
+  ProjBuffers pb;
+  BinBuffers bb;
+  float *image, *final_t, *dimage, *loss_scratch, *slots, *absg, *loss_one;
+  uint32_t *n_contrib, *small, *big_list, *scan_sums;
+  unsigned long long* total64;
+  splat_proj_grad* pgrad;
+  StepDyn* dyn;
+  StepStatus* status;
+  StepReport* reports;
+};
+
+struct PendingStep {
+  uint64_t seq;
+  uint32_t view;
+  splat_step_params p;
+  uint64_t adam_step;
+  float* loss_out;
+  hipEvent_t done;
 };
 
 struct splat_trainer {
@@ -29,9 +70,30 @@ struct splat_trainer {
   std::map<std::string, DevBuf> store;  // per-view buffers (projection, binning, images)
   size_t entry_cap = 0;
   float* loss_target = nullptr;  // when set, each step writes its loss value here (device)
+  // graph-captured steps
+  bool use_graph = true;
+  uint64_t seq = 0;
+  std::deque<PendingStep> pending;
+  std::vector<std::pair<StepPlan, hipGraphExec_t>> graphs;
+  std::vector<hipEvent_t> events;
+  StepReport* reports_host = nullptr;  // pinned, mapped; written by the step's last kernel
+  StepReport* reports_dev = nullptr;
+  unsigned long long last_total = 0;
+  uint64_t graph_captures = 0, graph_replays = 0, step_aborts = 0;
 };
 
 namespace {
+
+void drain(splat_trainer* t);
+
+// After a failed call, finish the steps in flight; if that fails too, forget them.
+void settle(splat_trainer* t) {
+  try {
+    drain(t);
+  } catch (...) {
+    t->pending.clear();
+  }
+}
 
 template <class F>
 int tguard(splat_trainer* t, F&& f) {
@@ -43,12 +105,15 @@ int tguard(splat_trainer* t, F&& f) {
     ctx->err.clear();
     return SPLAT_OK;
   } catch (const Error& e) {
+    settle(t);
     ctx->err = e.msg;
     return e.code;
   } catch (const std::bad_alloc&) {
+    settle(t);
     ctx->err = "host allocation failed";
     return SPLAT_ERR_OOM;
   } catch (const std::exception& e) {
+    settle(t);
     ctx->err = e.what();
     return SPLAT_ERR_INVALID;
   }
@@ -113,6 +178,21 @@ __global__ void __launch_bounds__(256) count_visible_kernel(const uint32_t* __re
   if ((threadIdx.x & 63) == 0 && mask) atomicAdd(count, (uint32_t)__popcll(mask));
 }
 
+__global__ void set_dyn_kernel(StepDyn d, StepDyn* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = d;
+}
+
+__global__ void step_report_kernel(const StepDyn* __restrict__ dyn,
+                                   const StepStatus* __restrict__ status,
+                                   StepReport* __restrict__ reports) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  StepReport r;
+  r.seq = dyn->seq;
+  r.abort = status->abort;
+  r.total = status->total;
+  reports[dyn->report_slot] = r;
+}
+
 }  // namespace
 
 namespace splat {
@@ -121,6 +201,240 @@ void normals_fill(splat_ctx* ctx, float* out, size_t count, uint64_t seed) {
          seed);
 }
 }  // namespace splat
+
+namespace {
+
+constexpr uint32_t kReportSlots = 64;
+constexpr size_t kStepLag = 2;     // steps in flight before the host checks the oldest
+constexpr size_t kMaxGraphs = 8;   // cached step graphs per trainer
+
+float adam_c(float beta, uint64_t step) { return 1.0f - powi(beta, (int32_t)step); }
+
+splat_proj_opts step_opts(const splat_step_params& p) {
+  splat_proj_opts o;
+  o.near_plane = p.near_plane;
+  o.eps2d = 0.3f;
+  o.antialias = p.antialias;
+  o.sh_degree = p.sh_degree;
+  return o;
+}
+
+// Reserves every buffer of a graph-captured step and returns its plan.
+StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_params& p) {
+  StepPlan P;
+  std::memset(&P, 0, sizeof(P));
+  auto& st = t->store;
+  const uint32_t W = cam.width, H = cam.height;
+  const size_t npx = (size_t)W * H;
+  const uint32_t tx = (W + kTile - 1) / kTile, ty = (H + kTile - 1) / kTile;
+  const size_t T = (size_t)tx * ty;
+  const size_t n = t->n;
+  if (t->entry_cap == 0) t->entry_cap = std::max<size_t>(4 * n, 1 << 16);
+  const size_t ec = t->entry_cap;
+  P.n = n;
+  P.stride = t->stride;
+  P.entry_cap = ec;
+  P.width = W;
+  P.height = H;
+  P.sh_degree = p.sh_degree;
+  P.antialias = p.antialias;
+  P.ordered = p.ordered_backward != 0;
+  P.accumulate = p.accumulate_stats != 0;
+  P.absgrad = p.absgrad != 0;
+  P.near_plane = p.near_plane;
+  P.lambda = p.lambda_dssim;
+  for (int c = 0; c < 3; c++) P.bg[c] = p.background[c];
+  P.params = t->params.as<float>();
+  P.m = t->m.as<float>();
+  P.v = t->v.as<float>();
+  P.grads = t->grads.as<float>();
+  P.grad_accum = t->grad_accum.as<float>();
+  P.denom = t->denom.as<uint32_t>();
+  P.max_radii = t->max_radii.as<uint32_t>();
+  P.pb = proj_buffers(t);
+  BinBuffers& bb = P.bb;
+  bb.tiles_x = tx;
+  bb.tiles_y = ty;
+  bb.entry_offset = (uint32_t*)st["entry_offset"].reserve((t->cap + 1) * 4);
+  bb.tile_count = (uint32_t*)st["tile_count"].reserve(T * 4);
+  bb.tile_start = (uint32_t*)st["tile_start"].reserve((T + 1) * 4);
+  bb.entry_gaussian = (uint32_t*)st["entry_gaussian"].reserve(ec * 4);
+  bb.entry_tile = (uint32_t*)st["entry_tile"].reserve(ec * 4);
+  bb.entry_slot = (uint32_t*)st["entry_slot"].reserve(ec * 4);
+  bb.bucket = (uint64_t*)st["bucket"].reserve(ec * 8);
+  bb.sorted_entry = (uint32_t*)st["sorted_entry"].reserve(ec * 4);
+  bb.sorted_gauss = (uint32_t*)st["sorted_gauss"].reserve(ec * 4);
+  P.small = (uint32_t*)st["bin_small"].reserve(64);
+  P.total64 = (unsigned long long*)st["bin_total64"].reserve(8);
+  P.big_list = (uint32_t*)st["bin_big_list"].reserve(T * 4);
+  P.scan_sums = (uint32_t*)st["step_scan_sums"].reserve(scan_temp_bytes(std::max(t->cap + 1, T)));
+  P.image = (float*)st["image"].reserve(npx * 12 + 4);
+  P.final_t = (float*)st["final_t"].reserve(npx * 4 + 4);
+  P.n_contrib = (uint32_t*)st["n_contrib"].reserve(npx * 4 + 4);
+  P.dimage = (float*)st["dimage"].reserve(npx * 12 + 4);
+  P.loss_scratch = (float*)st["loss_scratch"].reserve(loss_scratch_floats(W, H) * 4);
+  P.slots = (float*)st["slots"].reserve((ec + 1) * kSlots * 4);
+  P.pgrad = (splat_proj_grad*)st["pgrad"].reserve((t->cap + 1) * sizeof(splat_proj_grad));
+  P.absg = (float*)st["absg"].reserve((t->cap + 1) * 8);
+  P.loss_one = (float*)st["step_loss"].reserve(4);
+  P.dyn = (StepDyn*)st["step_dyn"].reserve(sizeof(StepDyn));
+  P.status = (StepStatus*)st["step_status"].reserve(sizeof(StepStatus));
+  P.reports = t->reports_dev;
+  return P;
+}
+
+// The launches of one step, reading per-iteration values from P.dyn.
+void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target) {
+  splat_ctx* ctx = t->ctx;
+  const size_t n = P.n;
+  splat_camera cam0;
+  std::memset(&cam0, 0, sizeof(cam0));
+  cam0.width = P.width;
+  cam0.height = P.height;
+  splat_proj_opts opts;
+  opts.near_plane = P.near_plane;
+  opts.eps2d = 0.3f;
+  opts.antialias = P.antialias;
+  opts.sh_degree = P.sh_degree;
+  project_forward(ctx, P.params, n, P.stride, cam0, opts, P.pb, P.dyn);
+  bin_build_device(ctx, P.pb, n, P.bb, P.entry_cap, P.small, P.total64, P.big_list, P.scan_sums,
+                   P.status);
+  raster_forward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.image, P.final_t, P.n_contrib);
+  loss_forward_backward(ctx, P.image, nullptr, any_target, P.width, P.height, P.lambda, true,
+                        P.dimage, P.loss_scratch, nullptr, P.dyn);
+  loss_value_to(ctx, P.loss_scratch, P.width, P.height, P.lambda, nullptr, P.dyn);
+  raster_backward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.final_t, P.n_contrib, P.dimage,
+                  P.ordered != 0, P.slots, P.accumulate && P.absgrad);
+  reduce_slots(ctx, P.bb.entry_offset, P.slots, n, P.pgrad, P.absg, P.status);
+  project_backward(ctx, P.params, n, P.stride, cam0, opts, P.pgrad, P.grads, P.dyn);
+  if (P.accumulate)
+    stats_accumulate_dev(ctx, P.pb.radius, P.pgrad, P.absgrad ? P.absg : nullptr, n, P.width,
+                         P.height, P.grad_accum, P.denom, P.max_radii, P.status);
+  splat_adam_state ast{0.9f, 0.999f, 1e-15f, 0, 0};
+  splat_learning_rates lr0;
+  std::memset(&lr0, 0, sizeof(lr0));
+  adam_planes(ctx, P.params, P.grads, P.m, P.v, n, P.stride, 1.0f, 1.0f, ast, lr0, P.dyn,
+              P.status);
+  launch(ctx, "step_report", step_report_kernel, dim3(1), dim3(64), 0, (const StepDyn*)P.dyn,
+         (const StepStatus*)P.status, P.reports);
+}
+
+hipGraphExec_t graph_for(splat_trainer* t, const StepPlan& P, const uint8_t* any_target) {
+  for (auto& g : t->graphs)
+    if (std::memcmp(&g.first, &P, sizeof(P)) == 0) return g.second;
+  splat_ctx* ctx = t->ctx;
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  SPLAT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+  try {
+    enqueue_step(t, P, any_target);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(ctx->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    ctx->timing = timing;
+    throw;
+  }
+  ctx->timing = timing;
+  hipGraph_t g = nullptr;
+  SPLAT_HIP(hipStreamEndCapture(ctx->stream, &g));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  SPLAT_HIP(e);
+  if (t->graphs.size() >= kMaxGraphs) {
+    (void)hipGraphExecDestroy(t->graphs.front().second);
+    t->graphs.erase(t->graphs.begin());
+  }
+  t->graphs.emplace_back(P, exec);
+  t->graph_captures++;
+  return exec;
+}
+
+void ensure_reports(splat_trainer* t) {
+  if (t->reports_host) return;
+  void* h = nullptr;
+  SPLAT_HIP(hipHostMalloc(&h, kReportSlots * sizeof(StepReport),
+                          hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(h, 0xff, kReportSlots * sizeof(StepReport));
+  t->reports_host = (StepReport*)h;
+  void* d = nullptr;
+  SPLAT_HIP(hipHostGetDevicePointer(&d, h, 0));
+  t->reports_dev = (StepReport*)d;
+}
+
+void clear_abort(splat_trainer* t) {
+  StepStatus* st = (StepStatus*)t->store["step_status"].reserve(sizeof(StepStatus));
+  fill_u32(t->ctx, reinterpret_cast<uint32_t*>(st), sizeof(StepStatus) / 4, 0u);
+}
+
+void launch_graph_step(splat_trainer* t, uint32_t view, const splat_step_params& p,
+                       uint64_t adam_step, float* loss_out) {
+  splat_ctx* ctx = t->ctx;
+  ensure_reports(t);
+  const bool fresh = !t->store.count("step_status");
+  const TrainerView& tv = *t->views[view];
+  const StepPlan P = plan_step(t, tv.cam, p);
+  if (fresh) clear_abort(t);
+  hipGraphExec_t exec = graph_for(t, P, tv.rgb.as<uint8_t>());
+  StepDyn d;
+  std::memset(&d, 0, sizeof(d));
+  d.cam = camera_dev(tv.cam);
+  d.target = tv.rgb.as<uint8_t>();
+  d.loss_out = loss_out ? loss_out : P.loss_one;
+  d.lr = p.lr;
+  d.c1 = adam_c(0.9f, adam_step);
+  d.c2 = adam_c(0.999f, adam_step);
+  d.seq = (uint32_t)t->seq;
+  d.report_slot = (uint32_t)(t->seq % kReportSlots);
+  launch(ctx, "step_args", set_dyn_kernel, dim3(1), dim3(64), 0, d, P.dyn);
+  SPLAT_HIP(hipGraphLaunch(exec, ctx->stream));
+  t->graph_replays++;
+  hipEvent_t ev;
+  if (t->events.empty()) {
+    SPLAT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  } else {
+    ev = t->events.back();
+    t->events.pop_back();
+  }
+  SPLAT_HIP(hipEventRecord(ev, ctx->stream));
+  t->pending.push_back(PendingStep{t->seq, view, p, adam_step, loss_out, ev});
+  t->seq++;
+}
+
+// Waits for the oldest step in flight and checks its report; an aborted step (entry
+// arrays too small) grows the arrays and replays it and every later step in order.
+void verify_front(splat_trainer* t) {
+  PendingStep pe = t->pending.front();
+  SPLAT_HIP(hipEventSynchronize(pe.done));
+  const volatile StepReport* r = t->reports_host + pe.seq % kReportSlots;
+  const uint32_t seq = r->seq, abort = r->abort;
+  const unsigned long long total = r->total;
+  if (seq != (uint32_t)pe.seq) fail(SPLAT_ERR_HIP, "step report missing");
+  if (!abort) {
+    t->last_total = total;
+    t->events.push_back(pe.done);
+    t->pending.pop_front();
+    return;
+  }
+  t->step_aborts++;
+  SPLAT_HIP(hipStreamSynchronize(t->ctx->stream));
+  SPLAT_REQUIRE(total < 0xffffffffull, "too many tile entries");
+  t->entry_cap = std::max<size_t>(t->entry_cap, (size_t)(total + total / 4 + 1024));
+  clear_abort(t);
+  std::deque<PendingStep> redo;
+  redo.swap(t->pending);
+  for (auto& q : redo) {
+    t->events.push_back(q.done);
+    launch_graph_step(t, q.view, q.p, q.adam_step, q.loss_out);
+  }
+}
+
+void drain(splat_trainer* t) {
+  while (!t->pending.empty()) verify_front(t);
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -161,6 +475,10 @@ int splat_trainer_destroy(splat_trainer* t) {
   // The context may already be gone: use only the device id captured at creation.
   (void)hipSetDevice(t->device);
   (void)hipDeviceSynchronize();
+  for (auto& g : t->graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto& pe : t->pending) (void)hipEventDestroy(pe.done);
+  for (hipEvent_t e : t->events) (void)hipEventDestroy(e);
+  if (t->reports_host) (void)hipHostFree(t->reports_host);
   delete t;
   return SPLAT_OK;
 }
@@ -189,6 +507,32 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     SPLAT_REQUIRE(p && view < t->views.size(), "bad view index");
     SPLAT_REQUIRE(p->sh_degree >= 0 && p->sh_degree <= 3, "sh_degree must be 0..3");
     splat_ctx* ctx = t->ctx;
+    if (t->use_graph) {
+      t->adam_step += 1;
+      launch_graph_step(t, view, *p, t->adam_step, t->loss_target);
+      while (t->pending.size() > kStepLag) verify_front(t);
+      if (!r) return;
+      drain(t);
+      r->num_entries = t->last_total;
+      r->num_visible = 0;
+      r->loss = 0.0f;
+      if (p->want_loss) {
+        const float* lp = t->loss_target ? t->loss_target : (const float*)t->store["step_loss"].ptr;
+        uint32_t* cnt = (uint32_t*)t->store["visible_count"].reserve(4);
+        fill_u32(ctx, cnt, 1, 0u);
+        launch(ctx, "count_visible", count_visible_kernel, dim3(blocks_for(t->n, 256)), dim3(256),
+               0, (const uint32_t*)t->store["radius"].ptr, (uint32_t)t->n, cnt);
+        uint32_t vis = 0;
+        float loss = 0.0f;
+        SPLAT_HIP(hipMemcpyAsync(&vis, cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SPLAT_HIP(hipMemcpyAsync(&loss, lp, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+        r->num_visible = vis;
+        r->loss = loss;
+      }
+      return;
+    }
+    drain(t);
     const TrainerView& tv = *t->views[view];
     const uint32_t W = tv.cam.width, H = tv.cam.height;
     const size_t npx = (size_t)W * H;
@@ -271,6 +615,7 @@ int splat_trainer_densify(splat_trainer* t, const splat_densify_config* cfg, flo
                           uint64_t seed, splat_densify_report* report) {
   return tguard(t, [&] {
     SPLAT_REQUIRE(cfg && report, "null argument");
+    drain(t);
     splat_ctx* ctx = t->ctx;
     const size_t n = t->n;
     *report = splat_densify_report{0, 0, 0};
@@ -312,6 +657,7 @@ int splat_trainer_densify(splat_trainer* t, const splat_densify_config* cfg, flo
 
 int splat_trainer_reset_opacity(splat_trainer* t, float max_alpha) {
   return tguard(t, [&] {
+    drain(t);
     const float limit = logf(max_alpha / (1.0f - max_alpha));  // inverse_sigmoid
     reset_opacity_planes(t->ctx, t->params.as<float>(), t->m.as<float>(), t->v.as<float>(), t->n,
                          t->stride, limit);
@@ -323,6 +669,7 @@ int splat_trainer_render(splat_trainer* t, uint32_t view, int32_t sh_degree, int
   return tguard(t, [&] {
     SPLAT_REQUIRE(view < t->views.size() && background && image, "bad argument");
     SPLAT_REQUIRE(sh_degree >= 0 && sh_degree <= 3, "sh_degree must be 0..3");
+    drain(t);
     splat_ctx* ctx = t->ctx;
     const splat_camera& cam = t->views[view]->cam;
     const uint32_t W = cam.width, H = cam.height;
@@ -342,6 +689,37 @@ int splat_trainer_render(splat_trainer* t, uint32_t view, int32_t sh_degree, int
   });
 }
 
+int splat_trainer_set_graph(splat_trainer* t, int32_t enabled) {
+  return tguard(t, [&] {
+    drain(t);
+    t->use_graph = enabled != 0;
+  });
+}
+
+int splat_trainer_sync(splat_trainer* t) {
+  return tguard(t, [&] {
+    drain(t);
+    SPLAT_HIP(hipStreamSynchronize(t->ctx->stream));
+  });
+}
+
+int splat_trainer_reserve_entries(splat_trainer* t, uint64_t entries) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(entries > 0 && entries < 0xffffffffull, "bad entry capacity");
+    drain(t);
+    t->entry_cap = (size_t)entries;
+  });
+}
+
+int splat_trainer_graph_stats(splat_trainer* t, uint64_t* captures, uint64_t* replays,
+                              uint64_t* aborts) {
+  if (!t) return SPLAT_ERR_INVALID;
+  if (captures) *captures = t->graph_captures;
+  if (replays) *replays = t->graph_replays;
+  if (aborts) *aborts = t->step_aborts;
+  return SPLAT_OK;
+}
+
 int splat_trainer_info(splat_trainer* t, uint64_t* n, uint64_t* adam_step) {
   if (!t) return SPLAT_ERR_INVALID;
   if (n) *n = t->n;
@@ -354,6 +732,7 @@ int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* 
                            uint64_t capacity) {
   return tguard(t, [&] {
     SPLAT_REQUIRE(capacity >= t->n, "output capacity too small");
+    drain(t);
     splat_ctx* ctx = t->ctx;
     const size_t n = t->n;
     if (n == 0) return;
@@ -495,8 +874,8 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
   int status = SPLAT_OK;
   const int st = tguard(t, [&] {
     const uint32_t log_iv = cfg->log_interval;
-    DevBuf loss_hist;
-    float* hist = (float*)loss_hist.reserve((size_t)(log_iv ? log_iv : 1) * 4 + 4);
+    // Kept in the trainer's store: steps still in flight write into it.
+    float* hist = (float*)t->store["loss_hist"].reserve((size_t)(log_iv ? log_iv : 1) * 4 + 4);
     ViewOrder order(cfg->seed, (uint32_t)t->views.size());
     const float bg = cfg->white_background ? 1.0f : 0.0f;
     int32_t sh = 0;  // model.sh_degree starts at 0 (src/train.rs:186-187)
@@ -563,6 +942,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
       if (log_now) {
         // EMA over every iteration's loss (src/train.rs:274-278), read back in one copy.
         losses.resize(since_log);
+        drain(t);
         SPLAT_HIP(hipMemcpyAsync(losses.data(), hist, since_log * 4, hipMemcpyDeviceToHost,
                                  ctx->stream));
         SPLAT_HIP(hipStreamSynchronize(ctx->stream));
@@ -602,6 +982,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
       for (uint32_t k = 0; k < cfg->num_save_iterations; k++)
         if (cfg->save_iterations && cfg->save_iterations[k] == it) save = true;
       if (save && it != cfg->iterations && on_save) {
+        drain(t);
         SPLAT_HIP(hipStreamSynchronize(ctx->stream));
         if (on_save(it, t, user) != 0) {
           status = SPLAT_ERR_IO;
@@ -609,6 +990,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
         }
       }
     }
+    drain(t);
     SPLAT_HIP(hipStreamSynchronize(ctx->stream));
     if (on_save && on_save(cfg->iterations, t, user) != 0) status = SPLAT_ERR_IO;
   });

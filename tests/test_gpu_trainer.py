@@ -151,3 +151,45 @@ def test_trainer_densify_reset_and_render(ctx):
     r = tr.step(0, rec.step_params(1, lr, want_loss=True))
     assert np.isfinite(float(r["loss"]))
     tr.close()
+
+
+def test_graph_steps_equal_eager_steps(ctx):
+    """Graph-captured steps (with entry arrays far too small at first, so steps abort, the
+    arrays grow and the aborted steps are replayed) give the same model as steps launched
+    one by one: 24 iterations with a degree ramp, AbsGS statistics, a densification in the
+    middle (new graph) and an opacity reset."""
+    g, cams, views = small_training_problem(ctx)
+    extent = scenes.scene_extent(cams)
+    eager, graph = abi.Trainer(ctx, g), abi.Trainer(ctx, g)
+    eager.set_graph(False)
+    graph.reserve_entries(1024)
+    for c, v in zip(cams, views):
+        eager.add_view(c, v)
+        graph.add_view(c, v)
+    for it in range(1, 25):
+        view = (it * 5) % len(cams)
+        deg = min(3, it // 6)
+        lr = rec.learning_rates(rec.position_lr(it, extent))
+        p = rec.step_params(deg, lr, accumulate_stats=True, absgrad=it > 8,
+                            ordered_backward=True)
+        say(f"graph vs eager step {it} view {view}")
+        eager.step(view, p)
+        graph.step(view, p, result=(it % 7 == 0))
+        if it == 12:
+            cfg = rec.densify_config(grad_threshold=0.0001)
+            ra = eager.densify(cfg, extent, 77)
+            rb = graph.densify(cfg, extent, 77)
+            assert ra.tobytes() == rb.tobytes()
+        if it == 18:
+            eager.reset_opacity(0.01)
+            graph.reset_opacity(0.01)
+    a = eager.download(moments=True, stats=True)
+    b = graph.download(moments=True, stats=True)
+    for k in a:
+        assert np.array_equal(np.asarray(a[k]).view(np.uint8), np.asarray(b[k]).view(np.uint8)), k
+    captures, replays, aborts = graph.graph_stats()
+    say(f"graph captures {captures}, replays {replays}, aborted steps {aborts}")
+    assert aborts >= 1 and captures >= 3 and replays >= 24 + aborts
+    ea = eager.render(0, 3)
+    eb = graph.render(0, 3)
+    assert np.array_equal(ea, eb)
