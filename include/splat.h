@@ -413,6 +413,45 @@ int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* 
                            float* grad_accum, uint32_t* denom, uint32_t* max_radii,
                            uint64_t capacity);
 
+/* ---- learned densification scorer --------------------------------------------------
+ * No counterpart in the reference, whose densification is the threshold heuristic
+ * (src/densify.rs:125-221).  A small MLP scores each Gaussian from 12 statistics:
+ * 12 -> 64 -> 64 -> 64 (tanh GELU) -> 3 logits (clone, split, prune), bfloat16 weights and
+ * activations on the matrix cores with binary32 accumulation, binary32 master weights.
+ * Parameters (SPLAT_SCORER_PARAMS floats): W1[64][12] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[3][64] b4[3].  Labels are bit masks (1 clone, 2 split, 4 prune);
+ * the training loss is the weighted binary cross-entropy of the three logits, averaged over
+ * the minibatch.  For callers that want a learned densification policy. */
+#define SPLAT_SCORER_PARAMS 9347
+#define SPLAT_SCORER_FEATURES 12
+
+typedef struct splat_scorer splat_scorer;
+
+typedef struct {
+  uint32_t epochs;   /* passes over the samples */
+  uint32_t batch;    /* minibatch size (the last partial minibatch of an epoch is skipped) */
+  float lr;          /* Adam learning rate (betas 0.9, 0.999, eps 1e-8) */
+  uint32_t pad_;
+  uint64_t seed;     /* per-epoch sample permutation */
+} splat_scorer_fit_config;
+
+/* params: SPLAT_SCORER_PARAMS floats, or NULL for a seeded N(0, 1/fan_in) initialisation. */
+int splat_scorer_create(splat_ctx* ctx, const float* params, uint64_t seed, splat_scorer** out);
+int splat_scorer_destroy(splat_scorer* s);
+int splat_scorer_get_params(splat_scorer* s, float* params);
+/* Replaces the weights and resets the scorer's Adam state. */
+int splat_scorer_set_params(splat_scorer* s, const float* params);
+/* logits[n][3] of features[n][12] (host arrays). */
+int splat_scorer_forward(splat_scorer* s, const float* features, uint64_t n, float* logits);
+/* Gradient (SPLAT_SCORER_PARAMS floats) and value of the loss over the n samples. */
+int splat_scorer_loss_grad(splat_scorer* s, const float* features, const uint8_t* labels,
+                           const float* weights, uint64_t n, float* grad, float* loss);
+void splat_scorer_fit_config_default(splat_scorer_fit_config* cfg);
+/* Minibatch Adam over the samples; *loss = mean minibatch loss of the last epoch. */
+int splat_scorer_fit(splat_scorer* s, const float* features, const uint8_t* labels,
+                     const float* weights, uint64_t n, const splat_scorer_fit_config* cfg,
+                     float* loss);
+
 #ifdef __cplusplus
 }
 #endif

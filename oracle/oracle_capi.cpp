@@ -9,6 +9,7 @@
 
 #include "../include/splat.h"
 #include "splat_oracle.hpp"
+#include "mlp_oracle.hpp"
 
 using namespace oracle;
 
@@ -425,6 +426,51 @@ double oracle_chain_loss_f64(const double* g, uint64_t n, const double* cam, con
     for (uint64_t i = 0; i < n; i++) project_backward_one<double>(g + i * 59, c, o, pg[i], grads + i * 59);
   }
   return loss;
+}
+
+
+// ---------------------------------------------------------------- scorer MLP -------
+
+void oracle_mlp_forward_f32(const float* params, const float* feats, int64_t n, float* logits,
+                            int bf16) {
+  oracle::mlp::Trace<float> t;
+  for (int64_t s = 0; s < n; s++) {
+    if (bf16)
+      oracle::mlp::forward<float, true>(params, feats + s * 12, t);
+    else
+      oracle::mlp::forward<float, false>(params, feats + s * 12, t);
+    for (int o = 0; o < 3; o++) logits[s * 3 + o] = t.z4[o];
+  }
+}
+
+float oracle_mlp_loss_grad_f32(const float* params, const float* feats, const uint8_t* labels,
+                               const float* weights, int64_t n, float* grad, int bf16) {
+  if (bf16)
+    return oracle::mlp::loss_and_grad<float, true>(params, feats, labels, weights, n, grad);
+  return oracle::mlp::loss_and_grad<float, false>(params, feats, labels, weights, n, grad);
+}
+
+void oracle_mlp_forward_f64(const double* params, const double* feats, int64_t n,
+                            double* logits) {
+  oracle::mlp::Trace<double> t;
+  for (int64_t s = 0; s < n; s++) {
+    oracle::mlp::forward<double, false>(params, feats + s * 12, t);
+    for (int o = 0; o < 3; o++) logits[s * 3 + o] = t.z4[o];
+  }
+}
+
+double oracle_mlp_loss_grad_f64(const double* params, const double* feats, const uint8_t* labels,
+                                const double* weights, int64_t n, double* grad) {
+  return oracle::mlp::loss_and_grad<double, false>(params, feats, labels, weights, n, grad);
+}
+
+void oracle_bf16_round(const float* in, float* out, int64_t n) {
+  for (int64_t i = 0; i < n; i++) out[i] = oracle::mlp::bf16_round(in[i]);
+}
+
+void oracle_mlp_adam(float* p, const float* g, float* m, float* v, int n, float lr, float b1,
+                     float b2, float eps, float c1, float c2) {
+  oracle::mlp::adam(p, g, m, v, n, lr, b1, b2, eps, c1, c2);
 }
 
 }  // extern "C"

@@ -172,6 +172,14 @@ _SIGS = {
     "splat_trainer_graph_stats": (i32, [P, P, P, P]),
     "splat_trainer_reserve_entries": (i32, [P, u64]),
     "splat_trainer_sync": (i32, [P]),
+    "splat_scorer_create": (i32, [P, P, u64, P]),
+    "splat_scorer_destroy": (i32, [P]),
+    "splat_scorer_get_params": (i32, [P, P]),
+    "splat_scorer_set_params": (i32, [P, P]),
+    "splat_scorer_forward": (i32, [P, P, u64, P]),
+    "splat_scorer_loss_grad": (i32, [P, P, P, P, u64, P, P]),
+    "splat_scorer_fit_config_default": (None, [P]),
+    "splat_scorer_fit": (i32, [P, P, P, P, u64, P, P]),
     "splat_trainer_download": (i32, [P, P, P, P, P, P, P, u64]),
 }
 
@@ -219,7 +227,7 @@ class Context:
         return self._h
 
     def close(self):
-        for t in list(self._trainers):  # trainers run on this context's stream
+        for t in list(self._trainers):  # trainers and scorers use this context's stream
             t.close()
         if self._h:
             lib().splat_ctx_destroy(self._h)
@@ -577,3 +585,66 @@ class Trainer:
         if stats:
             out.update(grad_accum=acc, denom=den, max_radii=mr)
         return out
+
+
+class Scorer:
+    """Learned densification scorer (splat_scorer_*): 12 -> 3x64 GELU -> 3 logits on the
+    matrix cores (bfloat16 operands, binary32 accumulation)."""
+
+    def __init__(self, ctx, params=None, seed=0):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        ctx.check(lib().splat_scorer_create(ctx.handle, _p(p), seed, C.byref(self._h)),
+                  "scorer_create")
+        ctx._trainers.add(self)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().splat_scorer_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.zeros(rec.SCORER_PARAMS, np.float32)
+        self.ctx.check(lib().splat_scorer_get_params(self._h, _p(out)), "scorer_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self.ctx.check(lib().splat_scorer_set_params(self._h, _p(p)), "scorer_set_params")
+
+    def forward(self, feats):
+        x = np.ascontiguousarray(feats, np.float32)
+        out = np.zeros((len(x), 3), np.float32)
+        self.ctx.check(lib().splat_scorer_forward(self._h, _p(x), len(x), _p(out)),
+                       "scorer_forward")
+        return out
+
+    def loss_grad(self, feats, labels, weights):
+        x = np.ascontiguousarray(feats, np.float32)
+        y = np.ascontiguousarray(labels, np.uint8)
+        w = np.ascontiguousarray(weights, np.float32)
+        g = np.zeros(rec.SCORER_PARAMS, np.float32)
+        loss = C.c_float()
+        self.ctx.check(lib().splat_scorer_loss_grad(self._h, _p(x), _p(y), _p(w), len(x), _p(g),
+                                                    C.byref(loss)), "scorer_loss_grad")
+        return loss.value, g
+
+    def fit(self, feats, labels, weights, cfg):
+        x = np.ascontiguousarray(feats, np.float32)
+        y = np.ascontiguousarray(labels, np.uint8)
+        w = np.ascontiguousarray(weights, np.float32)
+        loss = C.c_float()
+        self.ctx.check(lib().splat_scorer_fit(self._h, _p(x), _p(y), _p(w), len(x), _p(cfg),
+                                              C.byref(loss)), "scorer_fit")
+        return loss.value

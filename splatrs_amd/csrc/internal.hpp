@@ -18,6 +18,8 @@
 
 #include <algorithm>
 #include <map>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -155,6 +157,27 @@ inline void launch(splat_ctx* ctx, const char* name, K kernel, dim3 grid, dim3 b
     SPLAT_HIP(hipEventRecord(b, ctx->stream));
     ctx->pending.push_back({name, a, b});
     if (ctx->pending.size() > 4096) ctx->drain_timing();
+  }
+}
+
+// Runs f for a C-ABI entry point: HIP device selection, errors -> status + message.
+template <class F>
+int ctx_guard(splat_ctx* ctx, F&& f) {
+  if (!ctx) return SPLAT_ERR_INVALID;
+  try {
+    SPLAT_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->err.clear();
+    return SPLAT_OK;
+  } catch (const Error& e) {
+    ctx->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    ctx->err = "host allocation failed";
+    return SPLAT_ERR_OOM;
+  } catch (const std::exception& e) {
+    ctx->err = e.what();
+    return SPLAT_ERR_INVALID;
   }
 }
 

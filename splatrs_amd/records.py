@@ -35,6 +35,10 @@ DENSIFY_CONFIG = np.dtype([
     ("has_max_screen_size", "<i4"), ("max_screen_size", "<u4"), ("split_children", "<u4"),
 ])
 DENSIFY_REPORT = np.dtype([("cloned", "<u8"), ("split", "<u8"), ("pruned", "<u8")])
+SCORER_FIT_CONFIG = np.dtype([("epochs", "<u4"), ("batch", "<u4"), ("lr", "<f4"), ("pad_", "<u4"),
+                              ("seed", "<u8")])
+SCORER_PARAMS = 9347
+SCORER_FEATURES = 12
 
 STEP_PARAMS = np.dtype([
     ("sh_degree", "<i4"), ("accumulate_stats", "<i4"), ("absgrad", "<i4"),
@@ -131,4 +135,13 @@ def densify_config(grad_threshold=0.0002, percent_dense=0.01, min_opacity=0.005,
     c["has_max_screen_size"] = 0 if max_screen_size is None else 1
     c["max_screen_size"] = 0 if max_screen_size is None else max_screen_size
     c["split_children"] = split_children
+    return c
+
+
+def scorer_fit_config(epochs=4, batch=4096, lr=1e-3, seed=0):
+    c = np.zeros((), SCORER_FIT_CONFIG)
+    c["epochs"] = epochs
+    c["batch"] = batch
+    c["lr"] = lr
+    c["seed"] = seed
     return c

@@ -57,6 +57,12 @@ def lib():
             "oracle_chain_loss_f64": (f64, [P, u64, P, P, P, P, f64, P]),
             "oracle_train_step_f32": (f32, [P, P, P, u64, P, P, P, i32, i32, f32, P, P, i32,
                                             i32, P, P, P, P]),
+            "oracle_mlp_forward_f32": (None, [P, P, C.c_int64, P, i32]),
+            "oracle_mlp_loss_grad_f32": (f32, [P, P, P, P, C.c_int64, P, i32]),
+            "oracle_mlp_forward_f64": (None, [P, P, C.c_int64, P]),
+            "oracle_mlp_loss_grad_f64": (f64, [P, P, P, P, C.c_int64, P]),
+            "oracle_mlp_adam": (None, [P, P, P, P, i32, f32, f32, f32, f32, f32, f32]),
+            "oracle_bf16_round": (None, [P, P, C.c_int64]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -318,3 +324,55 @@ def chain_loss_f64(g, cam64_, opts64_, bg, target, lam, want_grad=False):
     loss = lib().oracle_chain_loss_f64(ptr(g), len(g), ptr(cam64_), ptr(opts64_), ptr(bg),
                                        ptr(target), lam, ptr(grads))
     return (loss, grads) if want_grad else loss
+
+
+# ---- learned densification scorer (12 -> 64 -> 64 -> 64 -> 3, tanh GELU) ----------------
+
+MLP_PARAMS = 9347
+
+
+def mlp_forward(params, feats, bf16=True):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(feats, np.float32)
+    out = np.zeros((len(x), 3), np.float32)
+    lib().oracle_mlp_forward_f32(ptr(p), ptr(x), len(x), ptr(out), int(bf16))
+    return out
+
+
+def mlp_loss_grad(params, feats, labels, weights, bf16=True):
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(feats, np.float32)
+    y = np.ascontiguousarray(labels, np.uint8)
+    w = np.ascontiguousarray(weights, np.float32)
+    g = np.zeros(MLP_PARAMS, np.float32)
+    loss = lib().oracle_mlp_loss_grad_f32(ptr(p), ptr(x), ptr(y), ptr(w), len(x), ptr(g), int(bf16))
+    return loss, g
+
+
+def mlp_forward_f64(params, feats):
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(feats, np.float64)
+    out = np.zeros((len(x), 3), np.float64)
+    lib().oracle_mlp_forward_f64(ptr(p), ptr(x), len(x), ptr(out))
+    return out
+
+
+def mlp_loss_grad_f64(params, feats, labels, weights):
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(feats, np.float64)
+    y = np.ascontiguousarray(labels, np.uint8)
+    w = np.ascontiguousarray(weights, np.float64)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = lib().oracle_mlp_loss_grad_f64(ptr(p), ptr(x), ptr(y), ptr(w), len(x), ptr(g))
+    return loss, g
+
+
+def mlp_adam(p, g, m, v, lr, c1, c2, b1=0.9, b2=0.999, eps=1e-8):
+    lib().oracle_mlp_adam(ptr(p), ptr(g), ptr(m), ptr(v), len(p), lr, b1, b2, eps, c1, c2)
+
+
+def bf16_round(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().oracle_bf16_round(ptr(x), ptr(out), x.size)
+    return out
