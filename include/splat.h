@@ -452,6 +452,54 @@ int splat_scorer_fit(splat_scorer* s, const float* features, const uint8_t* labe
                      const float* weights, uint64_t n, const splat_scorer_fit_config* cfg,
                      float* loss);
 
+/* Learned densification inside splat_train (no counterpart in the reference).
+ * mode 1 (learn): the reference's decisions are applied unchanged; a sample of them is
+ *   logged at each densification event with the 12 features, labelled `outcome_delay`
+ *   iterations later (a clone/split counts as positive while any descendant is alive),
+ *   weighted by the PSNR change over that window, and the scorer is fit online on every
+ *   labelled sample.
+ * mode 2 (learned): the scorer's logits decide clone / split / prune at each event; the
+ *   reference's prune rules still apply.
+ * Features (binary32, per Gaussian): log2(mean grad / thr + 1e-6) / 8, log2(max grad / thr
+ * + 1e-6) / 8, strong steps / max(visible steps, 1), (log-scale_k - ln(percent_dense *
+ * extent)) / 4 for k = 0..2, sigmoid(opacity), log2(1 + max radius) / 8, log2(1 + visible
+ * steps) / 8, DC luma (0.299 r + 0.587 g + 0.114 b of SH_C0 dc + 0.5), log2(1 + age) / 12,
+ * (max log-scale - ln(percent_dense * extent)) / 4; "strong" steps are those whose
+ * screen-space gradient norm reached the densify threshold. */
+typedef struct {
+  int32_t mode;                /* 1 learn, 2 learned */
+  uint32_t samples_per_event;  /* logged decisions per densification event */
+  uint32_t max_events;         /* sample store capacity, in events */
+  uint32_t outcome_delay;      /* iterations before a logged decision is labelled */
+  float threshold;             /* mode 2: clone/split probability threshold */
+  float prune_threshold;       /* mode 2: prune probability threshold */
+  uint32_t eval_views;         /* views whose mean PSNR weights the samples */
+  uint32_t pad_;
+  splat_scorer_fit_config fit; /* each online update */
+} splat_learn_config;
+
+typedef struct {
+  uint64_t logged, resolved, updates, positives, events;
+  float last_loss;
+  uint32_t pad_;
+} splat_learn_stats;
+
+void splat_learn_config_default(splat_learn_config* cfg);
+/* Attach a scorer (s = NULL detaches).  Turns on the extra per-Gaussian statistics and
+ * lineage planes; splat_train then densifies through the scorer's mode. */
+int splat_trainer_attach_scorer(splat_trainer* t, splat_scorer* s, const splat_learn_config* cfg);
+int splat_trainer_learn_stats(splat_trainer* t, splat_learn_stats* out);
+/* The scorer's features of the resident model ([n][12]) at `iteration`. */
+int splat_trainer_features(splat_trainer* t, float extent, float grad_threshold,
+                           float percent_dense, uint32_t iteration, float* out);
+/* The extra statistics (largest per-step gradient norm, strong steps) and lineage planes
+ * (birth iteration, logged decision the row descends from or 0xffffffff). */
+int splat_trainer_download_lineage(splat_trainer* t, float* max_grad, uint32_t* strong,
+                                   uint32_t* birth, uint32_t* origin, uint64_t capacity);
+/* The labelled samples of mode 1 (features [k][12], label bits, weights); *count = k. */
+int splat_trainer_learn_samples(splat_trainer* t, float* features, uint8_t* labels,
+                                float* weights, uint64_t capacity, uint64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -464,6 +464,33 @@ double oracle_mlp_loss_grad_f64(const double* params, const double* feats, const
   return oracle::mlp::loss_and_grad<double, false>(params, feats, labels, weights, n, grad);
 }
 
+// The learned scorer's 12 features of each Gaussian (AoS parameters, flat order of
+// tests/common/mod.rs:111-139), as specified next to splat_learn_config.
+void oracle_scorer_features_f32(const float* g, int64_t n, const float* grad_accum,
+                                const uint32_t* denom, const uint32_t* max_radii,
+                                const float* max_grad, const uint32_t* strong,
+                                const uint32_t* birth, float thr, float log_size,
+                                uint32_t iteration, float* out) {
+  const float c0s = 0.28209479177387814f;
+  for (int64_t i = 0; i < n; i++) {
+    const float* p = g + i * 59;
+    float* f = out + i * 12;
+    const uint32_t d = denom[i];
+    const float mean = d > 0 ? grad_accum[i] / (float)d : 0.0f;
+    f[0] = log2f(mean / thr + 1e-6f) * 0.125f;
+    f[1] = log2f(max_grad[i] / thr + 1e-6f) * 0.125f;
+    f[2] = (float)strong[i] / (float)(d > 1u ? d : 1u);
+    for (int k = 0; k < 3; k++) f[3 + k] = (p[3 + k] - log_size) * 0.25f;
+    f[6] = 1.0f / (1.0f + expf(-p[10]));
+    f[7] = log2f(1.0f + (float)max_radii[i]) * 0.125f;
+    f[8] = log2f(1.0f + (float)d) * 0.125f;
+    const float r = c0s * p[11] + 0.5f, gg = c0s * p[12] + 0.5f, b = c0s * p[13] + 0.5f;
+    f[9] = 0.299f * r + 0.587f * gg + 0.114f * b;
+    f[10] = log2f(1.0f + (float)(iteration - birth[i])) / 12.0f;
+    f[11] = (fmaxf(fmaxf(p[3], p[4]), p[5]) - log_size) * 0.25f;
+  }
+}
+
 void oracle_bf16_round(const float* in, float* out, int64_t n) {
   for (int64_t i = 0; i < n; i++) out[i] = oracle::mlp::bf16_round(in[i]);
 }

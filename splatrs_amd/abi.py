@@ -180,6 +180,12 @@ _SIGS = {
     "splat_scorer_loss_grad": (i32, [P, P, P, P, u64, P, P]),
     "splat_scorer_fit_config_default": (None, [P]),
     "splat_scorer_fit": (i32, [P, P, P, P, u64, P, P]),
+    "splat_learn_config_default": (None, [P]),
+    "splat_trainer_attach_scorer": (i32, [P, P, P]),
+    "splat_trainer_learn_stats": (i32, [P, P]),
+    "splat_trainer_features": (i32, [P, f32, f32, f32, u32, P]),
+    "splat_trainer_download_lineage": (i32, [P, P, P, P, P, u64]),
+    "splat_trainer_learn_samples": (i32, [P, P, P, P, u64, P]),
     "splat_trainer_download": (i32, [P, P, P, P, P, P, P, u64]),
 }
 
@@ -522,6 +528,44 @@ class Trainer:
     def reserve_entries(self, entries):
         self.ctx.check(lib().splat_trainer_reserve_entries(self._h, entries),
                        "trainer_reserve_entries")
+
+    def attach_scorer(self, scorer, cfg=None):
+        """Learned densification in `train` (cfg: records.learn_config); None detaches."""
+        self._scorer = scorer
+        self.ctx.check(lib().splat_trainer_attach_scorer(
+            self._h, None if scorer is None else scorer.handle, _p(cfg)), "trainer_attach_scorer")
+
+    def learn_stats(self):
+        s = np.zeros((), rec.LEARN_STATS)
+        self.ctx.check(lib().splat_trainer_learn_stats(self._h, _p(s)), "trainer_learn_stats")
+        return s
+
+    def features(self, extent, grad_threshold=0.0002, percent_dense=0.01, iteration=0):
+        n, _ = self.info()
+        out = np.zeros((n, 12), np.float32)
+        self.ctx.check(lib().splat_trainer_features(self._h, extent, grad_threshold,
+                                                    percent_dense, iteration, _p(out)),
+                       "trainer_features")
+        return out
+
+    def lineage(self):
+        n, _ = self.info()
+        mg = np.zeros(n, np.float32)
+        st, bi, og = (np.zeros(n, np.uint32) for _ in range(3))
+        self.ctx.check(lib().splat_trainer_download_lineage(self._h, _p(mg), _p(st), _p(bi),
+                                                            _p(og), n), "trainer_lineage")
+        return dict(max_grad=mg, strong=st, birth=bi, origin=og)
+
+    def learn_samples(self):
+        st = self.learn_stats()
+        k = int(st["resolved"])
+        f = np.zeros((k, 12), np.float32)
+        y = np.zeros(k, np.uint8)
+        w = np.zeros(k, np.float32)
+        cnt = C.c_uint64()
+        self.ctx.check(lib().splat_trainer_learn_samples(self._h, _p(f), _p(y), _p(w), k,
+                                                         C.byref(cnt)), "trainer_learn_samples")
+        return f, y, w
 
     def graph_stats(self):
         """(captures, replays, aborted steps replayed after growing the entry arrays)."""

@@ -342,18 +342,50 @@ void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, fl
                  size_t n, size_t stride, float c1, float c2, const splat_adam_state& st,
                  const splat_learning_rates& lr, const StepDyn* dyn = nullptr,
                  const StepStatus* status = nullptr);
+struct DensifyParams {
+  float grad_threshold, size_limit, min_opacity, world_limit, shrink;
+  int prune_big_world, has_max_screen;
+  uint32_t max_screen, children;
+};
+DensifyParams densify_params(const splat_densify_config& cfg, float extent);
+
+// Densification decisions of one Gaussian as bits: clone, split, prune the row, prune a
+// copy of the row (the copy is not subject to the screen-size rule).
+constexpr uint32_t kDecClone = 1, kDecSplit = 2, kDecPrune = 4, kDecPruneCopy = 8;
+constexpr uint32_t kNoSample = 0xffffffffu;
+
+// Per-Gaussian lineage carried through densification (the learned scorer's age feature and
+// outcome labels): birth iteration and the logged decision it descends from.
+struct DensifyLineage {
+  const uint32_t* birth_in;
+  const uint32_t* origin_in;
+  const uint32_t* sample_of;  // logged decision id of each row at this event, or kNoSample
+  uint32_t* birth_out;
+  uint32_t* origin_out;
+  uint32_t iteration;
+};
+
+// The reference's decisions (DecisionBits) for every Gaussian.
+void densify_decide(splat_ctx* ctx, const float* params, size_t n, size_t stride,
+                    const float* grad_accum, const uint32_t* denom, const uint32_t* max_radii,
+                    const splat_densify_config& cfg, float extent, uint8_t* out);
+// `decide` (optional) replaces the reference's clone/split/prune decisions; `lineage`
+// (optional) carries birth/origin planes.
 size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
                       size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
                       const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
                       const float* normals_dev, size_t normals_len, float* po, float* mo,
                       float* vo, size_t out_capacity, size_t out_stride,
-                      splat_densify_report* report);
+                      splat_densify_report* report, const uint8_t* decide = nullptr,
+                      const DensifyLineage* lineage = nullptr);
 void reset_opacity_planes(splat_ctx* ctx, float* params, float* m, float* v, size_t n,
                           size_t stride, float limit);
 void normals_fill(splat_ctx* ctx, float* out, size_t count, uint64_t seed);
 void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius,
                           const splat_proj_grad* grads, const float* abs_grads, size_t n,
                           uint32_t width, uint32_t height, float* grad_accum, uint32_t* denom,
-                          uint32_t* max_radii, const StepStatus* status = nullptr);
+                          uint32_t* max_radii, const StepStatus* status = nullptr,
+                          float* max_grad = nullptr, uint32_t* strong = nullptr,
+                          float strong_threshold = 0.0f);
 
 }  // namespace splat

@@ -71,7 +71,10 @@ __global__ void __launch_bounds__(256) stats_kernel(const uint32_t* __restrict__
                                                     float* __restrict__ grad_accum,
                                                     uint32_t* __restrict__ denom,
                                                     uint32_t* __restrict__ max_radii,
-                                                    const StepStatus* __restrict__ status) {
+                                                    const StepStatus* __restrict__ status,
+                                                    float* __restrict__ max_grad,
+                                                    uint32_t* __restrict__ strong,
+                                                    float strong_threshold) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (status && status->abort) return;
@@ -87,22 +90,47 @@ __global__ void __launch_bounds__(256) stats_kernel(const uint32_t* __restrict__
   }
   const float gx = g0 * sx;
   const float gy = g1 * sy;
-  grad_accum[i] += sqrtf(gx * gx + gy * gy);
+  const float norm = sqrtf(gx * gx + gy * gy);
+  grad_accum[i] += norm;
   denom[i] += 1u;
   max_radii[i] = max(max_radii[i], r);
+  if (max_grad) {  // extra statistics of the learned scorer's features
+    max_grad[i] = rmax(max_grad[i], norm);
+    strong[i] += norm >= strong_threshold ? 1u : 0u;
+  }
 }
 
-struct DensifyParams {
-  float grad_threshold, size_limit, min_opacity, world_limit, shrink;
-  int prune_big_world, has_max_screen;
-  uint32_t max_screen, children;
-};
 
 __device__ __forceinline__ float max_scale(float l0, float l1, float l2) {  // scales().max_elem()
   return rmax(rmax(splat_expf(l0), splat_expf(l1)), splat_expf(l2));
 }
 
 __device__ __forceinline__ float sigmoid_dev(float x) { return 1.0f / (1.0f + splat_expf(-x)); }
+
+// The reference's decisions as DecisionBits (src/densify.rs:125-214).
+__global__ void __launch_bounds__(256) densify_decide_kernel(const float* __restrict__ params,
+                                                             uint32_t n, uint32_t stride,
+                                                             const float* __restrict__ grad_accum,
+                                                             const uint32_t* __restrict__ denom,
+                                                             const uint32_t* __restrict__ max_radii,
+                                                             DensifyParams dp,
+                                                             uint8_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float grad = denom[i] > 0 ? grad_accum[i] / (float)denom[i] : 0.0f;
+  const float smax = max_scale(params[3 * (size_t)stride + i], params[4 * (size_t)stride + i],
+                               params[5 * (size_t)stride + i]);
+  const float alpha = sigmoid_dev(params[10 * (size_t)stride + i]);
+  bool prune_same = alpha < dp.min_opacity;
+  if (dp.prune_big_world) prune_same = prune_same || smax > dp.world_limit;
+  bool prune_orig = prune_same;
+  if (dp.has_max_screen) prune_orig = prune_orig || max_radii[i] > dp.max_screen;
+  uint32_t d = 0;
+  if (grad >= dp.grad_threshold) d |= smax <= dp.size_limit ? kDecClone : kDecSplit;
+  if (prune_orig) d |= kDecPrune;
+  if (prune_same) d |= kDecPruneCopy;
+  out[i] = (uint8_t)d;
+}
 
 // Per original Gaussian: keep-original, keep-clone, split, keep-children counts.
 __global__ void __launch_bounds__(256) densify_flags_kernel(const float* __restrict__ params,
@@ -115,21 +143,29 @@ __global__ void __launch_bounds__(256) densify_flags_kernel(const float* __restr
                                                             uint32_t* __restrict__ keep_clone,
                                                             uint32_t* __restrict__ split,
                                                             uint32_t* __restrict__ keep_child,
-                                                            uint32_t* __restrict__ clone) {
+                                                            uint32_t* __restrict__ clone,
+                                                            const uint8_t* __restrict__ decide) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float grad = denom[i] > 0 ? grad_accum[i] / (float)denom[i] : 0.0f;
   const float l0 = params[3 * (size_t)stride + i], l1 = params[4 * (size_t)stride + i],
               l2 = params[5 * (size_t)stride + i];
   const float smax = max_scale(l0, l1, l2);
-  const bool is_clone = grad >= dp.grad_threshold && smax <= dp.size_limit;
-  const bool is_split = grad >= dp.grad_threshold && smax > dp.size_limit;
+  bool is_clone = grad >= dp.grad_threshold && smax <= dp.size_limit;
+  bool is_split = grad >= dp.grad_threshold && smax > dp.size_limit;
   const float alpha = sigmoid_dev(params[10 * (size_t)stride + i]);
   // prune of a row with these parameters (src/densify.rs:205-214)
   bool prune_same = alpha < dp.min_opacity;
   if (dp.prune_big_world) prune_same = prune_same || smax > dp.world_limit;
   bool prune_orig = prune_same;
   if (dp.has_max_screen) prune_orig = prune_orig || max_radii[i] > dp.max_screen;
+  if (decide) {  // decisions made elsewhere (DecisionBits)
+    const uint32_t d = decide[i];
+    is_clone = (d & kDecClone) != 0;
+    is_split = (d & kDecSplit) != 0 && !is_clone;
+    prune_orig = (d & kDecPrune) != 0;
+    prune_same = (d & kDecPruneCopy) != 0;
+  }
   bool prune_child = alpha < dp.min_opacity;
   if (dp.prune_big_world)
     prune_child = prune_child ||
@@ -149,9 +185,30 @@ __global__ void __launch_bounds__(256) densify_scatter_kernel(
     const uint32_t* __restrict__ pos_clone, const uint32_t* __restrict__ pos_split,
     const uint32_t* __restrict__ pos_child, uint32_t n_orig_kept, uint32_t n_clone_kept,
     DensifyParams dp, const float* __restrict__ normals, float* __restrict__ po,
-    float* __restrict__ mo, float* __restrict__ vo, uint32_t out_stride) {
+    float* __restrict__ mo, float* __restrict__ vo, uint32_t out_stride, DensifyLineage ln) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // lineage: kept rows keep theirs; new rows are born now and descend from the logged
+  // decision on their parent (or from the parent's own origin when it was not logged)
+  const uint32_t new_origin =
+      ln.birth_out ? (ln.sample_of && ln.sample_of[i] != kNoSample ? ln.sample_of[i]
+                                                                   : ln.origin_in[i])
+                   : 0u;
+  if (keep_orig[i] && ln.birth_out) {
+    ln.birth_out[pos_orig[i]] = ln.birth_in[i];
+    ln.origin_out[pos_orig[i]] = ln.origin_in[i];
+  }
+  if (keep_clone[i] && ln.birth_out) {
+    ln.birth_out[n_orig_kept + pos_clone[i]] = ln.iteration;
+    ln.origin_out[n_orig_kept + pos_clone[i]] = new_origin;
+  }
+  if (split[i] && keep_child[i] && ln.birth_out) {
+    for (uint32_t c = 0; c < dp.children; c++) {
+      const uint32_t o = n_orig_kept + n_clone_kept + pos_child[i] * dp.children + c;
+      ln.birth_out[o] = ln.iteration;
+      ln.origin_out[o] = new_origin;
+    }
+  }
   if (keep_orig[i]) {
     const uint32_t o = pos_orig[i];
     for (int j = 0; j < kParams; j++) {
@@ -233,20 +290,15 @@ void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, fl
 void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius, const splat_proj_grad* grads,
                           const float* abs_grads, size_t n, uint32_t width, uint32_t height,
                           float* grad_accum, uint32_t* denom, uint32_t* max_radii,
-                          const StepStatus* status) {
+                          const StepStatus* status, float* max_grad, uint32_t* strong,
+                          float strong_threshold) {
   const float sx = (float)width * 0.5f, sy = (float)height * 0.5f;
   launch(ctx, "stats", stats_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, radius, grads,
-         abs_grads, (uint32_t)n, sx, sy, grad_accum, denom, max_radii, status);
+         abs_grads, (uint32_t)n, sx, sy, grad_accum, denom, max_radii, status, max_grad, strong,
+         strong_threshold);
 }
 
-// Densify on planes.  Returns the new count; fills report.  Output planes have stride
-// plane_stride(out_capacity) and must hold the result (checked).
-size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
-                      size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
-                      const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
-                      const float* normals_dev, size_t normals_len, float* po, float* mo,
-                      float* vo, size_t out_capacity, size_t out_stride,
-                      splat_densify_report* report) {
+DensifyParams densify_params(const splat_densify_config& cfg, float extent) {
   DensifyParams dp;
   dp.grad_threshold = cfg.grad_threshold;
   dp.size_limit = cfg.percent_dense * extent;
@@ -257,6 +309,29 @@ size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const
   dp.has_max_screen = cfg.has_max_screen_size;
   dp.max_screen = cfg.max_screen_size;
   dp.children = cfg.split_children;
+  return dp;
+}
+
+void densify_decide(splat_ctx* ctx, const float* params, size_t n, size_t stride,
+                    const float* grad_accum, const uint32_t* denom, const uint32_t* max_radii,
+                    const splat_densify_config& cfg, float extent, uint8_t* out) {
+  launch(ctx, "densify_decide", densify_decide_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         params, (uint32_t)n, (uint32_t)stride, grad_accum, denom, max_radii,
+         densify_params(cfg, extent), out);
+}
+
+// Densify on planes.  Returns the new count; fills report.  Output planes have stride
+// plane_stride(out_capacity) and must hold the result (checked).
+size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
+                      size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
+                      const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
+                      const float* normals_dev, size_t normals_len, float* po, float* mo,
+                      float* vo, size_t out_capacity, size_t out_stride,
+                      splat_densify_report* report, const uint8_t* decide,
+                      const DensifyLineage* lineage) {
+  const DensifyParams dp = densify_params(cfg, extent);
+  DensifyLineage ln{};
+  if (lineage) ln = *lineage;
   uint32_t* flags = ctx->buf<uint32_t>("densify_flags", 5 * (n + 1));
   uint32_t* pos = ctx->buf<uint32_t>("densify_pos", 5 * (n + 1));
   uint32_t* totals = ctx->buf<uint32_t>("densify_totals", 5);
@@ -266,7 +341,7 @@ size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const
            *po_ch = pos + 3 * (n + 1), *po_cl = pos + 4 * (n + 1);
   launch(ctx, "densify_flags", densify_flags_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
          params, (uint32_t)n, (uint32_t)stride, grad_accum, denom, max_radii, dp, ko, kc, sp, kch,
-         cl);
+         cl, decide);
   exclusive_scan_u32(ctx, ko, po_o, n, totals + 0);
   exclusive_scan_u32(ctx, kc, po_c, n, totals + 1);
   exclusive_scan_u32(ctx, sp, po_s, n, totals + 2);
@@ -289,7 +364,7 @@ size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const
          params, m, v, (uint32_t)n, (uint32_t)stride, (const uint32_t*)ko, (const uint32_t*)kc,
          (const uint32_t*)sp, (const uint32_t*)kch, (const uint32_t*)po_o,
          (const uint32_t*)po_c, (const uint32_t*)po_s, (const uint32_t*)po_ch, n_orig_kept,
-         n_clone_kept, dp, normals_dev, po, mo, vo, (uint32_t)out_stride);
+         n_clone_kept, dp, normals_dev, po, mo, vo, (uint32_t)out_stride, ln);
   return n_out;
 }
 

@@ -21,66 +21,9 @@
 #include <new>
 
 #include "internal.hpp"
+#include "trainer.hpp"
 
 using namespace splat;
-
-struct TrainerView {
-  splat_camera cam;
-  DevBuf rgb;
-};
-
-// Everything a captured step bakes in: sizes, flags and device buffers.  Two steps with
-// byte-identical plans can replay the same graph (every buffer in the plan is reserved to
-// at least the sizes in the plan before the replay).
-struct StepPlan {
-  uint64_t n, stride, entry_cap;
-  uint32_t width, height;
-  int32_t sh_degree, antialias, ordered, accumulate, absgrad, pad0;
-  float near_plane, lambda, bg[3], pad1;
-  float *params, *m, *v, *grads, *grad_accum;
-  uint32_t *denom, *max_radii;
-  ProjBuffers pb;
-  BinBuffers bb;
-  float *image, *final_t, *dimage, *loss_scratch, *slots, *absg, *loss_one;
-  uint32_t *n_contrib, *small, *big_list, *scan_sums;
-  unsigned long long* total64;
-  splat_proj_grad* pgrad;
-  StepDyn* dyn;
-  StepStatus* status;
-  StepReport* reports;
-};
-
-struct PendingStep {
-  uint64_t seq;
-  uint32_t view;
-  splat_step_params p;
-  uint64_t adam_step;
-  float* loss_out;
-  hipEvent_t done;
-};
-
-struct splat_trainer {
-  splat_ctx* ctx = nullptr;
-  int device = 0;
-  size_t n = 0, cap = 0, stride = 0;
-  uint64_t adam_step = 0;
-  DevBuf params, m, v, grads;           // planes [59][stride]
-  DevBuf grad_accum, denom, max_radii;  // statistics, capacity cap
-  std::vector<std::unique_ptr<TrainerView>> views;
-  std::map<std::string, DevBuf> store;  // per-view buffers (projection, binning, images)
-  size_t entry_cap = 0;
-  float* loss_target = nullptr;  // when set, each step writes its loss value here (device)
-  // graph-captured steps
-  bool use_graph = true;
-  uint64_t seq = 0;
-  std::deque<PendingStep> pending;
-  std::vector<std::pair<StepPlan, hipGraphExec_t>> graphs;
-  std::vector<hipEvent_t> events;
-  StepReport* reports_host = nullptr;  // pinned, mapped; written by the step's last kernel
-  StepReport* reports_dev = nullptr;
-  unsigned long long last_total = 0;
-  uint64_t graph_captures = 0, graph_replays = 0, step_aborts = 0;
-};
 
 namespace {
 
@@ -137,6 +80,7 @@ void zero_stats(splat_trainer* t) {
   fill_u32(ctx, t->grad_accum.as<uint32_t>(), t->n, 0u);  // +0.0f has all bits clear
   fill_u32(ctx, t->denom.as<uint32_t>(), t->n, 0u);
   fill_u32(ctx, t->max_radii.as<uint32_t>(), t->n, 0u);
+  if (t->track) learn_zero_stats(t);
 }
 
 ProjBuffers proj_buffers(splat_trainer* t) {
@@ -176,6 +120,24 @@ __global__ void __launch_bounds__(256) count_visible_kernel(const uint32_t* __re
   const bool vis = i < n && radius[i] > 0;
   const uint64_t mask = __ballot(vis);
   if ((threadIdx.x & 63) == 0 && mask) atomicAdd(count, (uint32_t)__popcll(mask));
+}
+
+// Per-block sums of squared errors (binary64) between a clamped image and RGB8 targets.
+__global__ void __launch_bounds__(256) sq_err_kernel(const float* __restrict__ img,
+                                                     const uint8_t* __restrict__ rgb, size_t n,
+                                                     double* __restrict__ partial) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double d = (double)rclamp(img[i], 0.0f, 1.0f) -
+                     (double)rclamp((float)rgb[i] / 255.0f, 0.0f, 1.0f);
+    acc += d * d;
+  }
+  for (int k = 32; k >= 1; k >>= 1) acc += __shfl_xor(acc, k, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
 __global__ void set_dyn_kernel(StepDyn d, StepDyn* __restrict__ out) {
@@ -251,6 +213,11 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   P.grad_accum = t->grad_accum.as<float>();
   P.denom = t->denom.as<uint32_t>();
   P.max_radii = t->max_radii.as<uint32_t>();
+  if (t->track) {
+    P.max_grad = t->max_grad.as<float>();
+    P.strong = t->strong.as<uint32_t>();
+    P.strong_threshold = t->strong_threshold;
+  }
   P.pb = proj_buffers(t);
   BinBuffers& bb = P.bb;
   bb.tiles_x = tx;
@@ -309,7 +276,8 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   project_backward(ctx, P.params, n, P.stride, cam0, opts, P.pgrad, P.grads, P.dyn);
   if (P.accumulate)
     stats_accumulate_dev(ctx, P.pb.radius, P.pgrad, P.absgrad ? P.absg : nullptr, n, P.width,
-                         P.height, P.grad_accum, P.denom, P.max_radii, P.status);
+                         P.height, P.grad_accum, P.denom, P.max_radii, P.status, P.max_grad,
+                         P.strong, P.strong_threshold);
   splat_adam_state ast{0.9f, 0.999f, 1e-15f, 0, 0};
   splat_learning_rates lr0;
   std::memset(&lr0, 0, sizeof(lr0));
@@ -435,6 +403,109 @@ void drain(splat_trainer* t) {
 }
 
 }  // namespace
+
+namespace splat {
+
+void trainer_drain(splat_trainer* t) { drain(t); }
+
+float trainer_eval_psnr(splat_trainer* t, uint32_t views, int32_t sh_degree, const float bg[3]) {
+  drain(t);
+  splat_ctx* ctx = t->ctx;
+  views = std::min<uint32_t>(views, (uint32_t)t->views.size());
+  if (views == 0) return 0.0f;
+  constexpr unsigned kBlocks = 512;
+  double* partial = (double*)t->store["psnr_partial"].reserve(kBlocks * 8 * (size_t)views);
+  for (uint32_t v = 0; v < views; v++) {
+    const TrainerView& tv = *t->views[v];
+    const uint32_t W = tv.cam.width, H = tv.cam.height;
+    const size_t npx = (size_t)W * H;
+    splat_proj_opts opts{0.2f, 0.3f, 0, sh_degree};
+    ProjBuffers pb = proj_buffers(t);
+    project_forward(ctx, t->params.as<float>(), t->n, t->stride, tv.cam, opts, pb);
+    BinBuffers bb{};
+    bin_build(ctx, pb, t->n, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile, bb,
+              &t->entry_cap, t->store, nullptr);
+    float* img = (float*)t->store["image"].reserve(npx * 12 + 4);
+    float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
+    uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
+    raster_forward(ctx, pb, bb, W, H, bg, img, final_t, n_contrib);
+    launch(ctx, "sq_err", sq_err_kernel, dim3(kBlocks), dim3(256), 0, (const float*)img,
+           tv.rgb.as<uint8_t>(), npx * 3, partial + (size_t)v * kBlocks);
+  }
+  std::vector<double> h((size_t)kBlocks * views);
+  SPLAT_HIP(hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+  double tot = 0;
+  for (uint32_t v = 0; v < views; v++) {
+    double sse = 0;
+    for (unsigned b = 0; b < kBlocks; b++) sse += h[(size_t)v * kBlocks + b];
+    const TrainerView& tv = *t->views[v];
+    const double mse = sse / (double)((size_t)tv.cam.width * tv.cam.height * 3);
+    tot += mse <= 0 ? 100.0 : -10.0 * std::log10(mse);
+  }
+  return (float)(tot / views);
+}
+
+void trainer_densify(splat_trainer* t, const splat_densify_config& cfg, float extent,
+                     uint64_t seed, const uint8_t* decide, const uint32_t* sample_of,
+                     uint32_t iteration, splat_densify_report* report) {
+  drain(t);
+  splat_ctx* ctx = t->ctx;
+  const size_t n = t->n;
+  *report = splat_densify_report{0, 0, 0};
+  if (n == 0) return;
+  const size_t children = cfg.split_children;
+  const size_t nnorm = 3 * children * n;
+  float* normals = (float*)t->store["normals"].reserve((nnorm + 1) * 4);
+  normals_fill(ctx, normals, nnorm, seed);
+  const size_t cap2 = std::max(t->cap, n * (1 + std::max<size_t>(children, 1)));
+  const size_t stride2 = plane_stride(cap2);
+  DevBuf po, mo, vo, bo, oo;
+  po.reserve(stride2 * kParams * 4);
+  mo.reserve(stride2 * kParams * 4);
+  vo.reserve(stride2 * kParams * 4);
+  DensifyLineage ln{};
+  if (t->track) {
+    ln.birth_in = t->birth.as<uint32_t>();
+    ln.origin_in = t->origin.as<uint32_t>();
+    ln.sample_of = sample_of;
+    ln.birth_out = (uint32_t*)bo.reserve((cap2 + 1) * 4);
+    ln.origin_out = (uint32_t*)oo.reserve((cap2 + 1) * 4);
+    ln.iteration = iteration;
+  }
+  const size_t k = densify_planes(ctx, t->params.as<float>(), t->m.as<float>(), t->v.as<float>(),
+                                  n, t->stride, t->grad_accum.as<float>(), t->denom.as<uint32_t>(),
+                                  t->max_radii.as<uint32_t>(), cfg, extent, normals, nnorm,
+                                  po.as<float>(), mo.as<float>(), vo.as<float>(), cap2, stride2,
+                                  report, decide, t->track ? &ln : nullptr);
+  SPLAT_REQUIRE(k != (size_t)-1 && k <= cap2, "densify output exceeds its buffer");
+  SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+  std::swap(t->params.ptr, po.ptr);
+  std::swap(t->params.bytes, po.bytes);
+  std::swap(t->m.ptr, mo.ptr);
+  std::swap(t->m.bytes, mo.bytes);
+  std::swap(t->v.ptr, vo.ptr);
+  std::swap(t->v.bytes, vo.bytes);
+  if (t->track) {
+    std::swap(t->birth.ptr, bo.ptr);
+    std::swap(t->birth.bytes, bo.bytes);
+    std::swap(t->origin.ptr, oo.ptr);
+    std::swap(t->origin.bytes, oo.bytes);
+    t->max_grad.reserve((cap2 + 1) * 4);
+    t->strong.reserve((cap2 + 1) * 4);
+  }
+  t->cap = cap2;
+  t->stride = stride2;
+  t->grads.reserve(stride2 * kParams * 4);
+  t->grad_accum.reserve((cap2 + 1) * 4);
+  t->denom.reserve((cap2 + 1) * 4);
+  t->max_radii.reserve((cap2 + 1) * 4);
+  t->n = k;
+  zero_stats(t);  // src/densify.rs:219
+  SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+}  // namespace splat
 
 extern "C" {
 
@@ -575,7 +646,9 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     if (p->accumulate_stats)
       stats_accumulate_dev(ctx, pb.radius, pgrad, p->absgrad ? absg : nullptr, n, W, H,
                            t->grad_accum.as<float>(), t->denom.as<uint32_t>(),
-                           t->max_radii.as<uint32_t>());
+                           t->max_radii.as<uint32_t>(), nullptr,
+                           t->track ? t->max_grad.as<float>() : nullptr,
+                           t->track ? t->strong.as<uint32_t>() : nullptr, t->strong_threshold);
 
     // optimiser (src/train.rs:238-247, src/adam.rs:77-90)
     t->adam_step += 1;
@@ -615,43 +688,7 @@ int splat_trainer_densify(splat_trainer* t, const splat_densify_config* cfg, flo
                           uint64_t seed, splat_densify_report* report) {
   return tguard(t, [&] {
     SPLAT_REQUIRE(cfg && report, "null argument");
-    drain(t);
-    splat_ctx* ctx = t->ctx;
-    const size_t n = t->n;
-    *report = splat_densify_report{0, 0, 0};
-    if (n == 0) return;
-    const size_t children = cfg->split_children;
-    const size_t nnorm = 3 * children * n;
-    float* normals = (float*)t->store["normals"].reserve((nnorm + 1) * 4);
-    normals_fill(ctx, normals, nnorm, seed);
-    const size_t cap2 = std::max(t->cap, n * (1 + std::max<size_t>(children, 1)));
-    const size_t stride2 = plane_stride(cap2);
-    DevBuf po, mo, vo;
-    po.reserve(stride2 * kParams * 4);
-    mo.reserve(stride2 * kParams * 4);
-    vo.reserve(stride2 * kParams * 4);
-    const size_t k = densify_planes(ctx, t->params.as<float>(), t->m.as<float>(),
-                                    t->v.as<float>(), n, t->stride, t->grad_accum.as<float>(),
-                                    t->denom.as<uint32_t>(), t->max_radii.as<uint32_t>(), *cfg,
-                                    extent, normals, nnorm, po.as<float>(), mo.as<float>(),
-                                    vo.as<float>(), cap2, stride2, report);
-    SPLAT_REQUIRE(k != (size_t)-1 && k <= cap2, "densify output exceeds its buffer");
-    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
-    std::swap(t->params.ptr, po.ptr);
-    std::swap(t->params.bytes, po.bytes);
-    std::swap(t->m.ptr, mo.ptr);
-    std::swap(t->m.bytes, mo.bytes);
-    std::swap(t->v.ptr, vo.ptr);
-    std::swap(t->v.bytes, vo.bytes);
-    t->cap = cap2;
-    t->stride = stride2;
-    t->grads.reserve(stride2 * kParams * 4);
-    t->grad_accum.reserve((cap2 + 1) * 4);
-    t->denom.reserve((cap2 + 1) * 4);
-    t->max_radii.reserve((cap2 + 1) * 4);
-    t->n = k;
-    zero_stats(t);  // src/densify.rs:219
-    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    trainer_densify(t, *cfg, extent, seed, nullptr, nullptr, 0, report);
   });
 }
 
@@ -877,6 +914,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
     // Kept in the trainer's store: steps still in flight write into it.
     float* hist = (float*)t->store["loss_hist"].reserve((size_t)(log_iv ? log_iv : 1) * 4 + 4);
     ViewOrder order(cfg->seed, (uint32_t)t->views.size());
+    t->strong_threshold = cfg->densify_grad_threshold;
     const float bg = cfg->white_background ? 1.0f : 0.0f;
     int32_t sh = 0;  // model.sh_degree starts at 0 (src/train.rs:186-187)
     float ema = 0.0f;
@@ -924,8 +962,12 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
           dcfg.split_children = 2;
           uint64_t s = cfg->seed ^ (0x5851f42d4c957f2dull * (uint64_t)it);
           splat_densify_report rep;
-          const int s2 = splat_trainer_densify(t, &dcfg, extent, sm64_next(s), &rep);
-          if (s2 != SPLAT_OK) fail(s2, ctx->err);
+          if (t->learn) {
+            learn_densify(t, dcfg, extent, sm64_next(s), it, sh, p.background, &rep);
+          } else {
+            const int s2 = splat_trainer_densify(t, &dcfg, extent, sm64_next(s), &rep);
+            if (s2 != SPLAT_OK) fail(s2, ctx->err);
+          }
           dc += rep.cloned;
           ds += rep.split;
           dp += rep.pruned;

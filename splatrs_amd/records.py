@@ -37,6 +37,13 @@ DENSIFY_CONFIG = np.dtype([
 DENSIFY_REPORT = np.dtype([("cloned", "<u8"), ("split", "<u8"), ("pruned", "<u8")])
 SCORER_FIT_CONFIG = np.dtype([("epochs", "<u4"), ("batch", "<u4"), ("lr", "<f4"), ("pad_", "<u4"),
                               ("seed", "<u8")])
+LEARN_CONFIG = np.dtype([("mode", "<i4"), ("samples_per_event", "<u4"), ("max_events", "<u4"),
+                         ("outcome_delay", "<u4"), ("threshold", "<f4"),
+                         ("prune_threshold", "<f4"), ("eval_views", "<u4"), ("pad_", "<u4"),
+                         ("fit", SCORER_FIT_CONFIG)])
+LEARN_STATS = np.dtype([("logged", "<u8"), ("resolved", "<u8"), ("updates", "<u8"),
+                        ("positives", "<u8"), ("events", "<u8"), ("last_loss", "<f4"),
+                        ("pad_", "<u4")])
 SCORER_PARAMS = 9347
 SCORER_FEATURES = 12
 
@@ -144,4 +151,18 @@ def scorer_fit_config(epochs=4, batch=4096, lr=1e-3, seed=0):
     c["batch"] = batch
     c["lr"] = lr
     c["seed"] = seed
+    return c
+
+
+def learn_config(mode=1, samples_per_event=16384, max_events=64, outcome_delay=500,
+                 threshold=0.5, prune_threshold=0.5, eval_views=8, fit=None):
+    c = np.zeros((), LEARN_CONFIG)
+    c["mode"] = mode
+    c["samples_per_event"] = samples_per_event
+    c["max_events"] = max_events
+    c["outcome_delay"] = outcome_delay
+    c["threshold"] = threshold
+    c["prune_threshold"] = prune_threshold
+    c["eval_views"] = eval_views
+    c["fit"] = fit if fit is not None else scorer_fit_config()
     return c

@@ -63,6 +63,8 @@ def lib():
             "oracle_mlp_loss_grad_f64": (f64, [P, P, P, P, C.c_int64, P]),
             "oracle_mlp_adam": (None, [P, P, P, P, i32, f32, f32, f32, f32, f32, f32]),
             "oracle_bf16_round": (None, [P, P, C.c_int64]),
+            "oracle_scorer_features_f32": (None, [P, C.c_int64, P, P, P, P, P, P, f32, f32, u32,
+                                                  P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -375,4 +377,16 @@ def bf16_round(x):
     x = np.ascontiguousarray(x, np.float32)
     out = np.zeros_like(x)
     lib().oracle_bf16_round(ptr(x), ptr(out), x.size)
+    return out
+
+
+def scorer_features(g, grad_accum, denom, max_radii, max_grad, strong, birth, thr, log_size,
+                    iteration):
+    g = np.ascontiguousarray(g, np.float32)
+    out = np.zeros((len(g), 12), np.float32)
+    arrs = [np.ascontiguousarray(a, t) for a, t in
+            [(grad_accum, np.float32), (denom, np.uint32), (max_radii, np.uint32),
+             (max_grad, np.float32), (strong, np.uint32), (birth, np.uint32)]]
+    lib().oracle_scorer_features_f32(ptr(g), len(g), *[ptr(a) for a in arrs], thr, log_size,
+                                     iteration, ptr(out))
     return out
