@@ -47,6 +47,69 @@ def algorithmic_bytes(name, N, V, E, P):
     return table.get(name)
 
 
+def learned_compare(iters, log=print):
+    """Full training with densification on the ellipsoid scene (300k surface points, 100
+    cameras 1297x840 f = 1100 on a radius-6 hemisphere, targets from a 2M-Gaussian ground
+    truth, `iters` iterations, densify every 100 from 500 to 5000 with threshold 2e-4,
+    opacity reset every 3000), run twice from the same initialisation:
+      plain   -- the reference's threshold heuristic (with the scorer logging and fitting
+                 online; the model is bitwise the plain method's),
+      learned -- the scorer trained in the first run makes the decisions.
+    Returns PSNR over all views, Gaussian counts and wall-clock iterations/s of each run."""
+    import numpy as np
+    from splatrs_amd import abi, scenes
+    from splatrs_amd import records as rec
+    ctx = abi.Context(0)
+    t0 = time.perf_counter()
+    g_gt, pts, rgb = scenes.ellipsoid_scene(0)
+    cams = scenes.hemisphere_cameras()
+    gt = abi.Trainer(ctx, g_gt)
+    blank = np.zeros((840, 1297, 3), np.uint8)
+    for c in cams:
+        gt.add_view(c, blank)
+    views = [abi.to_rgb8(gt.render(i, 3)) for i in range(len(cams))]
+    gt.close()
+    del g_gt
+    init = ctx.from_points(pts, rgb)
+    extent = scenes.scene_extent(cams)
+    log(f"scene ready in {time.perf_counter() - t0:.1f} s (extent {extent:.3f})")
+    until = min(5000, iters)
+    cfg = dict(iterations=iters, densify=1, densify_from_iter=500, densify_until_iter=until,
+               densification_interval=100, opacity_reset_interval=3000,
+               densify_grad_threshold=0.0002, sh_degree=3, sh_degree_interval=1000,
+               position_lr_max_steps=iters, log_interval=0, seed=0)
+    scorer = abi.Scorer(ctx, seed=0)
+
+    def run(mode):
+        tr = abi.Trainer(ctx, init)
+        for c, v in zip(cams, views):
+            tr.add_view(c, v)
+        tr.attach_scorer(scorer, rec.learn_config(
+            mode=mode, samples_per_event=32768, max_events=64, outcome_delay=500,
+            eval_views=8, fit=rec.scorer_fit_config(epochs=2, batch=4096, lr=1e-3, seed=1)))
+        t = time.perf_counter()
+        tr.train(extent, abi.TrainConfig.default(**cfg))
+        tr.sync()
+        sec = time.perf_counter() - t
+        psnr = float(np.mean([abi.psnr(tr.render(i, 3), v.astype(np.float32) / np.float32(255))
+                              for i, v in enumerate(views)]))
+        st = tr.learn_stats()
+        out = {"psnr_db": round(psnr, 3), "gaussians": int(tr.info()[0]),
+               "it_per_s": round(iters / sec, 2), "seconds": round(sec, 2),
+               "samples": int(st["resolved"]), "positives": int(st["positives"]),
+               "fit_updates": int(st["updates"]), "fit_loss": round(float(st["last_loss"]), 4)}
+        tr.close()
+        log(f"mode {mode}: {out}")
+        return out
+    plain = run(1)
+    learned = run(2)
+    ctx.close()
+    return {"workload": "300k points on 20 ellipsoids -> densify, 100 cams 1297x840, "
+                        f"{iters} iterations", "plain": plain, "learned": learned,
+            "psnr_delta_db": round(learned["psnr_db"] - plain["psnr_db"], 3),
+            "gaussian_ratio": round(learned["gaussians"] / max(plain["gaussians"], 1), 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,7 +123,15 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (reference kernel of "
                          "known traffic for the HBM byte counters) and exit")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="only run the plain-vs-learned densification comparison")
+    ap.add_argument("--learned-iters", type=int, default=7000)
     args = ap.parse_args()
+    if args.learned_only:
+        r = learned_compare(args.learned_iters,
+                            log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(r), flush=True)
+        return
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4

@@ -413,6 +413,62 @@ int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* 
                            float* grad_accum, uint32_t* denom, uint32_t* max_radii,
                            uint64_t capacity);
 
+/* ---- batched training ----------------------------------------------------------------
+ * No counterpart in the reference (one view of one scene per iteration, train.rs:211-247).
+ * Many independent scenes and many views of uneven resolution train in one step: each
+ * stage (projection, binning, compositing, loss, backward, Adam) is one launch over every
+ * (scene, view) pair, and the step is captured in a HIP graph.  A step renders each listed
+ * view of its scene with the reference's forward pass, takes the reference's loss and
+ * gradient per view, sums each scene's parameter gradients over its views in list order,
+ * and applies one Adam step to every scene (its own position learning rate).  One view of
+ * one scene is exactly the reference's iteration.  At most 64 scenes; scenes are added
+ * before the first step. */
+typedef struct splat_batch splat_batch;
+
+typedef struct {
+  int32_t sh_degree;
+  int32_t ordered_backward;  /* 1: the reference's summation order (bitwise) */
+  int32_t antialias;
+  float near_plane;          /* 0.2 */
+  float lambda_dssim;        /* 0.2 */
+  float background[3];
+  splat_learning_rates lr;   /* lr.mean unused: per-scene rates are passed to the step */
+} splat_batch_step_params;
+
+/* Launch shapes of the compositing kernels (results do not depend on them): entries per
+ * LDS batch (forward 32/64/128, tree-reduced backward 16/32) and persistent launches
+ * (a fixed grid of per_cu workgroups per compute unit taking tiles from a counter). */
+typedef struct {
+  int32_t fwd_batch, fwd_persistent, fwd_per_cu;
+  int32_t bwd_batch, bwd_persistent, bwd_per_cu;
+  float fwd_ms, bwd_ms;                  /* best measured times */
+  float default_fwd_ms, default_bwd_ms;  /* times of the default shape (64 / 32, one
+                                            workgroup per tile) */
+  uint32_t candidates, pad_;
+} splat_tune_result;
+
+int splat_batch_create(splat_ctx* ctx, splat_batch** out);
+int splat_batch_destroy(splat_batch* b);
+int splat_batch_add_scene(splat_batch* b, const float* gaussians, uint64_t n, uint32_t* scene);
+int splat_batch_add_view(splat_batch* b, uint32_t scene, const splat_camera* cam,
+                         const uint8_t* rgb, uint32_t* view);
+/* One step over views[0..nviews) (each at most once); lr_mean[s] is scene s's position
+ * learning rate; losses (optional, host) receives each listed view's loss. */
+int splat_batch_step(splat_batch* b, const uint32_t* views, uint32_t nviews,
+                     const splat_batch_step_params* p, const float* lr_mean, float* losses);
+/* Autotuner: times every candidate launch shape of the compositing kernels on this batch
+ * (the model is not changed) and keeps the fastest for later steps. */
+int splat_batch_tune(splat_batch* b, const uint32_t* views, uint32_t nviews,
+                     const splat_batch_step_params* p, splat_tune_result* out);
+/* Use the given launch shapes (the shape fields of a tune result). */
+int splat_batch_set_shapes(splat_batch* b, const splat_tune_result* shapes);
+int splat_batch_download(splat_batch* b, uint32_t scene, float* gaussians, float* m, float* v,
+                         uint64_t capacity);
+/* Parameter rows (all scenes, padded), Adam steps, graph captures / replays, and steps
+ * run again because their tile entries outgrew the preallocated arrays. */
+int splat_batch_info(splat_batch* b, uint64_t* rows, uint64_t* adam_step, uint64_t* captures,
+                     uint64_t* replays, uint64_t* aborts);
+
 /* ---- learned densification scorer --------------------------------------------------
  * No counterpart in the reference, whose densification is the threshold heuristic
  * (src/densify.rs:125-221).  A small MLP scores each Gaussian from 12 statistics:

@@ -181,6 +181,15 @@ _SIGS = {
     "splat_scorer_fit_config_default": (None, [P]),
     "splat_scorer_fit": (i32, [P, P, P, P, u64, P, P]),
     "splat_learn_config_default": (None, [P]),
+    "splat_batch_create": (i32, [P, P]),
+    "splat_batch_destroy": (i32, [P]),
+    "splat_batch_add_scene": (i32, [P, P, u64, P]),
+    "splat_batch_add_view": (i32, [P, u32, P, P, P]),
+    "splat_batch_step": (i32, [P, P, u32, P, P, P]),
+    "splat_batch_tune": (i32, [P, P, u32, P, P]),
+    "splat_batch_set_shapes": (i32, [P, P]),
+    "splat_batch_download": (i32, [P, u32, P, P, P, u64]),
+    "splat_batch_info": (i32, [P, P, P, P, P, P]),
     "splat_trainer_attach_scorer": (i32, [P, P, P]),
     "splat_trainer_learn_stats": (i32, [P, P]),
     "splat_trainer_features": (i32, [P, f32, f32, f32, u32, P]),
@@ -692,3 +701,77 @@ class Scorer:
         self.ctx.check(lib().splat_scorer_fit(self._h, _p(x), _p(y), _p(w), len(x), _p(cfg),
                                               C.byref(loss)), "scorer_fit")
         return loss.value
+
+
+class Batch:
+    """Batched training of many scenes and views (splat_batch_*): one launch per stage over
+    every (scene, view) pair, the step captured in a HIP graph."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        ctx.check(lib().splat_batch_create(ctx.handle, C.byref(self._h)), "batch_create")
+        ctx._trainers.add(self)
+        self.scene_sizes = []
+        self.views = []
+
+    def close(self):
+        if self._h:
+            lib().splat_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, gaussians):
+        g = np.ascontiguousarray(gaussians, np.float32)
+        idx = C.c_uint32()
+        self.ctx.check(lib().splat_batch_add_scene(self._h, _p(g), len(g), C.byref(idx)),
+                       "batch_add_scene")
+        self.scene_sizes.append(len(g))
+        return idx.value
+
+    def add_view(self, scene, cam, rgb):
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        assert rgb.shape == (int(cam["height"]), int(cam["width"]), 3)
+        idx = C.c_uint32()
+        self.ctx.check(lib().splat_batch_add_view(self._h, scene, _p(cam), _p(rgb),
+                                                  C.byref(idx)), "batch_add_view")
+        self.views.append((scene, cam))
+        return idx.value
+
+    def step(self, views, params, lr_mean, want_losses=True):
+        v = np.ascontiguousarray(views, np.uint32)
+        lm = np.ascontiguousarray(lr_mean, np.float32)
+        losses = np.zeros(len(v), np.float32) if want_losses else None
+        self.ctx.check(lib().splat_batch_step(self._h, _p(v), len(v), _p(params), _p(lm),
+                                              _p(losses)), "batch_step")
+        return losses
+
+    def tune(self, views, params):
+        v = np.ascontiguousarray(views, np.uint32)
+        r = np.zeros((), rec.TUNE_RESULT)
+        self.ctx.check(lib().splat_batch_tune(self._h, _p(v), len(v), _p(params), _p(r)),
+                       "batch_tune")
+        return r
+
+    def set_shapes(self, shapes):
+        self.ctx.check(lib().splat_batch_set_shapes(self._h, _p(shapes)), "batch_set_shapes")
+
+    def download(self, scene, moments=False):
+        n = self.scene_sizes[scene]
+        g = np.zeros((n, rec.NUM_PARAMS), np.float32)
+        m = np.zeros_like(g) if moments else None
+        v = np.zeros_like(g) if moments else None
+        self.ctx.check(lib().splat_batch_download(self._h, scene, _p(g), _p(m), _p(v), n),
+                       "batch_download")
+        return {"params": g, "m": m, "v": v} if moments else {"params": g}
+
+    def info(self):
+        vals = [C.c_uint64() for _ in range(5)]
+        self.ctx.check(lib().splat_batch_info(self._h, *[C.byref(x) for x in vals]), "batch_info")
+        return dict(zip(["rows", "adam_step", "captures", "replays", "aborts"],
+                        [x.value for x in vals]))

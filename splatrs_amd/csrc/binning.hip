@@ -67,6 +67,41 @@ __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict_
   entry_slot[e] = atomicAdd(&tile_count[tile], 1u);
 }
 
+// keygen over the global tile space of a batched step: the row's pair gives its tile grid
+// and tile offset.
+__global__ void __launch_bounds__(256) keygen_batch_kernel(const uint32_t* __restrict__ entry_offset,
+                                                           uint32_t n, BatchDev bd,
+                                                           const uint2* __restrict__ rect,
+                                                           uint32_t* __restrict__ entry_gaussian,
+                                                           uint32_t* __restrict__ entry_tile,
+                                                           uint32_t* __restrict__ entry_slot,
+                                                           uint32_t* __restrict__ tile_count,
+                                                           const StepStatus* __restrict__ status) {
+  if (status->abort) return;
+  const uint32_t e_total = status->entries;
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e_total) return;
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (entry_offset[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const uint32_t g = lo;
+  const uint32_t v = find_segment(bd.work_base, bd.npairs, g);
+  const uint32_t local = e - entry_offset[g];
+  const uint2 rc = rect[g];
+  const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
+  const uint32_t w = x1 - x0;
+  const uint32_t ty = y0 + local / w, tx = x0 + local % w;
+  const uint32_t tile = bd.tile_base[v] + ty * bd.tiles_x[v] + tx;
+  entry_gaussian[e] = g;
+  entry_tile[e] = tile;
+  entry_slot[e] = atomicAdd(&tile_count[tile], 1u);
+}
+
 __global__ void __launch_bounds__(256) scatter_kernel(uint32_t e_total,
                                                       const uint32_t* __restrict__ entry_gaussian,
                                                       const uint32_t* __restrict__ entry_tile,
@@ -285,7 +320,8 @@ void bin_guard(splat_ctx* ctx, const unsigned long long* total64, size_t capacit
 
 void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
                       size_t capacity, uint32_t* small, unsigned long long* total64,
-                      uint32_t* big_list, uint32_t* scan_sums, StepStatus* status) {
+                      uint32_t* big_list, uint32_t* scan_sums, StepStatus* status,
+                      const BatchDev* bd) {
   const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
   fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
   launch(ctx, "bin_total", sum_u64_kernel, dim3(std::min<unsigned>(blocks_for(n, 256), 1024u)),
@@ -295,10 +331,16 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
   fill_u32(ctx, bb.tile_count, T, 0);
   fill_u32(ctx, small + 1, 1, 0);
   // The entry kernels cover the capacity and read the step's count from `status`.
-  launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0,
-         (const uint32_t*)bb.entry_offset, (uint32_t)n, 0u, (const uint2*)pb.rect, bb.tiles_x,
-         bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
-         (const StepStatus*)status);
+  if (bd)
+    launch(ctx, "bin_keygen", keygen_batch_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0,
+           (const uint32_t*)bb.entry_offset, (uint32_t)n, *bd, (const uint2*)pb.rect,
+           bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
+           (const StepStatus*)status);
+  else
+    launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0,
+           (const uint32_t*)bb.entry_offset, (uint32_t)n, 0u, (const uint2*)pb.rect, bb.tiles_x,
+           bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
+           (const StepStatus*)status);
   exclusive_scan_u32(ctx, bb.tile_count, bb.tile_start, T, small + 2, scan_sums);
   launch(ctx, "bin_scatter", scatter_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0, 0u,
          (const uint32_t*)bb.entry_gaussian, (const uint32_t*)bb.entry_tile,

@@ -240,6 +240,45 @@ struct StepReport {
   unsigned long long total;
 };
 
+// Tables of a batched step: many (scene, view) pairs composited in one launch per stage.
+// Pair v renders scene pair_scene[v] through cams[v]; its projections occupy
+// [work_base[v], work_base[v+1]) of the projection buffers (one per scene Gaussian), its
+// tiles [tile_base[v], tile_base[v+1]) of the global tile space and its pixels
+// [pix_base[v], pix_base[v+1]) of the image buffers.  Scene s owns parameter rows
+// [scene_off[s], scene_off[s] + scene_n[s]) of the shared planes; its pairs are
+// scene_pairs[scene_pair_begin[s] .. scene_pair_begin[s+1]) in step order.
+struct BatchDev {
+  uint32_t npairs, nscenes;
+  const uint32_t* pair_scene;
+  const uint32_t* work_base;
+  const uint32_t* tile_base;
+  const uint32_t* pix_base;
+  const uint32_t* tiles_x;
+  const uint32_t* width;
+  const uint32_t* height;
+  const CameraDev* cams;
+  const uint32_t* scene_off;
+  const uint32_t* scene_n;
+  const uint32_t* scene_pair_begin;
+  const uint32_t* scene_pairs;
+  const uint64_t* targets;         // RGB8 target image of each pair (device addresses)
+  const uint32_t* loss_block_base; // loss-kernel blocks of each pair (prefix)
+  const float* lr_mean;            // per-scene position learning rate of the step
+};
+
+// Largest k with base[k] <= x, for base[0] = 0 <= x (base has n + 1 entries).
+__device__ __forceinline__ uint32_t find_segment(const uint32_t* base, uint32_t n, uint32_t x) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (base[mid] <= x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 // Projection outputs for one view (device pointers, capacity n).
 struct ProjBuffers {
   // One 64-byte rasterizer record per Gaussian (a single cache line per gather):
@@ -282,6 +321,14 @@ void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stri
                       const splat_camera& cam, const splat_proj_opts& opts,
                       const splat_proj_grad* dproj, float* grads, const StepDyn* dyn = nullptr);
 void aos_to_planes(splat_ctx* ctx, const float* aos, size_t n, float* planes, size_t stride);
+// Batched: projections of every (scene, view) pair (total `work` rows), and the summed
+// parameter gradients of every scene Gaussian (total `rows` plane rows).
+void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,
+                           const BatchDev& bd, size_t work, float near_plane, int antialias,
+                           int degree, ProjBuffers pb);
+void project_backward_batch(splat_ctx* ctx, const float* params, size_t stride,
+                            const BatchDev& bd, size_t rows, float near_plane, int antialias,
+                            int degree, const splat_proj_grad* dproj, float* grads);
 void planes_to_aos(splat_ctx* ctx, const float* planes, size_t n, size_t stride, float* aos);
 
 // scan.hip
@@ -299,22 +346,38 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
                  BinBuffers& bb, size_t* capacity_io, std::map<std::string, DevBuf>& store,
                  uint64_t* keys_out);
 // The same without a host synchronisation: the entry arrays of `bb` (capacity `capacity`)
-// and its per-Gaussian / per-tile arrays must already be allocated.  The entry count goes
+// and its per-Gaussian / per-tile arrays must already be allocated.  With `bd`, the rows
+// are the projections of a batched step and bb.tiles_x * bb.tiles_y is the global tile
+// count (bb.tiles_y = 1).  The entry count goes
 // to `status`; a count above the capacity sets the abort flag and the entry kernels exit.
 void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
                       size_t capacity, uint32_t* small, unsigned long long* total64,
-                      uint32_t* big_list, uint32_t* scan_sums, StepStatus* status);
+                      uint32_t* big_list, uint32_t* scan_sums, StepStatus* status,
+                      const BatchDev* bd = nullptr);
 void bin_guard(splat_ctx* ctx, const unsigned long long* total64, size_t capacity,
                StepStatus* status);
 
 // raster.hip
+// Launch shape of the compositing kernels (results do not depend on it): entries per LDS
+// batch of the forward (32, 64, 128) and of the tree-reduced backward (16, 32), and
+// whether a fixed grid of per_cu workgroups per compute unit takes tiles from a counter.
+struct RasterVariant {
+  int fwd_batch = 64;
+  int bwd_batch = 32;
+  bool persistent = false;
+  unsigned per_cu = 4;
+};
+// With `bd` the tiles are the global tile space of a batched step (bb.tiles_x * bb.tiles_y
+// tiles; width/height unused).  `counter` (one word) is needed by persistent launches.
 void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
                     uint32_t width, uint32_t height, const float bg[3], float* image,
-                    float* final_t, uint32_t* n_contrib);
+                    float* final_t, uint32_t* n_contrib, const RasterVariant& rv = {},
+                    const BatchDev* bd = nullptr, uint32_t* counter = nullptr);
 void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
                      uint32_t width, uint32_t height, const float bg[3], const float* final_t,
                      const uint32_t* n_contrib, const float* dimage, bool ordered,
-                     float* slots, bool need_abs);
+                     float* slots, bool need_abs, const RasterVariant& rv = {},
+                     const BatchDev* bd = nullptr, uint32_t* counter = nullptr);
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
                   splat_proj_grad* grads, float* abs_grads, const StepStatus* status = nullptr);
 
@@ -329,6 +392,14 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
                            float lambda, bool want_l1, float* dimage, float* scratch,
                            double* loss_parts_host, const StepDyn* dyn = nullptr);
 size_t loss_scratch_floats(uint32_t width, uint32_t height);
+// Blocks of the loss kernels for one image (the batched grid is the sum over pairs).
+unsigned loss_blocks(uint32_t width, uint32_t height);
+// Batched: loss and dL/dimage of every pair in two launches; scratch holds 3 maps of
+// total_values floats (pixels x 3 over all pairs) and 2 floats per block; losses (optional,
+// device) gets one value per pair.
+void loss_forward_backward_batch(splat_ctx* ctx, const float* pred, const BatchDev& bd,
+                                 unsigned total_blocks, size_t total_values, float lambda,
+                                 float* dimage, float* scratch, float* losses);
 // The loss value of the last loss_forward_backward on `scratch`, into device memory
 // (`out_dev`, or dyn->loss_out when `dyn` is given).
 void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
@@ -341,7 +412,7 @@ void gaussian_kernel_host(float k[11]);
 void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, float* v,
                  size_t n, size_t stride, float c1, float c2, const splat_adam_state& st,
                  const splat_learning_rates& lr, const StepDyn* dyn = nullptr,
-                 const StepStatus* status = nullptr);
+                 const StepStatus* status = nullptr, const BatchDev* bd = nullptr);
 struct DensifyParams {
   float grad_threshold, size_limit, min_opacity, world_limit, shrink;
   int prune_big_world, has_max_screen;

@@ -75,7 +75,40 @@ __device__ __forceinline__ float block_sum(float v, float* sh) {
 }
 
 // Pass 1.  partial[block] = {sum SSIM, sum |x - y|} over the block's outputs.
-template <bool U8>
+// Block coordinates of a loss kernel: (strip, row chunk) of one image.  BATCH: the 1D grid
+// covers every pair's blocks (BatchDev::loss_block_base); the image pointers are advanced
+// to the pair's pixels, whose maps lie at the same pixel offsets.
+struct LossBlock {
+  int bx, by;
+  uint32_t flat;  // partial-sum index
+};
+
+template <bool BATCH>
+__device__ __forceinline__ LossBlock loss_block(const BatchDev& bd, uint32_t* W, uint32_t* H,
+                                                size_t* pix0, uint32_t* pair) {
+  LossBlock b;
+  if (BATCH) {
+    const uint32_t v = find_segment(bd.loss_block_base, bd.npairs, blockIdx.x);
+    const uint32_t local = blockIdx.x - bd.loss_block_base[v];
+    *W = bd.width[v];
+    *H = bd.height[v];
+    const uint32_t gx = (*W * 3 + kStrip - 1) / kStrip;
+    b.bx = (int)(local % gx);
+    b.by = (int)(local / gx);
+    b.flat = blockIdx.x;
+    *pix0 = bd.pix_base[v];
+    *pair = v;
+  } else {
+    b.bx = (int)blockIdx.x;
+    b.by = (int)blockIdx.y;
+    b.flat = blockIdx.y * gridDim.x + blockIdx.x;
+    *pix0 = 0;
+    *pair = 0;
+  }
+  return b;
+}
+
+template <bool U8, bool BATCH>
 __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict__ xin,
                                                          const void* __restrict__ yin,
                                                          uint32_t W, uint32_t H, Taps taps,
@@ -84,15 +117,26 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
                                                          float* __restrict__ d_exx,
                                                          float* __restrict__ d_exy,
                                                          float2* __restrict__ partial,
-                                                         const StepDyn* __restrict__ dyn) {
+                                                         const StepDyn* __restrict__ dyn,
+                                                         BatchDev bd) {
   if (dyn) yin = dyn->target;
+  size_t pix0;
+  uint32_t pair;
+  const LossBlock lb = loss_block<BATCH>(bd, &W, &H, &pix0, &pair);
+  if (BATCH) {
+    xin += pix0 * 3;
+    yin = reinterpret_cast<const void*>(bd.targets[pair]);
+    d_mu += pix0 * 3;
+    d_exx += pix0 * 3;
+    d_exy += pix0 * 3;
+  }
   __shared__ float sx[kStrip + 2 * kHalo], sy[kStrip + 2 * kHalo];
   __shared__ float red[4];
   const int width3 = (int)W * 3;
-  const int q0 = blockIdx.x * kStrip;
+  const int q0 = lb.bx * kStrip;
   const int q = q0 + threadIdx.x;
   const bool col_ok = q < width3;
-  const int ybeg = blockIdx.y * kRowsPerBlock;
+  const int ybeg = lb.by * kRowsPerBlock;
   const int yend = min(ybeg + kRowsPerBlock, (int)H);
   float h[5][11];
 #pragma unroll
@@ -164,11 +208,11 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
   }
   const float bs = block_sum(ssum, red);
   const float bl = block_sum(l1sum, red);
-  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = make_float2(bs, bl);
+  if (threadIdx.x == 0) partial[lb.flat] = make_float2(bs, bl);
 }
 
 // Pass 2.  MODE 0: photometric-loss gradient; MODE 1: d(mean SSIM)/dx only.
-template <bool U8, int MODE>
+template <bool U8, int MODE, bool BATCH>
 __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict__ xin,
                                                         const void* __restrict__ yin,
                                                         uint32_t W, uint32_t H, Taps taps,
@@ -177,14 +221,29 @@ __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict_
                                                         const float* __restrict__ d_exy,
                                                         float inv_n, float w1, float lambda,
                                                         float* __restrict__ grad,
-                                                        const StepDyn* __restrict__ dyn) {
+                                                        const StepDyn* __restrict__ dyn,
+                                                        BatchDev bd) {
   if (dyn) yin = dyn->target;
+  size_t pix0;
+  uint32_t pair;
+  const LossBlock lb = loss_block<BATCH>(bd, &W, &H, &pix0, &pair);
+  if (BATCH) {
+    xin += pix0 * 3;
+    yin = reinterpret_cast<const void*>(bd.targets[pair]);
+    d_mu += pix0 * 3;
+    d_exx += pix0 * 3;
+    d_exy += pix0 * 3;
+    grad += pix0 * 3;
+    // per image: n = W * H * 3 (the same operations as the host computes)
+    inv_n = 1.0f / (float)((size_t)W * H * 3);
+    w1 = (1.0f - lambda) * inv_n;
+  }
   __shared__ float s0[kStrip + 2 * kHalo], s1[kStrip + 2 * kHalo], s2[kStrip + 2 * kHalo];
   const int width3 = (int)W * 3;
-  const int q0 = blockIdx.x * kStrip;
+  const int q0 = lb.bx * kStrip;
   const int q = q0 + threadIdx.x;
   const bool col_ok = q < width3;
-  const int ybeg = blockIdx.y * kRowsPerBlock;
+  const int ybeg = lb.by * kRowsPerBlock;
   const int yend = min(ybeg + kRowsPerBlock, (int)H);
   float h[3][11];
 #pragma unroll
@@ -280,7 +339,49 @@ __global__ void loss_value_kernel(const float2* __restrict__ partial, uint32_t n
   *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
 }
 
+// Per-pair loss values of a batched step (each pair's partials summed in block order).
+__global__ void loss_values_batch_kernel(const float2* __restrict__ partial, BatchDev bd,
+                                         float lambda, float* __restrict__ out) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= bd.npairs) return;
+  double a = 0, b = 0;
+  for (uint32_t i = bd.loss_block_base[v]; i < bd.loss_block_base[v + 1]; i++) {
+    a += partial[i].x;
+    b += partial[i].y;
+  }
+  const float n = (float)((size_t)bd.width[v] * bd.height[v] * 3);
+  const float inv_n = 1.0f / n;
+  const float l1 = (float)b * inv_n;
+  const float ssim = (float)a / n;
+  out[v] = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
+}
+
 }  // namespace
+
+unsigned loss_blocks(uint32_t width, uint32_t height) {
+  return blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
+}
+
+void loss_forward_backward_batch(splat_ctx* ctx, const float* pred, const BatchDev& bd,
+                                 unsigned total_blocks, size_t total_values, float lambda,
+                                 float* dimage, float* scratch, float* losses) {
+  Taps taps;
+  gaussian_kernel_host(taps.k);
+  const float c1 = 0.01f * 0.01f, c2 = 0.03f * 0.03f;
+  float* d_mu = scratch;
+  float* d_exx = scratch + total_values;
+  float* d_exy = scratch + 2 * total_values;
+  float2* partial = reinterpret_cast<float2*>(scratch + 3 * total_values);
+  launch(ctx, "loss_ssim_stats", ssim_stats_kernel<true, true>, dim3(total_blocks), dim3(256), 0,
+         pred, (const void*)nullptr, 0u, 0u, taps, c1, c2, d_mu, d_exx, d_exy, partial,
+         (const StepDyn*)nullptr, bd);
+  launch(ctx, "loss_grad", ssim_grad_kernel<true, 0, true>, dim3(total_blocks), dim3(256), 0,
+         pred, (const void*)nullptr, 0u, 0u, taps, (const float*)d_mu, (const float*)d_exx,
+         (const float*)d_exy, 0.0f, 0.0f, lambda, dimage, (const StepDyn*)nullptr, bd);
+  if (losses)
+    launch(ctx, "loss_value", loss_values_batch_kernel, dim3(blocks_for(bd.npairs, 64)),
+           dim3(64), 0, (const float2*)partial, bd, lambda, losses);
+}
 
 void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
                    float lambda, float* out_dev, const StepDyn* dyn) {
@@ -327,27 +428,31 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
   const float inv_n = 1.0f / (float)n;
   const float w1 = (1.0f - lambda) * inv_n;
   if (target_u8) {
-    launch(ctx, "loss_ssim_stats", ssim_stats_kernel<true>, grid, dim3(256), 0, pred,
-           (const void*)target_u8, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn);
+    launch(ctx, "loss_ssim_stats", ssim_stats_kernel<true, false>, grid, dim3(256), 0, pred,
+           (const void*)target_u8, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn, BatchDev{});
     if (want_l1)
-      launch(ctx, "loss_grad", ssim_grad_kernel<true, 0>, grid, dim3(256), 0, pred,
+      launch(ctx, "loss_grad", ssim_grad_kernel<true, 0, false>, grid, dim3(256), 0, pred,
              (const void*)target_u8, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
+             BatchDev{});
     else
-      launch(ctx, "loss_grad", ssim_grad_kernel<true, 1>, grid, dim3(256), 0, pred,
+      launch(ctx, "loss_grad", ssim_grad_kernel<true, 1, false>, grid, dim3(256), 0, pred,
              (const void*)target_u8, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
+             BatchDev{});
   } else {
-    launch(ctx, "loss_ssim_stats", ssim_stats_kernel<false>, grid, dim3(256), 0, pred,
-           (const void*)target_f32, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn);
+    launch(ctx, "loss_ssim_stats", ssim_stats_kernel<false, false>, grid, dim3(256), 0, pred,
+           (const void*)target_f32, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn, BatchDev{});
     if (want_l1)
-      launch(ctx, "loss_grad", ssim_grad_kernel<false, 0>, grid, dim3(256), 0, pred,
+      launch(ctx, "loss_grad", ssim_grad_kernel<false, 0, false>, grid, dim3(256), 0, pred,
              (const void*)target_f32, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
+             BatchDev{});
     else
-      launch(ctx, "loss_grad", ssim_grad_kernel<false, 1>, grid, dim3(256), 0, pred,
+      launch(ctx, "loss_grad", ssim_grad_kernel<false, 1, false>, grid, dim3(256), 0, pred,
              (const void*)target_f32, width, height, taps, (const float*)d_mu,
-             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn);
+             (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
+             BatchDev{});
   }
   if (loss_parts_host) {
     const size_t nb = (size_t)grid.x * grid.y;

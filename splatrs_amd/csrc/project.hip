@@ -285,21 +285,17 @@ __device__ __forceinline__ void raster_record(float mx, float my, float A, float
   *rec2 = make_float4(hy, r, gcol, bcol);
 }
 
-__global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restrict__ params,
-                                                          uint32_t n, uint32_t stride,
-                                                          CameraDev cam, float near_plane,
-                                                          float eps2d, int antialias, int degree,
-                                                          ProjBuffers pb,
-                                                          const StepDyn* __restrict__ dyn) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (dyn) cam = dyn->cam;
+// project_one (src/project.rs:277-330) of parameter row pi into projection row i.
+__device__ __forceinline__ void project_row(const float* __restrict__ params, size_t stride,
+                                            size_t pi, const CameraDev& cam, float near_plane,
+                                            float eps2d, int antialias, int degree,
+                                            const ProjBuffers& pb, size_t i) {
   float g[kParams];
   const int ncoef = (degree + 1) * (degree + 1);
 #pragma unroll
   for (int j = 0; j < kParams; j++) {
     // SH coefficients beyond the active degree are never read by the forward pass.
-    g[j] = (j < 11 + 3 * ncoef) ? params[(size_t)j * stride + i] : 0.0f;
+    g[j] = (j < 11 + 3 * ncoef) ? params[(size_t)j * stride + pi] : 0.0f;
   }
   Core k;
   bool ok = core(g, cam, near_plane, eps2d, antialias != 0, degree, k);
@@ -338,6 +334,32 @@ __global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restric
   pb.radius[i] = as_u32(r);
   pb.rect[i] = make_uint2(tx0 | (ty0 << 16), tx1 | (ty1 << 16));
   pb.ntiles[i] = (tx1 - tx0) * (ty1 - ty0);
+}
+
+__global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restrict__ params,
+                                                          uint32_t n, uint32_t stride,
+                                                          CameraDev cam, float near_plane,
+                                                          float eps2d, int antialias, int degree,
+                                                          ProjBuffers pb,
+                                                          const StepDyn* __restrict__ dyn) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (dyn) cam = dyn->cam;
+  project_row(params, stride, i, cam, near_plane, eps2d, antialias, degree, pb, i);
+}
+
+// All (scene, view) pairs of a batched step: row w of the projection buffers is Gaussian
+// w - work_base[v] of pair v's scene seen through pair v's camera.
+__global__ void __launch_bounds__(256) project_fwd_batch_kernel(const float* __restrict__ params,
+                                                                uint32_t stride, BatchDev bd,
+                                                                uint32_t work, float near_plane,
+                                                                float eps2d, int antialias,
+                                                                int degree, ProjBuffers pb) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= work) return;
+  const uint32_t v = find_segment(bd.work_base, bd.npairs, w);
+  const uint32_t pi = bd.scene_off[bd.pair_scene[v]] + (w - bd.work_base[v]);
+  project_row(params, stride, pi, bd.cams[v], near_plane, eps2d, antialias, degree, pb, w);
 }
 
 // ProjBuffers -> Proj records (src/project.rs:47-64).
@@ -409,25 +431,17 @@ __global__ void proj_unpack_kernel(const splat_proj* __restrict__ in, uint32_t n
 }
 
 // project_backward_one, src/project.rs:335-474.
-__global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restrict__ params,
-                                                          uint32_t n, uint32_t stride,
-                                                          CameraDev cam, float near_plane,
-                                                          float eps2d, int antialias, int degree,
-                                                          const splat_proj_grad* __restrict__ dproj,
-                                                          float* __restrict__ grads,
-                                                          const StepDyn* __restrict__ dyn) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (dyn) cam = dyn->cam;
-  float g[kParams];
-#pragma unroll
-  for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
-  float out[kParams];
+// Gradient of one Gaussian (parameters g) for one view into out (zeroed here).
+__device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev& cam,
+                                                float near_plane, float eps2d, int antialias,
+                                                int degree,
+                                                const splat_proj_grad* __restrict__ dproj_row,
+                                                float* out) {
 #pragma unroll
   for (int j = 0; j < kParams; j++) out[j] = 0.0f;
   Core k;
   if (core(g, cam, near_plane, eps2d, antialias != 0, degree, k)) {
-    const splat_proj_grad dp = dproj[i];
+    const splat_proj_grad dp = *dproj_row;
     const int nco = (degree + 1) * (degree + 1);
     V3 bgrad[16];
     sh_basis_grad(degree, k.dir, bgrad);
@@ -567,8 +581,63 @@ __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restric
       for (int t = 0; t < 4; t++) out[6 + t] = (dq[t] - u[t] * dot) / nn;
     }
   }
+}
+
+__global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restrict__ params,
+                                                          uint32_t n, uint32_t stride,
+                                                          CameraDev cam, float near_plane,
+                                                          float eps2d, int antialias, int degree,
+                                                          const splat_proj_grad* __restrict__ dproj,
+                                                          float* __restrict__ grads,
+                                                          const StepDyn* __restrict__ dyn) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (dyn) cam = dyn->cam;
+  float g[kParams];
+#pragma unroll
+  for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
+  float out[kParams];
+  project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, dproj + i, out);
 #pragma unroll
   for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + i] = out[j];
+}
+
+// Batched: each scene Gaussian sums its per-view gradients over the scene's pairs in step
+// order (the first view's gradient is stored, later ones are added to it).  Padding rows
+// between scenes get zero gradients.
+__global__ void __launch_bounds__(256) project_bwd_batch_kernel(const float* __restrict__ params,
+                                                                uint32_t stride, BatchDev bd,
+                                                                uint32_t rows, float near_plane,
+                                                                float eps2d, int antialias,
+                                                                int degree,
+                                                                const splat_proj_grad* __restrict__ dproj,
+                                                                float* __restrict__ grads) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const uint32_t s = find_segment(bd.scene_off, bd.nscenes, p);
+  const uint32_t i = p - bd.scene_off[s];
+  const uint32_t k0 = bd.scene_pair_begin[s], k1 = bd.scene_pair_begin[s + 1];
+  if (i >= bd.scene_n[s] || k0 == k1) {
+#pragma unroll
+    for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + p] = 0.0f;
+    return;
+  }
+  float g[kParams];
+#pragma unroll
+  for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + p];
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t v = bd.scene_pairs[k];
+    float out[kParams];
+    project_bwd_row(g, bd.cams[v], near_plane, eps2d, antialias, degree,
+                    dproj + bd.work_base[v] + i, out);
+    if (k == k0) {
+#pragma unroll
+      for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + p] = out[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + p] += out[j];
+    }
+  }
 }
 
 // AoS (n x 59) <-> planes, staged through LDS so both sides are read and written with
@@ -630,6 +699,21 @@ void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stri
   launch(ctx, "project_bwd", project_bwd_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params,
          (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
          opts.antialias, opts.sh_degree, dproj, grads, dyn);
+}
+
+void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,
+                           const BatchDev& bd, size_t work, float near_plane, int antialias,
+                           int degree, ProjBuffers pb) {
+  launch(ctx, "project_fwd", project_fwd_batch_kernel, dim3(blocks_for(work, 256)), dim3(256), 0,
+         params, (uint32_t)stride, bd, (uint32_t)work, near_plane, 0.3f, antialias, degree, pb);
+}
+
+void project_backward_batch(splat_ctx* ctx, const float* params, size_t stride,
+                            const BatchDev& bd, size_t rows, float near_plane, int antialias,
+                            int degree, const splat_proj_grad* dproj, float* grads) {
+  launch(ctx, "project_bwd", project_bwd_batch_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0,
+         params, (uint32_t)stride, bd, (uint32_t)rows, near_plane, 0.3f, antialias, degree, dproj,
+         grads);
 }
 
 void aos_to_planes(splat_ctx* ctx, const float* aos, size_t n, float* planes, size_t stride) {

@@ -16,8 +16,6 @@
 namespace splat {
 namespace {
 
-constexpr int kBatch = 64;       // entries staged per LDS batch (forward)
-constexpr int kBwdBatch = 32;    // entries per batch (fast backward)
 constexpr int kOrdBatch = 8;     // entries per batch (ordered backward)
 
 struct TileInfo {
@@ -32,6 +30,42 @@ __device__ __forceinline__ TileInfo tile_info(uint32_t tile, uint32_t tiles_x, u
   t.w = min(W - t.gx0, (uint32_t)kTile);
   t.h = min(H - t.gy0, (uint32_t)kTile);
   return t;
+}
+
+// Geometry of one tile of the (possibly batched) tile space: its pixel rectangle, the
+// width of its view and the view's first pixel in the image buffers.
+struct TileGeom {
+  TileInfo ti;
+  uint32_t width;
+  size_t pix0;
+};
+
+template <bool BATCH>
+__device__ __forceinline__ TileGeom tile_geom(uint32_t tile, uint32_t tiles_x, uint32_t W,
+                                              uint32_t H, const BatchDev& bd) {
+  TileGeom t;
+  if (BATCH) {
+    const uint32_t v = find_segment(bd.tile_base, bd.npairs, tile);
+    t.ti = tile_info(tile - bd.tile_base[v], bd.tiles_x[v], bd.width[v], bd.height[v]);
+    t.width = bd.width[v];
+    t.pix0 = bd.pix_base[v];
+  } else {
+    t.ti = tile_info(tile, tiles_x, W, H);
+    t.width = W;
+    t.pix0 = 0;
+  }
+  return t;
+}
+
+// The workgroup's next tile: its block index, or (persistent launch) the next one handed
+// out by a shared counter.  Every lane sees the same tile.
+template <bool PERSIST>
+__device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* s_tile) {
+  if (!PERSIST) return blockIdx.x;
+  __syncthreads();  // the previous tile's LDS use is over
+  if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1u);
+  __syncthreads();
+  return *s_tile;
 }
 
 // Tile-local pixel range of one entry (the clip closure of src/rasterize.rs:158-164).
@@ -74,17 +108,25 @@ __device__ __forceinline__ bool evaluate(float4 r0, float4 r1, float px, float p
   return true;
 }
 
+// B: entries per LDS batch; BATCH: tiles of many views (BatchDev); PERSIST: a fixed grid
+// that takes tiles from `counter` until `num_tiles` are done.
+template <int B, bool BATCH, bool PERSIST>
 __global__ void __launch_bounds__(256) raster_fwd_kernel(
     const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
-    const uint32_t* __restrict__ sorted_gauss, uint32_t tiles_x, uint32_t W, uint32_t H,
-    float bg0, float bg1, float bg2, float* __restrict__ image, float* __restrict__ final_t,
-    uint32_t* __restrict__ n_contrib) {
+    const uint32_t* __restrict__ sorted_gauss, uint32_t num_tiles, uint32_t tiles_x, uint32_t W,
+    uint32_t H, float bg0, float bg1, float bg2, float* __restrict__ image,
+    float* __restrict__ final_t, uint32_t* __restrict__ n_contrib, BatchDev bd,
+    uint32_t* __restrict__ counter) {
   __shared__ uint64_t tab[32];
-  __shared__ float4 s0[kBatch], s1[kBatch], s2[kBatch];
-  __shared__ uint32_t srect[kBatch];
+  __shared__ float4 s0[B], s1[B], s2[B];
+  __shared__ uint32_t srect[B];
+  __shared__ uint32_t s_tile;
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
-  const uint32_t tile = blockIdx.x;
-  const TileInfo ti = tile_info(tile, tiles_x, W, H);
+  for (;;) {
+  const uint32_t tile = next_tile<PERSIST>(counter, &s_tile);
+  if (tile >= num_tiles) break;
+  const TileGeom tg = tile_geom<BATCH>(tile, tiles_x, W, H, bd);
+  const TileInfo ti = tg.ti;
   const uint32_t i = threadIdx.x % kTile, j = threadIdx.x / kTile;
   const bool valid = i < ti.w && j < ti.h;
   const float px = (float)(ti.gx0 + i) + 0.5f;
@@ -93,9 +135,9 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
   uint32_t nc = 0;
   bool done = !valid;
   const uint32_t start = tile_start[tile], end = tile_start[tile + 1];
-  for (uint32_t b0 = start; b0 < end; b0 += kBatch) {
+  for (uint32_t b0 = start; b0 < end; b0 += B) {
     if (__syncthreads_and(done ? 1 : 0)) break;
-    const uint32_t cnt = min((uint32_t)kBatch, end - b0);
+    const uint32_t cnt = min((uint32_t)B, end - b0);
     if (threadIdx.x < cnt) {
       const uint32_t g = sorted_gauss[b0 + threadIdx.x];
       const float4 a = rec[4 * g], b = rec[4 * g + 1], c = rec[4 * g + 2];
@@ -133,12 +175,14 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
     }
   }
   if (valid) {
-    const size_t pix = (size_t)(ti.gy0 + j) * W + ti.gx0 + i;
+    const size_t pix = tg.pix0 + (size_t)(ti.gy0 + j) * tg.width + ti.gx0 + i;
     image[pix * 3 + 0] = c0 + T * bg0;
     image[pix * 3 + 1] = c1 + T * bg1;
     image[pix * 3 + 2] = c2 + T * bg2;
     final_t[pix] = T;
     n_contrib[pix] = nc;
+  }
+  if (!PERSIST) break;
   }
 }
 
@@ -220,24 +264,30 @@ __device__ __forceinline__ float row_sum16(float v) {
 // ORDERED: per-pixel terms staged in LDS and summed sequentially in row-major order.
 // Otherwise: 16-lane row sums by DPP, 16 row partials per entry summed in a fixed order.
 // ABS: also accumulate the AbsGS slots (9, 10); otherwise they are written as zero.
-template <bool ORDERED, bool ABS>
+// BF: entries per LDS batch of the tree-reduced form (the ordered form uses kOrdBatch).
+template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST>
 __global__ void __launch_bounds__(256) raster_bwd_kernel(
     const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
     const uint32_t* __restrict__ sorted_gauss, const uint32_t* __restrict__ sorted_entry,
-    uint32_t tiles_x, uint32_t W, uint32_t H, float bg0, float bg1, float bg2,
-    const float* __restrict__ final_t, const uint32_t* __restrict__ n_contrib,
-    const float* __restrict__ dimage, float* __restrict__ slots) {
-  constexpr int B = ORDERED ? kOrdBatch : kBwdBatch;
+    uint32_t num_tiles, uint32_t tiles_x, uint32_t W, uint32_t H, float bg0, float bg1,
+    float bg2, const float* __restrict__ final_t, const uint32_t* __restrict__ n_contrib,
+    const float* __restrict__ dimage, float* __restrict__ slots, BatchDev bd,
+    uint32_t* __restrict__ counter) {
+  constexpr int B = ORDERED ? kOrdBatch : BF;
   __shared__ uint64_t tab[32];
   __shared__ float4 s0[B], s1[B], s2[B];
   __shared__ uint32_t srect[B];
   __shared__ uint32_t smax_last;
+  __shared__ uint32_t s_tile;
   // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums.
   __shared__ float red[ORDERED ? B * kSlots * 256 : B * kSlots * 16];
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
+  for (;;) {
+  const uint32_t tile = next_tile<PERSIST>(counter, &s_tile);
+  if (tile >= num_tiles) break;
   if (threadIdx.x == 0) smax_last = 0;
-  const uint32_t tile = blockIdx.x;
-  const TileInfo ti = tile_info(tile, tiles_x, W, H);
+  const TileGeom tg = tile_geom<BATCH>(tile, tiles_x, W, H, bd);
+  const TileInfo ti = tg.ti;
   const uint32_t i = threadIdx.x % kTile, j = threadIdx.x / kTile;
   const bool valid = i < ti.w && j < ti.h;
   const float px = (float)(ti.gx0 + i) + 0.5f;
@@ -252,7 +302,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   s.bg_dot = 0.0f;
   s.last = 0;
   if (valid) {
-    const size_t pix = (size_t)(ti.gy0 + j) * W + ti.gx0 + i;
+    const size_t pix = tg.pix0 + (size_t)(ti.gy0 + j) * tg.width + ti.gx0 + i;
     s.d0 = dimage[pix * 3 + 0];
     s.d1 = dimage[pix * 3 + 1];
     s.d2 = dimage[pix * 3 + 2];
@@ -327,6 +377,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       slots[(size_t)sorted_entry[start + lo + kk] * kSlots + slot] = acc;
     }
   }
+  if (!PERSIST) break;
+  }
 }
 
 // Sum each Gaussian's slots over its entries in entry order (src/rasterize.rs:480-503).
@@ -367,33 +419,134 @@ __global__ void __launch_bounds__(256) reduce_slots_kernel(const uint32_t* __res
 
 }  // namespace
 
+namespace {
+
+// Workgroups of a persistent launch: `per_cu` per compute unit.
+unsigned persistent_grid(splat_ctx* ctx, unsigned per_cu, unsigned tiles) {
+  int cus = 0;
+  SPLAT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  return std::max(1u, std::min<unsigned>(tiles, (unsigned)cus * std::max(per_cu, 1u)));
+}
+
+template <int B, bool BATCH, bool PERSIST>
+void fwd_go(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, unsigned T,
+            uint32_t width, uint32_t height, const float bg[3], float* image, float* final_t,
+            uint32_t* n_contrib, const BatchDev& bd, uint32_t* counter, unsigned per_cu) {
+  if (PERSIST) fill_u32(ctx, counter, 1, 0u);
+  const unsigned grid = PERSIST ? persistent_grid(ctx, per_cu, T) : T;
+  launch(ctx, "raster_fwd", raster_fwd_kernel<B, BATCH, PERSIST>, dim3(grid), dim3(256), 0,
+         (const float4*)pb.rec, (const uint32_t*)bb.tile_start, (const uint32_t*)bb.sorted_gauss,
+         T, bb.tiles_x, width, height, bg[0], bg[1], bg[2], image, final_t, n_contrib, bd,
+         counter);
+}
+
+template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST>
+void bwd_go(splat_ctx* ctx, const char* name, const ProjBuffers& pb, const BinBuffers& bb,
+            unsigned T, uint32_t width, uint32_t height, const float bg[3], const float* final_t,
+            const uint32_t* n_contrib, const float* dimage, float* slots, const BatchDev& bd,
+            uint32_t* counter, unsigned per_cu) {
+  if (PERSIST) fill_u32(ctx, counter, 1, 0u);
+  const unsigned grid = PERSIST ? persistent_grid(ctx, per_cu, T) : T;
+  launch(ctx, name, raster_bwd_kernel<ORDERED, ABS, BF, BATCH, PERSIST>, dim3(grid), dim3(256),
+         0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start,
+         (const uint32_t*)bb.sorted_gauss, (const uint32_t*)bb.sorted_entry, T, bb.tiles_x, width,
+         height, bg[0], bg[1], bg[2], final_t, n_contrib, dimage, slots, bd, counter);
+}
+
+template <bool BATCH, bool PERSIST>
+void fwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
+              const BinBuffers& bb, unsigned T, uint32_t width, uint32_t height,
+              const float bg[3], float* image, float* final_t, uint32_t* n_contrib,
+              const BatchDev& bd, uint32_t* counter) {
+  switch (rv.fwd_batch) {
+    case 32:
+      return fwd_go<32, BATCH, PERSIST>(ctx, pb, bb, T, width, height, bg, image, final_t,
+                                        n_contrib, bd, counter, rv.per_cu);
+    case 128:
+      return fwd_go<128, BATCH, PERSIST>(ctx, pb, bb, T, width, height, bg, image, final_t,
+                                         n_contrib, bd, counter, rv.per_cu);
+    default:
+      return fwd_go<64, BATCH, PERSIST>(ctx, pb, bb, T, width, height, bg, image, final_t,
+                                        n_contrib, bd, counter, rv.per_cu);
+  }
+}
+
+template <bool BATCH, bool PERSIST>
+void bwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
+              const BinBuffers& bb, unsigned T, uint32_t width, uint32_t height,
+              const float bg[3], const float* final_t, const uint32_t* n_contrib,
+              const float* dimage, bool ordered, float* slots, bool need_abs, const BatchDev& bd,
+              uint32_t* counter) {
+  const unsigned pc = rv.per_cu;
+  if (ordered)
+    return bwd_go<true, true, 8, BATCH, PERSIST>(ctx, "raster_bwd_ordered", pb, bb, T, width,
+                                                 height, bg, final_t, n_contrib, dimage, slots,
+                                                 bd, counter, pc);
+  if (rv.bwd_batch == 16) {
+    if (need_abs)
+      return bwd_go<false, true, 16, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
+                                                     bg, final_t, n_contrib, dimage, slots, bd,
+                                                     counter, pc);
+    return bwd_go<false, false, 16, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
+                                                    bg, final_t, n_contrib, dimage, slots, bd,
+                                                    counter, pc);
+  }
+  if (need_abs)
+    return bwd_go<false, true, 32, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
+                                                   bg, final_t, n_contrib, dimage, slots, bd,
+                                                   counter, pc);
+  return bwd_go<false, false, 32, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
+                                                  bg, final_t, n_contrib, dimage, slots, bd,
+                                                  counter, pc);
+}
+
+}  // namespace
+
 void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, uint32_t width,
                     uint32_t height, const float bg[3], float* image, float* final_t,
-                    uint32_t* n_contrib) {
+                    uint32_t* n_contrib, const RasterVariant& rv, const BatchDev* bd,
+                    uint32_t* counter) {
   const unsigned T = bb.tiles_x * bb.tiles_y;
-  launch(ctx, "raster_fwd", raster_fwd_kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec,
-         (const uint32_t*)bb.tile_start,
-         (const uint32_t*)bb.sorted_gauss, bb.tiles_x, width, height, bg[0], bg[1], bg[2], image,
-         final_t, n_contrib);
+  const BatchDev none{};
+  SPLAT_REQUIRE(!rv.persistent || counter, "persistent launch needs a counter");
+  if (bd) {
+    if (rv.persistent)
+      fwd_pick<true, true>(ctx, rv, pb, bb, T, width, height, bg, image, final_t, n_contrib, *bd,
+                           counter);
+    else
+      fwd_pick<true, false>(ctx, rv, pb, bb, T, width, height, bg, image, final_t, n_contrib,
+                            *bd, counter);
+  } else if (rv.persistent) {
+    fwd_pick<false, true>(ctx, rv, pb, bb, T, width, height, bg, image, final_t, n_contrib, none,
+                          counter);
+  } else {
+    fwd_pick<false, false>(ctx, rv, pb, bb, T, width, height, bg, image, final_t, n_contrib,
+                           none, counter);
+  }
 }
 
 void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, uint32_t width,
                      uint32_t height, const float bg[3], const float* final_t,
                      const uint32_t* n_contrib, const float* dimage, bool ordered, float* slots,
-                     bool need_abs) {
+                     bool need_abs, const RasterVariant& rv, const BatchDev* bd,
+                     uint32_t* counter) {
   const unsigned T = bb.tiles_x * bb.tiles_y;
-  auto go = [&](const char* name, auto kernel) {
-    launch(ctx, name, kernel, dim3(T), dim3(256), 0, (const float4*)pb.rec,
-           (const uint32_t*)bb.tile_start,
-           (const uint32_t*)bb.sorted_gauss, (const uint32_t*)bb.sorted_entry, bb.tiles_x, width,
-           height, bg[0], bg[1], bg[2], final_t, n_contrib, dimage, slots);
-  };
-  if (ordered)
-    go("raster_bwd_ordered", raster_bwd_kernel<true, true>);
-  else if (need_abs)
-    go("raster_bwd", raster_bwd_kernel<false, true>);
-  else
-    go("raster_bwd", raster_bwd_kernel<false, false>);
+  const BatchDev none{};
+  SPLAT_REQUIRE(!rv.persistent || counter, "persistent launch needs a counter");
+  if (bd) {
+    if (rv.persistent)
+      bwd_pick<true, true>(ctx, rv, pb, bb, T, width, height, bg, final_t, n_contrib, dimage,
+                           ordered, slots, need_abs, *bd, counter);
+    else
+      bwd_pick<true, false>(ctx, rv, pb, bb, T, width, height, bg, final_t, n_contrib, dimage,
+                            ordered, slots, need_abs, *bd, counter);
+  } else if (rv.persistent) {
+    bwd_pick<false, true>(ctx, rv, pb, bb, T, width, height, bg, final_t, n_contrib, dimage,
+                          ordered, slots, need_abs, none, counter);
+  } else {
+    bwd_pick<false, false>(ctx, rv, pb, bb, T, width, height, bg, final_t, n_contrib, dimage,
+                           ordered, slots, need_abs, none, counter);
+  }
 }
 
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,

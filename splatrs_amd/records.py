@@ -44,6 +44,14 @@ LEARN_CONFIG = np.dtype([("mode", "<i4"), ("samples_per_event", "<u4"), ("max_ev
 LEARN_STATS = np.dtype([("logged", "<u8"), ("resolved", "<u8"), ("updates", "<u8"),
                         ("positives", "<u8"), ("events", "<u8"), ("last_loss", "<f4"),
                         ("pad_", "<u4")])
+BATCH_STEP_PARAMS = np.dtype([("sh_degree", "<i4"), ("ordered_backward", "<i4"),
+                              ("antialias", "<i4"), ("near_plane", "<f4"),
+                              ("lambda_dssim", "<f4"), ("background", "<f4", (3,)),
+                              ("lr", LEARNING_RATES)])
+TUNE_RESULT = np.dtype([("fwd_batch", "<i4"), ("fwd_persistent", "<i4"), ("fwd_per_cu", "<i4"),
+                        ("bwd_batch", "<i4"), ("bwd_persistent", "<i4"), ("bwd_per_cu", "<i4"),
+                        ("fwd_ms", "<f4"), ("bwd_ms", "<f4"), ("default_fwd_ms", "<f4"),
+                        ("default_bwd_ms", "<f4"), ("candidates", "<u4"), ("pad_", "<u4")])
 SCORER_PARAMS = 9347
 SCORER_FEATURES = 12
 
@@ -166,3 +174,16 @@ def learn_config(mode=1, samples_per_event=16384, max_events=64, outcome_delay=5
     c["eval_views"] = eval_views
     c["fit"] = fit if fit is not None else scorer_fit_config()
     return c
+
+
+def batch_step_params(sh_degree, lr, ordered_backward=False, antialias=False, near_plane=0.2,
+                      lambda_dssim=0.2, background=(0.0, 0.0, 0.0)):
+    p = np.zeros((), BATCH_STEP_PARAMS)
+    p["sh_degree"] = sh_degree
+    p["ordered_backward"] = int(bool(ordered_backward))
+    p["antialias"] = int(bool(antialias))
+    p["near_plane"] = near_plane
+    p["lambda_dssim"] = lambda_dssim
+    p["background"] = background
+    p["lr"] = lr
+    return p

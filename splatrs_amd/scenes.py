@@ -159,3 +159,82 @@ def large_scene(seed=0, n=1_000_000):
 def hd_camera():
     """One 1920x1080 camera, fx = fy = 1600, at distance 8 looking at the origin."""
     return look_at(1920, 1080, 1600.0, 1600.0, (0.0, 0.0, -8.0), (0.0, 0.0, 0.0))
+
+
+# ---- ellipsoid-surface scene (full training with densification) ------------------------
+
+SH_C0 = 0.28209479177387814
+
+
+def _quat_mats(q):
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def ellipsoids(rng, k=20):
+    """k random ellipsoids: centres U[-2.2, 2.2]^3, semi-axes U[0.25, 1.0], random
+    orientations, base colours U[0.1, 0.9]^3."""
+    return dict(center=rng.uniform(-2.2, 2.2, (k, 3)), radii=rng.uniform(0.25, 1.0, (k, 3)),
+                rot=_quat_mats(random_unit_quats(rng, k)), color=rng.uniform(0.1, 0.9, (k, 3)))
+
+
+def sample_surface(rng, ell, n):
+    """n points on the ellipsoid surfaces (ellipsoid chosen in proportion to its area, by
+    Thomsen's approximation), with their ellipsoid index."""
+    a, b, c = ell["radii"].T
+    p = 1.6075
+    area = 4 * np.pi * (((a * b) ** p + (a * c) ** p + (b * c) ** p) / 3) ** (1 / p)
+    idx = rng.choice(len(a), n, p=area / area.sum())
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    local = d * ell["radii"][idx]
+    pts = np.einsum("nij,nj->ni", ell["rot"][idx], local) + ell["center"][idx]
+    return pts, idx
+
+
+def surface_color(ell, idx, pts):
+    """Each ellipsoid's base colour with a smooth modulation and a fine 3D checker
+    (cells of ~0.08 units, a few pixels to tens of pixels on screen), so the surfaces carry
+    detail the initial point cloud cannot represent."""
+    smooth = 0.12 * np.sin(4.0 * pts[:, 0] + 3.0 * pts[:, 1]) * np.cos(5.0 * pts[:, 2])
+    cells = np.floor(pts * 12.0).astype(np.int64)
+    checker = np.where((cells.sum(axis=1) & 1) == 0, 0.3, -0.3)
+    rot = (idx % 3)[:, None] == np.arange(3)[None, :]   # the checker tints one channel
+    col = ell["color"][idx] + smooth[:, None] + checker[:, None] * (0.5 + 0.5 * rot)
+    return np.clip(col, 0.02, 0.98)
+
+
+def ellipsoid_scene(seed=0, n_gt=2_000_000, n_points=300_000, k=20):
+    """Ground-truth Gaussians on k ellipsoid surfaces (log-scales N(-4.6, 0.25), opacity
+    logits N(2, 0.5), coloured DC, higher bands N(0, 0.02)) and an initial point cloud of
+    n_points surface samples with per-point RGB8 colour."""
+    rng = np.random.default_rng(seed)
+    ell = ellipsoids(rng, k)
+    pos, idx = sample_surface(rng, ell, n_gt)
+    g = np.zeros((n_gt, NUM_PARAMS), np.float32)
+    g[:, 0:3] = pos
+    g[:, 3:6] = rng.normal(-4.6, 0.25, (n_gt, 3))
+    g[:, 6:10] = random_unit_quats(rng, n_gt)
+    g[:, 10] = rng.normal(2.0, 0.5, n_gt)
+    g[:, 11:14] = (surface_color(ell, idx, pos) - 0.5) / SH_C0
+    g[:, 14:] = rng.normal(0.0, 0.02, (n_gt, 45))
+    pts, pidx = sample_surface(np.random.default_rng(seed + 1), ell, n_points)
+    rgb = np.floor(surface_color(ell, pidx, pts) * 255 + 0.5).astype(np.uint8)
+    return g, pts.astype(np.float32), rgb
+
+
+def hemisphere_cameras(n=100, width=1297, height=840, f=1100.0, radius=6.0):
+    """n cameras on the upper hemisphere (world up is -y) at `radius`, looking at the
+    origin."""
+    cams = []
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    for i in range(n):
+        y = -(0.05 + 0.9 * (i + 0.5) / n)
+        r = math.sqrt(max(0.0, 1.0 - y * y))
+        th = golden * i
+        eye = (radius * r * math.cos(th), radius * y, radius * r * math.sin(th))
+        cams.append(look_at(width, height, f, f, eye, (0.0, 0.0, 0.0)))
+    return cams
