@@ -43,6 +43,7 @@ def algorithmic_bytes(name, N, V, E, P):
         "bin_scatter": 28 * E,
         "bin_sort_small": 24 * E,
         "stats": 24 * N,
+        "grad_adam": 44 * E + 1416 * N,    # slots read; params, m, v read and written
     }
     return table.get(name)
 
@@ -110,6 +111,93 @@ def learned_compare(iters, log=print):
             "gaussian_ratio": round(learned["gaussians"] / max(plain["gaussians"], 1), 4)}
 
 
+def batched_bench(steps, log=print):
+    """Batched configuration: 16 scenes of 50k-800k Gaussians (log-uniform; the small
+    configuration's generator) and 64 views (4 per scene, 320x240 .. 1600x1200) trained
+    in one graph-captured step, launch shapes autotuned.  Also times the same 64 views as
+    single-view trainer steps for comparison."""
+    import numpy as np
+    from splatrs_amd import abi, scenes
+    from splatrs_amd import records as rec
+    ctx = abi.Context(0)
+    t0 = time.perf_counter()
+    sizes = scenes.batch_scene_sizes(0)
+    pairs = scenes.batch_views(0)
+    models = [scenes.random_gaussians(np.random.default_rng(100 + s), n)
+              for s, n in enumerate(sizes)]
+    batch = abi.Batch(ctx)
+    for g in models:
+        batch.add_scene(g)
+    for s, n in enumerate(sizes):  # targets from a second seed of each scene
+        gt = abi.Trainer(ctx, scenes.random_gaussians(np.random.default_rng(200 + s), n))
+        mine = [(k, c) for k, (ps, c) in enumerate(pairs) if ps == s]
+        for _, c in mine:
+            gt.add_view(c, np.zeros((int(c["height"]), int(c["width"]), 3), np.uint8))
+        for j, (k, c) in enumerate(mine):
+            batch.add_view(s, c, abi.to_rgb8(gt.render(j, 3)))
+        gt.close()
+    extents = [scenes.scene_extent([c for ps, c in pairs if ps == s]) for s in range(len(sizes))]
+    views = list(range(len(pairs)))
+    pixels = sum(int(c["width"]) * int(c["height"]) for _, c in pairs)
+    log(f"batched scenes ready in {time.perf_counter() - t0:.1f} s: {sum(sizes)} Gaussians, "
+        f"{len(pairs)} views, {pixels / 1e6:.1f} MP")
+    lr = rec.learning_rates(0.0)
+    p = rec.batch_step_params(3, lr)
+    tune = batch.tune(views, p)
+    it = 0
+
+    def lr_mean(it):
+        return [rec.position_lr(it, e) for e in extents]
+    for _ in range(2):
+        it += 1
+        batch.step(views, p, lr_mean(it), want_losses=False)
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        it += 1
+        batch.step(views, p, lr_mean(it), want_losses=False)
+    ctx.sync()
+    sec = time.perf_counter() - t
+    info = batch.info()
+    batch.close()
+    # the same views as single-view trainer steps (one trainer per scene)
+    single_sec = 0.0
+    for s, n in enumerate(sizes):
+        tr = abi.Trainer(ctx, models[s])
+        mine = [c for ps, c in pairs if ps == s]
+        for c in mine:
+            tr.add_view(c, np.zeros((int(c["height"]), int(c["width"]), 3), np.uint8))
+        for j in range(len(mine)):
+            tr.step(j, rec.step_params(3, rec.learning_rates(1e-4)), result=False)
+        tr.sync()
+        t = time.perf_counter()
+        for _ in range(2):
+            for j in range(len(mine)):
+                tr.step(j, rec.step_params(3, rec.learning_rates(1e-4)), result=False)
+        tr.sync()
+        single_sec += (time.perf_counter() - t) / 2
+        tr.close()
+    ctx.close()
+    return {"workload": f"{len(sizes)} scenes ({min(sizes)}-{max(sizes)} Gaussians, "
+                        f"{sum(sizes)} total), {len(pairs)} views 320x240..1600x1200 "
+                        f"({pixels / 1e6:.1f} MP) per step",
+            "steps_per_s": round(steps / sec, 3), "ms_per_step": round(sec / steps * 1e3, 3),
+            "views_per_s": round(len(pairs) * steps / sec, 1),
+            "single_view_steps_views_per_s": round(len(pairs) / single_sec, 1),
+            "tuned_shapes": {"fwd_batch": int(tune["fwd_batch"]),
+                             "fwd_persistent": bool(tune["fwd_persistent"]),
+                             "fwd_per_cu": int(tune["fwd_per_cu"]),
+                             "bwd_batch": int(tune["bwd_batch"]),
+                             "bwd_persistent": bool(tune["bwd_persistent"]),
+                             "bwd_per_cu": int(tune["bwd_per_cu"]),
+                             "fwd_ms": round(float(tune["fwd_ms"]), 3),
+                             "bwd_ms": round(float(tune["bwd_ms"]), 3),
+                             "default_fwd_ms": round(float(tune["default_fwd_ms"]), 3),
+                             "default_bwd_ms": round(float(tune["default_bwd_ms"]), 3)},
+            "graph": {"captures": info["captures"], "replays": info["replays"],
+                      "aborted_steps": info["aborts"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,7 +214,14 @@ def main():
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain-vs-learned densification comparison")
     ap.add_argument("--learned-iters", type=int, default=7000)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the batched many-scene configuration")
+    ap.add_argument("--batched-steps", type=int, default=10)
     args = ap.parse_args()
+    if args.batched_only:
+        r = batched_bench(args.batched_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(r), flush=True)
+        return
     if args.learned_only:
         r = learned_compare(args.learned_iters,
                             log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -279,6 +279,26 @@ __device__ __forceinline__ uint32_t find_segment(const uint32_t* base, uint32_t 
   return lo;
 }
 
+// Learning rate of parameter plane j (LearningRates, src/adam.rs:11-26).
+__device__ __forceinline__ float plane_lr(int j, const splat_learning_rates& lr) {
+  if (j < 3) return lr.mean;
+  if (j < 6) return lr.log_scale;
+  if (j < 10) return lr.quat;
+  if (j == 10) return lr.opacity;
+  if (j < 14) return lr.sh_dc;
+  return lr.sh_rest;
+}
+
+// Adam's update of one scalar, src/adam.rs:44-61.
+__device__ __forceinline__ void adam_update(float& p, float g, float& m, float& v, float lr,
+                                            float b1, float b2, float c1, float c2, float eps) {
+  m = b1 * m + (1.0f - b1) * g;
+  v = b2 * v + (1.0f - b2) * g * g;
+  const float m_hat = m / c1;
+  const float v_hat = v / c2;
+  p -= lr * m_hat / (sqrtf(v_hat) + eps);
+}
+
 // Projection outputs for one view (device pointers, capacity n).
 struct ProjBuffers {
   // One 64-byte rasterizer record per Gaussian (a single cache line per gather):
@@ -321,6 +341,24 @@ void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stri
                       const splat_camera& cam, const splat_proj_opts& opts,
                       const splat_proj_grad* dproj, float* grads, const StepDyn* dyn = nullptr);
 void aos_to_planes(splat_ctx* ctx, const float* aos, size_t n, float* planes, size_t stride);
+// Densification statistics updated inside a fused kernel (grad_accum null: none).
+struct StatsArgs {
+  const uint32_t* radius;
+  float sx, sy;  // width / 2, height / 2
+  int absgrad;
+  float* grad_accum;
+  uint32_t* denom;
+  uint32_t* max_radii;
+  float* max_grad;  // optional extra statistics of the learned scorer
+  uint32_t* strong;
+  float strong_threshold;
+};
+// Fused tail of a train step: slot reduce + statistics + project backward + Adam.
+void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size_t stride,
+               const splat_camera& cam, const splat_proj_opts& opts,
+               const uint32_t* entry_offset, const float* slots, const StatsArgs& sa,
+               float c1, float c2, const splat_adam_state& st, const splat_learning_rates& lr,
+               const StepDyn* dyn = nullptr, const StepStatus* status = nullptr);
 // Batched: projections of every (scene, view) pair (total `work` rows), and the summed
 // parameter gradients of every scene Gaussian (total `rows` plane rows).
 void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,

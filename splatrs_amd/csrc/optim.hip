@@ -6,25 +6,6 @@
 namespace splat {
 namespace {
 
-__device__ __forceinline__ float plane_lr(int j, const splat_learning_rates& lr) {
-  if (j < 3) return lr.mean;
-  if (j < 6) return lr.log_scale;
-  if (j < 10) return lr.quat;
-  if (j == 10) return lr.opacity;
-  if (j < 14) return lr.sh_dc;
-  return lr.sh_rest;
-}
-
-// update, src/adam.rs:44-61.
-__device__ __forceinline__ void adam_update(float& p, float g, float& m, float& v, float lr,
-                                            float b1, float b2, float c1, float c2, float eps) {
-  m = b1 * m + (1.0f - b1) * g;
-  v = b2 * v + (1.0f - b2) * g * g;
-  const float m_hat = m / c1;
-  const float v_hat = v / c2;
-  p -= lr * m_hat / (sqrtf(v_hat) + eps);
-}
-
 // blockIdx.y = plane; four consecutive Gaussians per lane.
 __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ params,
                                                    const float* __restrict__ grads,

@@ -272,17 +272,24 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   loss_value_to(ctx, P.loss_scratch, P.width, P.height, P.lambda, nullptr, P.dyn);
   raster_backward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.final_t, P.n_contrib, P.dimage,
                   P.ordered != 0, P.slots, P.accumulate && P.absgrad);
-  reduce_slots(ctx, P.bb.entry_offset, P.slots, n, P.pgrad, P.absg, P.status);
-  project_backward(ctx, P.params, n, P.stride, cam0, opts, P.pgrad, P.grads, P.dyn);
-  if (P.accumulate)
-    stats_accumulate_dev(ctx, P.pb.radius, P.pgrad, P.absgrad ? P.absg : nullptr, n, P.width,
-                         P.height, P.grad_accum, P.denom, P.max_radii, P.status, P.max_grad,
-                         P.strong, P.strong_threshold);
+  StatsArgs sa{};
+  if (P.accumulate) {
+    sa.radius = P.pb.radius;
+    sa.sx = (float)P.width * 0.5f;
+    sa.sy = (float)P.height * 0.5f;
+    sa.absgrad = P.absgrad;
+    sa.grad_accum = P.grad_accum;
+    sa.denom = P.denom;
+    sa.max_radii = P.max_radii;
+    sa.max_grad = P.max_grad;
+    sa.strong = P.strong;
+    sa.strong_threshold = P.strong_threshold;
+  }
   splat_adam_state ast{0.9f, 0.999f, 1e-15f, 0, 0};
   splat_learning_rates lr0;
   std::memset(&lr0, 0, sizeof(lr0));
-  adam_planes(ctx, P.params, P.grads, P.m, P.v, n, P.stride, 1.0f, 1.0f, ast, lr0, P.dyn,
-              P.status);
+  grad_adam(ctx, P.params, P.m, P.v, n, P.stride, cam0, opts, P.bb.entry_offset, P.slots, sa,
+            1.0f, 1.0f, ast, lr0, P.dyn, P.status);
   launch(ctx, "step_report", step_report_kernel, dim3(1), dim3(64), 0, (const StepDyn*)P.dyn,
          (const StepStatus*)P.status, P.reports);
 }
@@ -637,20 +644,23 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     float* slots = (float*)t->store["slots"].reserve((E + 1) * kSlots * 4);
     raster_backward(ctx, pb, bb, W, H, p->background, final_t, n_contrib, dimage,
                     p->ordered_backward != 0, slots, p->accumulate_stats && p->absgrad);
-    splat_proj_grad* pgrad =
-        (splat_proj_grad*)t->store["pgrad"].reserve((t->cap + 1) * sizeof(splat_proj_grad));
-    float* absg = (float*)t->store["absg"].reserve((t->cap + 1) * 8);
-    reduce_slots(ctx, bb.entry_offset, slots, n, pgrad, absg);
-    project_backward(ctx, t->params.as<float>(), n, t->stride, tv.cam, opts, pgrad,
-                     t->grads.as<float>());
-    if (p->accumulate_stats)
-      stats_accumulate_dev(ctx, pb.radius, pgrad, p->absgrad ? absg : nullptr, n, W, H,
-                           t->grad_accum.as<float>(), t->denom.as<uint32_t>(),
-                           t->max_radii.as<uint32_t>(), nullptr,
-                           t->track ? t->max_grad.as<float>() : nullptr,
-                           t->track ? t->strong.as<uint32_t>() : nullptr, t->strong_threshold);
-
-    // optimiser (src/train.rs:238-247, src/adam.rs:77-90)
+    // slot reduce, statistics (src/train.rs:228-235), project backward and the optimiser
+    // step (src/train.rs:238-247, src/adam.rs:77-90) in one pass over the Gaussians
+    StatsArgs sa{};
+    if (p->accumulate_stats) {
+      sa.radius = pb.radius;
+      sa.sx = (float)W * 0.5f;
+      sa.sy = (float)H * 0.5f;
+      sa.absgrad = p->absgrad;
+      sa.grad_accum = t->grad_accum.as<float>();
+      sa.denom = t->denom.as<uint32_t>();
+      sa.max_radii = t->max_radii.as<uint32_t>();
+      if (t->track) {
+        sa.max_grad = t->max_grad.as<float>();
+        sa.strong = t->strong.as<uint32_t>();
+        sa.strong_threshold = t->strong_threshold;
+      }
+    }
     t->adam_step += 1;
     splat_adam_state st;
     st.beta1 = 0.9f;
@@ -659,8 +669,8 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     st.step = t->adam_step;
     const float c1 = 1.0f - powi(st.beta1, (int32_t)st.step);
     const float c2 = 1.0f - powi(st.beta2, (int32_t)st.step);
-    adam_planes(ctx, t->params.as<float>(), t->grads.as<float>(), t->m.as<float>(),
-                t->v.as<float>(), n, t->stride, c1, c2, st, p->lr);
+    grad_adam(ctx, t->params.as<float>(), t->m.as<float>(), t->v.as<float>(), n, t->stride,
+              tv.cam, opts, bb.entry_offset, slots, sa, c1, c2, st, p->lr);
 
     if (r) {
       r->num_entries = E;

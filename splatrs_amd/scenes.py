@@ -238,3 +238,31 @@ def hemisphere_cameras(n=100, width=1297, height=840, f=1100.0, radius=6.0):
         eye = (radius * r * math.cos(th), radius * y, radius * r * math.sin(th))
         cams.append(look_at(width, height, f, f, eye, (0.0, 0.0, 0.0)))
     return cams
+
+
+# ---- batched configuration: many scenes, many views of uneven resolution ------------------
+
+RESOLUTIONS = [(320, 240), (480, 360), (640, 480), (800, 600), (1024, 768), (1280, 960),
+               (1600, 1200)]
+
+
+def batch_scene_sizes(seed=0, scenes=16, lo=50_000, hi=800_000):
+    """Scene sizes log-uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return [int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(scenes)]
+
+
+def batch_views(seed=0, scenes=16, per_scene=4, radius=4.0):
+    """per_scene cameras per scene on a radius-4 sphere looking at the origin, resolutions
+    drawn from 320x240 .. 1600x1200 (fx = fy = 1.1 width).  Returns [(scene, camera)]."""
+    rng = np.random.default_rng(seed + 1000)
+    dirs = fibonacci_sphere(per_scene * scenes)
+    out = []
+    for s in range(scenes):
+        for k in range(per_scene):
+            d = dirs[(k * scenes + s) % len(dirs)]
+            w, h = RESOLUTIONS[int(rng.integers(0, len(RESOLUTIONS)))]
+            eye = (radius * d[0], radius * d[1], radius * d[2])
+            up = (0.0, -1.0, 0.0) if abs(d[1]) < 0.99 else (0.0, 0.0, 1.0)
+            out.append((s, look_at(w, h, 1.1 * w, 1.1 * w, eye, (0.0, 0.0, 0.0), up)))
+    return out
