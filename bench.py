@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched many-scene configuration")
     ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched and plain-vs-learned sections of the JSON line")
     args = ap.parse_args()
     if args.batched_only:
         r = batched_bench(args.batched_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
@@ -362,6 +364,16 @@ def main():
         "cpu_baseline": cpu,
         "kernels": kernels,
     }
+    if not args.no_extras:
+        trainer.close()
+        ctx.close()
+        err = lambda m: print(m, file=sys.stderr, flush=True)
+        for key, fn in (("batched", lambda: batched_bench(args.batched_steps, err)),
+                        ("learned", lambda: learned_compare(args.learned_iters, err))):
+            try:
+                out[key] = fn()
+            except Exception as e:  # the headline line is printed regardless
+                out[key] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out), flush=True)
 
 

@@ -9,7 +9,8 @@
 //   * mode 1 (learn): the reference's decisions are applied unchanged, and a sample of them
 //     is logged (features, decision bits, parent tag).  Once `outcome_delay` iterations have
 //     passed, each logged clone/split is labelled by whether any descendant of it is still
-//     alive (a densification whose offspring were all pruned again is labelled negative);
+//     alive with opacity >= useful_opacity (a densification whose offspring were pruned or
+//     faded is labelled negative);
 //     samples are weighted by the PSNR change over the same window, and the scorer is fit
 //     on every labelled sample so far;
 //   * mode 2 (learned): the scorer's logits decide clone / split / prune; the reference's
@@ -126,15 +127,19 @@ __global__ void __launch_bounds__(256) log_kernel(
   sample_of[i] = id;
 }
 
-// Alive descendants of the samples [b, b + c): every live row walks up its lineage (sample
-// ids decrease towards older decisions) and credits the samples of this event it meets.
+// Useful descendants of the samples [b, b + c): every live row whose opacity logit is at
+// least min_logit walks up its lineage (sample ids decrease towards older decisions) and
+// credits the samples of this event it meets.
 __global__ void __launch_bounds__(256) alive_kernel(const uint32_t* __restrict__ origin,
                                                     uint32_t n,
+                                                    const float* __restrict__ opacity,
+                                                    float min_logit,
                                                     const uint32_t* __restrict__ s_parent,
                                                     uint32_t b, uint32_t c,
                                                     uint32_t* __restrict__ alive) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (opacity[i] < min_logit) return;
   uint32_t s = origin[i];
   for (int depth = 0; depth < 64 && s != kNoSample && s >= b; depth++) {
     if (s < b + c) atomicAdd(&alive[s - b], 1u);
@@ -229,7 +234,7 @@ uint32_t scan_total(splat_trainer* t, const uint32_t* flag, uint32_t* pos, size_
 
 void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float extent,
                    uint64_t seed, uint32_t iteration, int32_t sh_degree, const float bg[3],
-                   splat_densify_report* report) {
+                   uint32_t last_reset, splat_densify_report* report) {
   trainer_drain(t);
   splat_ctx* ctx = t->ctx;
   LearnState& L = *t->learn;
@@ -294,8 +299,13 @@ void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float exte
     if (e.count) {
       uint32_t* alive = (uint32_t*)L.alive.reserve((size_t)e.count * 4 + 4);
       fill_u32(ctx, alive, e.count, 0u);
+      // a descendant counts while it is visible enough (opacity >= useful_opacity); after
+      // an opacity reset inside the window every survivor counts
+      const bool reset_inside = last_reset > e.iteration;
+      const float min_logit = reset_inside ? -INFINITY : logit(L.cfg.useful_opacity);
       launch(ctx, "learn_alive", alive_kernel, dim3(blocks_for(t->n, 256)), dim3(256), 0,
              (const uint32_t*)t->origin.as<uint32_t>(), (uint32_t)t->n,
+             (const float*)(t->params.as<float>() + 10 * t->stride), min_logit,
              (const uint32_t*)L.parent.as<uint32_t>(), e.base, e.count, alive);
       const float wpos = rclamp(1.0f + (psnr - e.psnr), 0.25f, 4.0f);
       uint32_t* cnt = (uint32_t*)L.counter.reserve(16);
@@ -333,6 +343,7 @@ void splat_learn_config_default(splat_learn_config* c) {
   c->threshold = 0.5f;
   c->prune_threshold = 0.5f;
   c->eval_views = 8;
+  c->useful_opacity = 0.05f;
   splat_scorer_fit_config_default(&c->fit);
 }
 
@@ -347,7 +358,8 @@ int splat_trainer_attach_scorer(splat_trainer* t, splat_scorer* s, const splat_l
     }
     SPLAT_REQUIRE(cfg && (cfg->mode == 1 || cfg->mode == 2), "mode must be 1 (learn) or 2 (learned)");
     SPLAT_REQUIRE(cfg->threshold > 0.0f && cfg->threshold < 1.0f && cfg->prune_threshold > 0.0f &&
-                      cfg->prune_threshold < 1.0f,
+                      cfg->prune_threshold < 1.0f && cfg->useful_opacity > 0.0f &&
+                      cfg->useful_opacity < 1.0f,
                   "thresholds must lie in (0, 1)");
     SPLAT_REQUIRE(s->ctx == t->ctx, "scorer and trainer must share a context");
     auto L = std::make_unique<LearnState>();

@@ -87,6 +87,12 @@ __device__ __forceinline__ uint32_t entry_rect(float4 r0, float4 r1, float4 r2, 
   return x0 | (x1 << 8) | (y0 << 16) | (y1 << 24);
 }
 
+// Whether a packed rectangle has pixels in the 4 tile rows of wave w (rows 4w .. 4w+3).
+__device__ __forceinline__ bool wave_touches(uint32_t rc, uint32_t w) {
+  const uint32_t x0 = rc & 0xffu, x1 = (rc >> 8) & 0xffu, y0 = (rc >> 16) & 0xffu, y1 = rc >> 24;
+  return (x0 < x1) & (y0 < 4 * w + 4) & (y1 > 4 * w);
+}
+
 __device__ __forceinline__ bool in_rect(uint32_t rc, uint32_t i, uint32_t j) {
   // bitwise, not short-circuit: no divergent branches in the compositing loops
   return (i >= (rc & 0xffu)) & (i < ((rc >> 8) & 0xffu)) & (j >= ((rc >> 16) & 0xffu)) &
@@ -149,8 +155,14 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
     __syncthreads();
     // Predicated form of the per-pixel walk (src/rasterize.rs:262-290): every decision of
     // evaluate() and the T_MIN test is a lane predicate; only the exp is skipped when no
-    // lane of the wave needs it.
-    for (uint32_t k = 0; k < cnt; k++) {
+    // lane of the wave needs it.  Each wave (4 pixel rows) walks, in order, only the
+    // entries whose pixel rectangle reaches its rows (a ballot over the batch).
+    for (uint32_t c0k = 0; c0k < cnt; c0k += 64) {
+    const uint32_t kl = c0k + (threadIdx.x & 63);
+    uint64_t todo = __ballot(kl < cnt && wave_touches(srect[min(kl, cnt - 1)], threadIdx.x >> 6));
+    while (todo) {
+      const uint32_t k = c0k + (uint32_t)__builtin_ctzll(todo);
+      todo &= todo - 1;
       const float4 a = s0[k], b = s1[k];
       const float dx = px - a.x;
       const float dy = py - a.y;
@@ -172,6 +184,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
       c2 = comp ? c2 + w * c.w : c2;
       T = comp ? test_t : T;
       nc = comp ? b0 - start + k + 1 : nc;
+    }
     }
   }
   if (valid) {
@@ -323,6 +336,9 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     slots[(size_t)sorted_entry[start + k] * kSlots + q % kSlots] = 0.0f;
   }
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the last contributor of any pixel of this wave
+  uint32_t wave_last = valid ? s.last : 0u;
+  for (int d = 32; d >= 1; d >>= 1) wave_last = max(wave_last, (uint32_t)__shfl_xor((int)wave_last, d, 64));
   // Batches walk [0, kmax) from the back.
   for (int32_t hi = (int32_t)kmax; hi > 0; hi -= B) {
     const int32_t lo = max(hi - B, 0);
@@ -336,28 +352,38 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       s2[threadIdx.x] = c;
       srect[threadIdx.x] = entry_rect(a, b, c, ti);
     }
+    if (!ORDERED)  // partials of entries a wave skips stay zero
+      for (uint32_t q = threadIdx.x; q < cnt * kSlots * 16; q += blockDim.x) red[q] = 0.0f;
     __syncthreads();
-    for (int32_t kk = (int32_t)cnt - 1; kk >= 0; kk--) {
-      const uint32_t k = lo + kk;
-      float v[kSlots];
-      const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
-      const bool any = pair_terms(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v);
-      if (ORDERED) {
+    if (ORDERED) {
+      for (int32_t kk = (int32_t)cnt - 1; kk >= 0; kk--) {
+        const uint32_t k = lo + kk;
+        float v[kSlots];
+        const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
+        pair_terms(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v);
 #pragma unroll
         for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + threadIdx.x] = v[q];
-      } else {
-        constexpr int NS = ABS ? kSlots : 9;
-        const bool row_end = (lane & 15) == 15;
-        const int part = wave * 4 + (lane >> 4);
-        if (any) {
+      }
+    } else {
+      // each wave walks, back to front, only the entries that reach its rows and precede
+      // its last contributor (B <= 64: one ballot covers the batch)
+      constexpr int NS = ABS ? kSlots : 9;
+      const bool row_end = (lane & 15) == 15;
+      const int part = wave * 4 + (lane >> 4);
+      uint64_t todo = __ballot(lane < cnt && lo + lane < wave_last &&
+                               wave_touches(srect[min(lane, cnt - 1)], wave));
+      while (todo) {
+        const int kk = 63 - __builtin_clzll(todo);
+        todo &= ~(1ull << kk);
+        const uint32_t k = lo + kk;
+        float v[kSlots];
+        const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
+        if (pair_terms(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v)) {
 #pragma unroll
           for (int q = 0; q < NS; q++) {
             const float r = row_sum16(v[q]);
             if (row_end) red[(kk * kSlots + q) * 16 + part] = r;
           }
-        } else if (row_end) {
-#pragma unroll
-          for (int q = 0; q < NS; q++) red[(kk * kSlots + q) * 16 + part] = 0.0f;
         }
       }
     }

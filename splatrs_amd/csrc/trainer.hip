@@ -925,6 +925,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
     float* hist = (float*)t->store["loss_hist"].reserve((size_t)(log_iv ? log_iv : 1) * 4 + 4);
     ViewOrder order(cfg->seed, (uint32_t)t->views.size());
     t->strong_threshold = cfg->densify_grad_threshold;
+    uint32_t last_reset = 0;
     const float bg = cfg->white_background ? 1.0f : 0.0f;
     int32_t sh = 0;  // model.sh_degree starts at 0 (src/train.rs:186-187)
     float ema = 0.0f;
@@ -973,7 +974,8 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
           uint64_t s = cfg->seed ^ (0x5851f42d4c957f2dull * (uint64_t)it);
           splat_densify_report rep;
           if (t->learn) {
-            learn_densify(t, dcfg, extent, sm64_next(s), it, sh, p.background, &rep);
+            learn_densify(t, dcfg, extent, sm64_next(s), it, sh, p.background, last_reset,
+                          &rep);
           } else {
             const int s2 = splat_trainer_densify(t, &dcfg, extent, sm64_next(s), &rep);
             if (s2 != SPLAT_OK) fail(s2, ctx->err);
@@ -988,6 +990,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
         if (reset) {
           const int s3 = splat_trainer_reset_opacity(t, 0.01f);
           if (s3 != SPLAT_OK) fail(s3, ctx->err);
+          last_reset = it;
         }
       }
       const bool log_now = log_iv > 0 && (it % log_iv == 0 || it == cfg->iterations);

@@ -103,5 +103,5 @@ void learn_zero_stats(splat_trainer* t);
 // outcome labels of older events, online fit).
 void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float extent,
                    uint64_t seed, uint32_t iteration, int32_t sh_degree, const float bg[3],
-                   splat_densify_report* report);
+                   uint32_t last_reset, splat_densify_report* report);
 }  // namespace splat
