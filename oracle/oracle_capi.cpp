@@ -281,6 +281,7 @@ float oracle_train_step_f32(float* params, float* m, float* v, uint64_t n, uint6
   o.sh_degree = sh_degree;
   const uint32_t w = cam->width, h = cam->height;
   std::vector<Proj<float>> p(n);
+  ORACLE_PAR
   for (uint64_t i = 0; i < n; i++) p[i] = project_one<float>(params + i * 59, c, o);
   Binning b;
   if (!bin<float>(p.data(), n, w, h, b)) return NAN;
@@ -296,6 +297,7 @@ float oracle_train_step_f32(float* params, float* m, float* v, uint64_t n, uint6
   std::vector<float> ab(2 * n), grads(n * 59);
   rasterize_backward<float>(p.data(), n, b, w, h, bg, final_t.data(), nc.data(), dimage.data(),
                             pg.data(), ab.data());
+  ORACLE_PAR
   for (uint64_t i = 0; i < n; i++) project_backward_one<float>(params + i * 59, c, o, pg[i], &grads[i * 59]);
   if (accumulate_stats)
     stats_accumulate<float>(p.data(), pg.data(), absgrad ? ab.data() : nullptr, n, w, h,
@@ -489,6 +491,17 @@ void oracle_scorer_features_f32(const float* g, int64_t n, const float* grad_acc
     f[10] = log2f(1.0f + (float)(iteration - birth[i])) / 12.0f;
     f[11] = (fmaxf(fmaxf(p[3], p[4]), p[5]) - log_size) * 0.25f;
   }
+}
+
+// Threads of the parallel loops (1 = the scalar restatement); returns the count in use.
+int oracle_set_threads(int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+  return omp_get_max_threads();
+#else
+  (void)threads;
+  return 1;
+#endif
 }
 
 void oracle_bf16_round(const float* in, float* out, int64_t n) {

@@ -22,6 +22,19 @@
 #include <utility>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#include <parallel/algorithm>
+// Parallel loops along the reference's rayon axes (Gaussians, tiles, image rows, elements).
+// Every parallel loop writes disjoint outputs and keeps each output's arithmetic order, so
+// results are identical for any thread count.
+#define ORACLE_PAR _Pragma("omp parallel for schedule(dynamic, 64)")
+#define ORACLE_SORT __gnu_parallel::sort
+#else
+#define ORACLE_PAR
+#define ORACLE_SORT std::sort
+#endif
+
 namespace oracle {
 
 // Literal of type R spelled as the reference spells it (`0.3` is an f32 literal in the
@@ -677,11 +690,12 @@ bool bin(const Proj<R>* projs, size_t n, uint32_t width, uint32_t height, Binnin
   b.entry_offset[n] = (uint32_t)total;
   b.entry_gaussian.resize(total);
   b.sorted.resize(total);
-  size_t e = 0;
+  ORACLE_PAR
   for (size_t gi = 0; gi < n; gi++) {
     const Proj<R>& p = projs[gi];
     uint64_t depth_bits = f32_bits(p.depth);
     if (!p.visible()) continue;
+    size_t e = b.entry_offset[gi];  // this Gaussian's entries, in the reference's order
     for (uint32_t ty = p.tile_min[1]; ty < p.tile_max[1]; ty++)
       for (uint32_t tx = p.tile_min[0]; tx < p.tile_max[0]; tx++) {
         uint64_t tile = (uint64_t)(ty * b.tiles_x + tx);
@@ -690,7 +704,7 @@ bool bin(const Proj<R>* projs, size_t n, uint32_t width, uint32_t height, Binnin
         e++;
       }
   }
-  std::sort(b.sorted.begin(), b.sorted.end());
+  ORACLE_SORT(b.sorted.begin(), b.sorted.end());  // keys are unique: any sort gives one order
   size_t num_tiles = (size_t)b.tiles_x * b.tiles_y;
   b.tile_range.assign(num_tiles, std::make_pair(0u, 0u));
   size_t s = 0;
@@ -792,6 +806,7 @@ template <class R>
 void rasterize_forward(const Proj<R>* projs, const Binning& b, uint32_t width, uint32_t height,
                        const R bg[3], R* image, R* final_t, uint32_t* n_contrib) {
   size_t num_tiles = (size_t)b.tiles_x * b.tiles_y;
+  ORACLE_PAR
   for (size_t tile = 0; tile < num_tiles; tile++) {
     TileGeom geom = tile_geom(b, width, height, tile);
     uint32_t s = b.tile_range[tile].first, e = b.tile_range[tile].second;
@@ -848,6 +863,7 @@ void rasterize_backward(const Proj<R>* projs, size_t np, const Binning& b, uint3
   size_t num_entries = b.sorted.size();
   std::vector<R> slots(num_entries * SLOTS, (R)0);
   size_t num_tiles = (size_t)b.tiles_x * b.tiles_y;
+  ORACLE_PAR
   for (size_t tile = 0; tile < num_tiles; tile++) {
     TileGeom geom = tile_geom(b, width, height, tile);
     uint32_t start = b.tile_range[tile].first, end = b.tile_range[tile].second;
@@ -933,6 +949,7 @@ void rasterize_backward(const Proj<R>* projs, size_t np, const Binning& b, uint3
       for (int s = 0; s < SLOTS; s++) slots[(size_t)entry * SLOTS + s] = acc[kk * SLOTS + s];
     }
   }
+  ORACLE_PAR
   for (size_t g = 0; g < np; g++) {
     R sum[SLOTS];
     for (int k = 0; k < SLOTS; k++) sum[k] = 0;
@@ -975,6 +992,7 @@ void blur(const R* src, size_t width, size_t height, size_t channels, const R ke
   const long half = SSIM_WINDOW / 2;
   const size_t row = width * channels;
   std::vector<R> tmp(width * height * channels, (R)0);
+  ORACLE_PAR
   for (size_t y = 0; y < height; y++) {
     const R* line = src + y * row;
     R* out = &tmp[y * row];
@@ -989,6 +1007,7 @@ void blur(const R* src, size_t width, size_t height, size_t channels, const R ke
       }
   }
   for (size_t i = 0; i < width * height * channels; i++) dst[i] = 0;
+  ORACLE_PAR
   for (size_t y = 0; y < height; y++) {
     R* out = dst + y * row;
     for (int k = 0; k < SSIM_WINDOW; k++) {
@@ -1022,8 +1041,8 @@ R ssim_with_grad(const R* x, const R* y, size_t width, size_t height, R* grad) {
   blur(xx.data(), width, height, 3, kernel, e_xx.data());
   blur(yy.data(), width, height, 3, kernel, e_yy.data());
   blur(xy.data(), width, height, 3, kernel, e_xy.data());
-  std::vector<R> d_mu(n), d_exx(n), d_exy(n);
-  R ssim_sum = 0;
+  std::vector<R> d_mu(n), d_exx(n), d_exy(n), smap(n);
+  ORACLE_PAR
   for (size_t i = 0; i < n; i++) {
     R mx = mu_x[i], my = mu_y[i];
     R sx2 = e_xx[i] - mx * mx;
@@ -1041,14 +1060,17 @@ R ssim_with_grad(const R* x, const R* y, size_t width, size_t height, R* grad) {
     d_mu[i] = (dn_dmx * den - num * dd_dmx) / (den * den);
     d_exx[i] = -num * d1 / (den * den);
     d_exy[i] = RL(2.0) * n1 / den;
-    ssim_sum += s;
+    smap[i] = s;
   }
+  R ssim_sum = 0;
+  for (size_t i = 0; i < n; i++) ssim_sum += smap[i];
   R mean = ssim_sum / (R)n;
   std::vector<R> b_mu(n), b_exx(n), b_exy(n);
   blur(d_mu.data(), width, height, 3, kernel, b_mu.data());
   blur(d_exx.data(), width, height, 3, kernel, b_exx.data());
   blur(d_exy.data(), width, height, 3, kernel, b_exy.data());
   R inv_n = RL(1.0) / (R)n;
+  ORACLE_PAR
   for (size_t i = 0; i < n; i++)
     grad[i] = (b_mu[i] + RL(2.0) * x[i] * b_exx[i] + y[i] * b_exy[i]) * inv_n;
   return mean;
@@ -1068,6 +1090,7 @@ R photometric_loss(const R* pred, const R* target, size_t width, size_t height, 
   R ssim = ssim_with_grad(pred, target, width, height, dssim.data());
   R loss = (RL(1.0) - lambda) * l1 + lambda * (RL(1.0) - ssim);
   R w1 = (RL(1.0) - lambda) * inv_n;
+  ORACLE_PAR
   for (size_t i = 0; i < n; i++) {
     R d = pred[i] - target[i];
     R sign = d > (R)0 ? RL(1.0) : (d < (R)0 ? RL(-1.0) : RL(0.0));
@@ -1123,6 +1146,7 @@ void adam_step(R* params, const R* grads, R* m, R* v, size_t n, uint64_t* step, 
   *step += 1;
   R c1 = RL(1.0) - powi(b1, (int)*step);
   R c2 = RL(1.0) - powi(b2, (int)*step);
+  ORACLE_PAR
   for (size_t gi = 0; gi < n; gi++) {
     R* p = params + gi * 59;
     const R* g = grads + gi * 59;

@@ -63,6 +63,7 @@ def lib():
             "oracle_mlp_loss_grad_f64": (f64, [P, P, P, P, C.c_int64, P]),
             "oracle_mlp_adam": (None, [P, P, P, P, i32, f32, f32, f32, f32, f32, f32]),
             "oracle_bf16_round": (None, [P, P, C.c_int64]),
+            "oracle_set_threads": (i32, [i32]),
             "oracle_scorer_features_f32": (None, [P, C.c_int64, P, P, P, P, P, P, f32, f32, u32,
                                                   P]),
         }
@@ -390,3 +391,8 @@ def scorer_features(g, grad_accum, denom, max_radii, max_grad, strong, birth, th
     lib().oracle_scorer_features_f32(ptr(g), len(g), *[ptr(a) for a in arrs], thr, log_size,
                                      iteration, ptr(out))
     return out
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (1 = scalar); returns the count in use."""
+    return lib().oracle_set_threads(n)

@@ -212,3 +212,23 @@ def test_nn_distances_handle_coincident_points():
     assert abs(got[40] - 1.0) < 1e-12 and abs(got[41] - 4.0) < 1e-12
     two = O.mean_nn_dist2(np.array([[0, 0, 0], [0, 0, 3.0]]))
     assert abs(two[0] - 9) < 1e-12 and abs(two[1] - 9) < 1e-12
+
+
+def test_train_step_independent_of_threads():
+    """The oracle's parallel loops write disjoint outputs: a full train step gives the same
+    bits with one thread and with all of them."""
+    from splatrs_amd import scenes
+    g = scenes.small_scene(0, 3000)
+    cam = scenes.small_scene_cameras()[1]
+    target = np.random.default_rng(5).integers(0, 256, (256, 256, 3)).astype(np.uint8)
+    lr = rec.learning_rates(1e-4)
+    out = []
+    for threads in (1, 8):
+        O.set_threads(threads)
+        st = O.TrainState(g)
+        for _ in range(2):
+            st.train_step(cam, target, 3, lr)
+        out.append((st.params.copy(), st.m.copy(), st.v.copy()))
+    O.set_threads(0)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
