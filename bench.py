@@ -273,6 +273,15 @@ def main():
                         visible=int(r["num_visible"]))
     ctx.sync()
 
+    # autotuned compositing launch shapes (results do not depend on them)
+    tune = trainer.tune(view, params(it + 1))
+    tuned = {"fwd_batch": int(tune["fwd_batch"]), "fwd_persistent": bool(tune["fwd_persistent"]),
+             "fwd_per_cu": int(tune["fwd_per_cu"]), "bwd_batch": int(tune["bwd_batch"]),
+             "bwd_persistent": bool(tune["bwd_persistent"]), "bwd_per_cu": int(tune["bwd_per_cu"]),
+             "fwd_ms": round(float(tune["fwd_ms"]), 4), "bwd_ms": round(float(tune["bwd_ms"]), 4),
+             "default_fwd_ms": round(float(tune["default_fwd_ms"]), 4),
+             "default_bwd_ms": round(float(tune["default_bwd_ms"]), 4)}
+
     def timed_pass():
         nonlocal it
         trainer.sync()
@@ -372,7 +381,8 @@ def main():
                    "loss_at_step1": round(info["loss0"], 6),
                    "step_mode": "hipGraph replay",
                    "eager_it_per_s": round(args.steps / eager_sec, 3),
-                   "graph_captures_replays_aborts": list(graph_stats)},
+                   "graph_captures_replays_aborts": list(graph_stats),
+                   "tuned_shapes": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

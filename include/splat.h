@@ -447,6 +447,13 @@ typedef struct {
   uint32_t candidates, pad_;
 } splat_tune_result;
 
+/* No counterpart in the reference.  Autotuner of the single-view trainer: times every
+ * candidate launch shape of the compositing kernels on view `view` (the model is not
+ * changed) and keeps the fastest for later steps; set_shapes pins given shapes. */
+int splat_trainer_tune(splat_trainer* t, uint32_t view, const splat_step_params* p,
+                       splat_tune_result* out);
+int splat_trainer_set_shapes(splat_trainer* t, const splat_tune_result* shapes);
+
 int splat_batch_create(splat_ctx* ctx, splat_batch** out);
 int splat_batch_destroy(splat_batch* b);
 int splat_batch_add_scene(splat_batch* b, const float* gaussians, uint64_t n, uint32_t* scene);

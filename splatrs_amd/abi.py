@@ -181,6 +181,8 @@ _SIGS = {
     "splat_scorer_fit_config_default": (None, [P]),
     "splat_scorer_fit": (i32, [P, P, P, P, u64, P, P]),
     "splat_learn_config_default": (None, [P]),
+    "splat_trainer_tune": (i32, [P, u32, P, P]),
+    "splat_trainer_set_shapes": (i32, [P, P]),
     "splat_batch_create": (i32, [P, P]),
     "splat_batch_destroy": (i32, [P]),
     "splat_batch_add_scene": (i32, [P, P, u64, P]),
@@ -533,6 +535,15 @@ class Trainer:
 
     def sync(self):
         self.ctx.check(lib().splat_trainer_sync(self._h), "trainer_sync")
+
+    def tune(self, view, params):
+        """Autotune the compositing launch shapes on `view` (records.TUNE_RESULT)."""
+        r = np.zeros((), rec.TUNE_RESULT)
+        self.ctx.check(lib().splat_trainer_tune(self._h, view, _p(params), _p(r)), "trainer_tune")
+        return r
+
+    def set_shapes(self, shapes):
+        self.ctx.check(lib().splat_trainer_set_shapes(self._h, _p(shapes)), "trainer_set_shapes")
 
     def reserve_entries(self, entries):
         self.ctx.check(lib().splat_trainer_reserve_entries(self._h, entries),

@@ -501,83 +501,13 @@ int splat_batch_tune(splat_batch* b, const uint32_t* views, uint32_t nviews,
                    &b->bd, x.counter);
     loss_forward_backward_batch(ctx, x.image, b->bd, b->loss_blocks_total, b->pixels * 3,
                                 p->lambda_dssim, x.dimage, x.scratch, nullptr);
-    hipEvent_t e0, e1;
-    SPLAT_HIP(hipEventCreate(&e0));
-    SPLAT_HIP(hipEventCreate(&e1));
-    const bool timing = ctx->timing;
-    ctx->timing = false;
-    auto time_it = [&](auto&& fn) {
-      float best = 1e30f;
-      for (int r = 0; r < 3; r++) {
-        SPLAT_HIP(hipEventRecord(e0, ctx->stream));
-        fn();
-        SPLAT_HIP(hipEventRecord(e1, ctx->stream));
-        SPLAT_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        SPLAT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        best = std::min(best, ms);
-      }
-      return best;
-    };
-    std::vector<RasterVariant> shapes;
-    for (int fb : {32, 64, 128})
-      for (int mode = 0; mode < 4; mode++) {
-        RasterVariant rv;
-        rv.fwd_batch = fb;
-        rv.persistent = mode > 0;
-        rv.per_cu = mode == 1 ? 2 : (mode == 2 ? 4 : 8);
-        shapes.push_back(rv);
-      }
-    float best_f = 1e30f, best_b = 1e30f, def_f = 0, def_b = 0;
-    RasterVariant bf, bb_;
-    uint32_t cand = 0;
-    for (const RasterVariant& rv : shapes) {
-      const float ms = time_it([&] {
-        raster_forward(ctx, x.pb, x.bb, 0, 0, p->background, x.image, x.final_t, x.n_contrib,
-                       rv, &b->bd, x.counter);
-      });
-      cand++;
-      if (rv.fwd_batch == def.fwd_batch && !rv.persistent) def_f = ms;
-      if (ms < best_f) {
-        best_f = ms;
-        bf = rv;
-      }
-    }
-    for (int bw : {16, 32})
-      for (int mode = 0; mode < 4; mode++) {
-        RasterVariant rv;
-        rv.bwd_batch = bw;
-        rv.persistent = mode > 0;
-        rv.per_cu = mode == 1 ? 2 : (mode == 2 ? 4 : 8);
-        const float ms = time_it([&] {
-          raster_backward(ctx, x.pb, x.bb, 0, 0, p->background, x.final_t, x.n_contrib, x.dimage,
-                          p->ordered_backward != 0, x.slots, false, rv, &b->bd, x.counter);
-        });
-        cand++;
-        if (rv.bwd_batch == def.bwd_batch && !rv.persistent) def_b = ms;
-        if (ms < best_b) {
-          best_b = ms;
-          bb_ = rv;
-        }
-      }
-    ctx->timing = timing;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    b->fwd = bf;
-    b->bwd = bb_;
+    const RasterTune tr = tune_raster(ctx, x.pb, x.bb, 0, 0, p->background, x.image, x.final_t,
+                                      x.n_contrib, x.dimage, p->ordered_backward != 0, x.slots,
+                                      false, &b->bd, x.counter);
+    b->fwd = tr.fwd;
+    b->bwd = tr.bwd;
     b->tuned = true;
-    out->fwd_batch = bf.fwd_batch;
-    out->fwd_persistent = bf.persistent;
-    out->fwd_per_cu = (int32_t)bf.per_cu;
-    out->bwd_batch = bb_.bwd_batch;
-    out->bwd_persistent = bb_.persistent;
-    out->bwd_per_cu = (int32_t)bb_.per_cu;
-    out->fwd_ms = best_f;
-    out->bwd_ms = best_b;
-    out->default_fwd_ms = def_f;
-    out->default_bwd_ms = def_b;
-    out->candidates = cand;
-    out->pad_ = 0;
+    fill_tune_result(tr, out);
   });
 }
 

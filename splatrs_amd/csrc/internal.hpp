@@ -418,6 +418,18 @@ void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb
                      const BatchDev* bd = nullptr, uint32_t* counter = nullptr);
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
                   splat_proj_grad* grads, float* abs_grads, const StepStatus* status = nullptr);
+// Autotuner of the compositing launch shapes: times every candidate (best of 3, HIP events)
+// on buffers holding a binned view (forward) and its dL/dimage (backward).
+struct RasterTune {
+  RasterVariant fwd, bwd;
+  float fwd_ms = 0, bwd_ms = 0, default_fwd_ms = 0, default_bwd_ms = 0;
+  uint32_t candidates = 0;
+};
+RasterTune tune_raster(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                       uint32_t width, uint32_t height, const float bg[3], float* image,
+                       float* final_t, uint32_t* n_contrib, const float* dimage, bool ordered,
+                       float* slots, bool need_abs, const BatchDev* bd, uint32_t* counter);
+void fill_tune_result(const RasterTune& t, splat_tune_result* out);
 
 // loss.hip
 struct LossOut {

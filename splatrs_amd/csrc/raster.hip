@@ -575,6 +575,93 @@ void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb
   }
 }
 
+RasterTune tune_raster(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                       uint32_t width, uint32_t height, const float bg[3], float* image,
+                       float* final_t, uint32_t* n_contrib, const float* dimage, bool ordered,
+                       float* slots, bool need_abs, const BatchDev* bd, uint32_t* counter) {
+  hipEvent_t e0, e1;
+  SPLAT_HIP(hipEventCreate(&e0));
+  SPLAT_HIP(hipEventCreate(&e1));
+  const bool timing = ctx->timing;
+  ctx->timing = false;
+  RasterTune out;
+  try {
+    auto time_it = [&](auto&& fn) {
+      float best = 1e30f;
+      for (int r = 0; r < 3; r++) {
+        SPLAT_HIP(hipEventRecord(e0, ctx->stream));
+        fn();
+        SPLAT_HIP(hipEventRecord(e1, ctx->stream));
+        SPLAT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        SPLAT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+      }
+      return best;
+    };
+    const RasterVariant def;
+    out.fwd_ms = out.bwd_ms = 1e30f;
+    for (int fb : {32, 64, 128})
+      for (int mode = 0; mode < 4; mode++) {
+        RasterVariant rv;
+        rv.fwd_batch = fb;
+        rv.persistent = mode > 0;
+        rv.per_cu = mode == 1 ? 2 : (mode == 2 ? 4 : 8);
+        const float ms = time_it([&] {
+          raster_forward(ctx, pb, bb, width, height, bg, image, final_t, n_contrib, rv, bd,
+                         counter);
+        });
+        out.candidates++;
+        if (rv.fwd_batch == def.fwd_batch && !rv.persistent) out.default_fwd_ms = ms;
+        if (ms < out.fwd_ms) {
+          out.fwd_ms = ms;
+          out.fwd = rv;
+        }
+      }
+    for (int bw : {16, 32})
+      for (int mode = 0; mode < 4; mode++) {
+        RasterVariant rv;
+        rv.bwd_batch = bw;
+        rv.persistent = mode > 0;
+        rv.per_cu = mode == 1 ? 2 : (mode == 2 ? 4 : 8);
+        const float ms = time_it([&] {
+          raster_backward(ctx, pb, bb, width, height, bg, final_t, n_contrib, dimage, ordered,
+                          slots, need_abs, rv, bd, counter);
+        });
+        out.candidates++;
+        if (rv.bwd_batch == def.bwd_batch && !rv.persistent) out.default_bwd_ms = ms;
+        if (ms < out.bwd_ms) {
+          out.bwd_ms = ms;
+          out.bwd = rv;
+        }
+      }
+  } catch (...) {
+    ctx->timing = timing;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  ctx->timing = timing;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return out;
+}
+
+void fill_tune_result(const RasterTune& t, splat_tune_result* out) {
+  out->fwd_batch = t.fwd.fwd_batch;
+  out->fwd_persistent = t.fwd.persistent;
+  out->fwd_per_cu = (int32_t)t.fwd.per_cu;
+  out->bwd_batch = t.bwd.bwd_batch;
+  out->bwd_persistent = t.bwd.persistent;
+  out->bwd_per_cu = (int32_t)t.bwd.per_cu;
+  out->fwd_ms = t.fwd_ms;
+  out->bwd_ms = t.bwd_ms;
+  out->default_fwd_ms = t.default_fwd_ms;
+  out->default_bwd_ms = t.default_bwd_ms;
+  out->candidates = t.candidates;
+  out->pad_ = 0;
+}
+
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
                   splat_proj_grad* grads, float* abs_grads, const StepStatus* status) {
   launch(ctx, "reduce_slots", reduce_slots_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,

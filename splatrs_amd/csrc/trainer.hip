@@ -218,6 +218,13 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
     P.strong = t->strong.as<uint32_t>();
     P.strong_threshold = t->strong_threshold;
   }
+  P.fwd_batch = t->fwd_shape.fwd_batch;
+  P.fwd_persistent = t->fwd_shape.persistent;
+  P.fwd_per_cu = (int32_t)t->fwd_shape.per_cu;
+  P.bwd_batch = t->bwd_shape.bwd_batch;
+  P.bwd_persistent = t->bwd_shape.persistent;
+  P.bwd_per_cu = (int32_t)t->bwd_shape.per_cu;
+  P.counter = (uint32_t*)st["raster_counter"].reserve(16);
   P.pb = proj_buffers(t);
   BinBuffers& bb = P.bb;
   bb.tiles_x = tx;
@@ -266,12 +273,20 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   project_forward(ctx, P.params, n, P.stride, cam0, opts, P.pb, P.dyn);
   bin_build_device(ctx, P.pb, n, P.bb, P.entry_cap, P.small, P.total64, P.big_list, P.scan_sums,
                    P.status);
-  raster_forward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.image, P.final_t, P.n_contrib);
+  RasterVariant fv, bv;
+  fv.fwd_batch = P.fwd_batch;
+  fv.persistent = P.fwd_persistent != 0;
+  fv.per_cu = (unsigned)P.fwd_per_cu;
+  bv.bwd_batch = P.bwd_batch;
+  bv.persistent = P.bwd_persistent != 0;
+  bv.per_cu = (unsigned)P.bwd_per_cu;
+  raster_forward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.image, P.final_t, P.n_contrib, fv,
+                 nullptr, P.counter);
   loss_forward_backward(ctx, P.image, nullptr, any_target, P.width, P.height, P.lambda, true,
                         P.dimage, P.loss_scratch, nullptr, P.dyn);
   loss_value_to(ctx, P.loss_scratch, P.width, P.height, P.lambda, nullptr, P.dyn);
   raster_backward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.final_t, P.n_contrib, P.dimage,
-                  P.ordered != 0, P.slots, P.accumulate && P.absgrad);
+                  P.ordered != 0, P.slots, P.accumulate && P.absgrad, bv, nullptr, P.counter);
   StatsArgs sa{};
   if (P.accumulate) {
     sa.radius = P.pb.radius;
@@ -630,7 +645,9 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     float* image = (float*)t->store["image"].reserve(npx * 12 + 4);
     float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
     uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
-    raster_forward(ctx, pb, bb, W, H, p->background, image, final_t, n_contrib);
+    uint32_t* counter = (uint32_t*)t->store["raster_counter"].reserve(16);
+    raster_forward(ctx, pb, bb, W, H, p->background, image, final_t, n_contrib, t->fwd_shape,
+                   nullptr, counter);
 
     // loss (src/train.rs:221-222)
     float* dimage = (float*)t->store["dimage"].reserve(npx * 12 + 4);
@@ -643,7 +660,8 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     // backward (src/train.rs:225-235)
     float* slots = (float*)t->store["slots"].reserve((E + 1) * kSlots * 4);
     raster_backward(ctx, pb, bb, W, H, p->background, final_t, n_contrib, dimage,
-                    p->ordered_backward != 0, slots, p->accumulate_stats && p->absgrad);
+                    p->ordered_backward != 0, slots, p->accumulate_stats && p->absgrad,
+                    t->bwd_shape, nullptr, counter);
     // slot reduce, statistics (src/train.rs:228-235), project backward and the optimiser
     // step (src/train.rs:238-247, src/adam.rs:77-90) in one pass over the Gaussians
     StatsArgs sa{};
@@ -740,6 +758,58 @@ int splat_trainer_set_graph(splat_trainer* t, int32_t enabled) {
   return tguard(t, [&] {
     drain(t);
     t->use_graph = enabled != 0;
+  });
+}
+
+int splat_trainer_tune(splat_trainer* t, uint32_t view, const splat_step_params* p,
+                       splat_tune_result* out) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(p && out && view < t->views.size(), "bad argument");
+    SPLAT_REQUIRE(p->sh_degree >= 0 && p->sh_degree <= 3, "sh_degree must be 0..3");
+    drain(t);
+    splat_ctx* ctx = t->ctx;
+    const TrainerView& tv = *t->views[view];
+    const uint32_t W = tv.cam.width, H = tv.cam.height;
+    const size_t npx = (size_t)W * H;
+    // the forward half of a step (the model is not changed) fills the tuning buffers
+    ProjBuffers pb = proj_buffers(t);
+    project_forward(ctx, t->params.as<float>(), t->n, t->stride, tv.cam, step_opts(*p), pb);
+    BinBuffers bb{};
+    const size_t E = bin_build(ctx, pb, t->n, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile,
+                               bb, &t->entry_cap, t->store, nullptr);
+    float* image = (float*)t->store["image"].reserve(npx * 12 + 4);
+    float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
+    uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
+    float* dimage = (float*)t->store["dimage"].reserve(npx * 12 + 4);
+    float* scratch = (float*)t->store["loss_scratch"].reserve(loss_scratch_floats(W, H) * 4);
+    float* slots = (float*)t->store["slots"].reserve((std::max(E, t->entry_cap) + 1) * kSlots * 4);
+    uint32_t* counter = (uint32_t*)t->store["raster_counter"].reserve(16);
+    raster_forward(ctx, pb, bb, W, H, p->background, image, final_t, n_contrib);
+    loss_forward_backward(ctx, image, nullptr, tv.rgb.as<uint8_t>(), W, H, p->lambda_dssim, true,
+                          dimage, scratch, nullptr);
+    const RasterTune tr = tune_raster(ctx, pb, bb, W, H, p->background, image, final_t,
+                                      n_contrib, dimage, p->ordered_backward != 0, slots,
+                                      p->accumulate_stats && p->absgrad, nullptr, counter);
+    t->fwd_shape = tr.fwd;
+    t->bwd_shape = tr.bwd;
+    fill_tune_result(tr, out);
+  });
+}
+
+int splat_trainer_set_shapes(splat_trainer* t, const splat_tune_result* shapes) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(shapes, "null argument");
+    SPLAT_REQUIRE((shapes->fwd_batch == 32 || shapes->fwd_batch == 64 ||
+                   shapes->fwd_batch == 128) &&
+                      (shapes->bwd_batch == 16 || shapes->bwd_batch == 32),
+                  "unsupported batch shape");
+    drain(t);
+    t->fwd_shape.fwd_batch = shapes->fwd_batch;
+    t->fwd_shape.persistent = shapes->fwd_persistent != 0;
+    t->fwd_shape.per_cu = (unsigned)std::max(1, shapes->fwd_per_cu);
+    t->bwd_shape.bwd_batch = shapes->bwd_batch;
+    t->bwd_shape.persistent = shapes->bwd_persistent != 0;
+    t->bwd_shape.per_cu = (unsigned)std::max(1, shapes->bwd_per_cu);
   });
 }
 

@@ -29,6 +29,7 @@ struct StepPlan {
   uint32_t width, height;
   int32_t sh_degree, antialias, ordered, accumulate, absgrad, pad0;
   float near_plane, lambda, bg[3], strong_threshold;
+  int32_t fwd_batch, fwd_persistent, fwd_per_cu, bwd_batch, bwd_persistent, bwd_per_cu;
   float *params, *m, *v, *grads, *grad_accum;
   uint32_t *denom, *max_radii;
   float* max_grad;   // extra statistics for the learned scorer (null when not tracked)
@@ -36,7 +37,7 @@ struct StepPlan {
   ProjBuffers pb;
   BinBuffers bb;
   float *image, *final_t, *dimage, *loss_scratch, *slots, *absg, *loss_one;
-  uint32_t *n_contrib, *small, *big_list, *scan_sums;
+  uint32_t *n_contrib, *small, *big_list, *scan_sums, *counter;
   unsigned long long* total64;
   splat_proj_grad* pgrad;
   StepDyn* dyn;
@@ -73,6 +74,7 @@ struct splat_trainer {
   StepReport* reports_host = nullptr;  // pinned, mapped; written by the step's last kernel
   StepReport* reports_dev = nullptr;
   unsigned long long last_total = 0;
+  splat::RasterVariant fwd_shape, bwd_shape;  // compositing launch shapes (splat_trainer_tune)
   uint64_t graph_captures = 0, graph_replays = 0, step_aborts = 0;
   // learned densification (learn.hip): extra statistics and lineage planes, the attached
   // scorer and its sample store

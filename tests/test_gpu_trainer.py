@@ -193,3 +193,35 @@ def test_graph_steps_equal_eager_steps(ctx):
     ea = eager.render(0, 3)
     eb = graph.render(0, 3)
     assert np.array_equal(ea, eb)
+
+
+def test_tuned_and_persistent_shapes_give_the_same_model(ctx):
+    """Launch shapes change the schedule, not the arithmetic: a tuned trainer and one with
+    persistent compositing launches step exactly like the default one (fast backward)."""
+    g, cams, views = small_training_problem(ctx)
+    extent = scenes.scene_extent(cams)
+    trs = [abi.Trainer(ctx, g) for _ in range(3)]
+    for tr in trs:
+        for c, v in zip(cams, views):
+            tr.add_view(c, v)
+    p0 = rec.step_params(3, rec.learning_rates(1e-4), accumulate_stats=True)
+    r = trs[1].tune(0, p0)
+    say(f"trainer tune: fwd {int(r['fwd_batch'])}/{int(r['fwd_persistent'])}/"
+        f"{int(r['fwd_per_cu'])} {float(r['fwd_ms']):.4f} ms (default "
+        f"{float(r['default_fwd_ms']):.4f}), bwd {int(r['bwd_batch'])}/"
+        f"{int(r['bwd_persistent'])}/{int(r['bwd_per_cu'])} {float(r['bwd_ms']):.4f} ms "
+        f"(default {float(r['default_bwd_ms']):.4f})")
+    forced = r.copy()
+    forced["fwd_batch"], forced["fwd_persistent"], forced["fwd_per_cu"] = 32, 1, 4
+    forced["bwd_batch"], forced["bwd_persistent"], forced["bwd_per_cu"] = 16, 1, 2
+    trs[2].set_shapes(forced)
+    for it in range(1, 7):
+        lr = rec.learning_rates(rec.position_lr(it, extent))
+        p = rec.step_params(3, lr, accumulate_stats=True, absgrad=it > 3)
+        for tr in trs:
+            tr.step(it % len(cams), p, result=False)
+    out = [tr.download(moments=True, stats=True) for tr in trs]
+    for k in out[0]:
+        for o in out[1:]:
+            assert np.array_equal(np.asarray(out[0][k]).view(np.uint8),
+                                  np.asarray(o[k]).view(np.uint8)), k
