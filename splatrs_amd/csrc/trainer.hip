@@ -69,7 +69,6 @@ void alloc_model(splat_trainer* t, size_t cap) {
   t->params.reserve(bytes);
   t->m.reserve(bytes);
   t->v.reserve(bytes);
-  t->grads.reserve(bytes);
   t->grad_accum.reserve((cap + 1) * 4);
   t->denom.reserve((cap + 1) * 4);
   t->max_radii.reserve((cap + 1) * 4);
@@ -209,7 +208,6 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   P.params = t->params.as<float>();
   P.m = t->m.as<float>();
   P.v = t->v.as<float>();
-  P.grads = t->grads.as<float>();
   P.grad_accum = t->grad_accum.as<float>();
   P.denom = t->denom.as<uint32_t>();
   P.max_radii = t->max_radii.as<uint32_t>();
@@ -248,8 +246,6 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   P.dimage = (float*)st["dimage"].reserve(npx * 12 + 4);
   P.loss_scratch = (float*)st["loss_scratch"].reserve(loss_scratch_floats(W, H) * 4);
   P.slots = (float*)st["slots"].reserve((ec + 1) * kSlots * 4);
-  P.pgrad = (splat_proj_grad*)st["pgrad"].reserve((t->cap + 1) * sizeof(splat_proj_grad));
-  P.absg = (float*)st["absg"].reserve((t->cap + 1) * 8);
   P.loss_one = (float*)st["step_loss"].reserve(4);
   P.dyn = (StepDyn*)st["step_dyn"].reserve(sizeof(StepDyn));
   P.status = (StepStatus*)st["step_status"].reserve(sizeof(StepStatus));
@@ -518,7 +514,6 @@ void trainer_densify(splat_trainer* t, const splat_densify_config& cfg, float ex
   }
   t->cap = cap2;
   t->stride = stride2;
-  t->grads.reserve(stride2 * kParams * 4);
   t->grad_accum.reserve((cap2 + 1) * 4);
   t->denom.reserve((cap2 + 1) * 4);
   t->max_radii.reserve((cap2 + 1) * 4);
