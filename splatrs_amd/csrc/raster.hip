@@ -210,6 +210,9 @@ struct PixBack {
 // says the pixel is inside the entry's rectangle and before its last contributor.
 // Returns whether any lane of the wave contributes; v[] holds the slot terms (zero for
 // lanes that do not), and the pixel state advances only on contributing lanes.
+// EXACT: the reference's correctly rounded divisions (ordered mode, bitwise); otherwise
+// one hardware reciprocal (1 ulp) replaces both divisions by 1 - alpha.
+template <bool EXACT>
 __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, float4 r1, float4 r2,
                                            float px, float py, const uint64_t* tab,
                                            float v[kSlots]) {
@@ -225,7 +228,8 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
   const float og = o * G;
   const bool p2 = p1 & !(og < kAlphaMin);
   const float alpha = rmin(og, kAlphaMax);
-  const float t = s.t / (1.0f - alpha);
+  const float inv = EXACT ? 0.0f : __builtin_amdgcn_rcpf(1.0f - alpha);
+  const float t = EXACT ? s.t / (1.0f - alpha) : s.t * inv;
   const float a0 = s.last_alpha * s.lc0 + (1.0f - s.last_alpha) * s.acc0;
   const float a1 = s.last_alpha * s.lc1 + (1.0f - s.last_alpha) * s.acc1;
   const float a2 = s.last_alpha * s.lc2 + (1.0f - s.last_alpha) * s.acc2;
@@ -234,7 +238,7 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
   dalpha += (r2.z - a1) * s.d1;
   dalpha += (r2.w - a2) * s.d2;
   dalpha *= t;
-  dalpha += -s.t_final / (1.0f - alpha) * s.bg_dot;
+  dalpha += (EXACT ? -s.t_final / (1.0f - alpha) : -s.t_final * inv) * s.bg_dot;
   const bool p3 = p2 & (og < kAlphaMax);
   const float dgauss = o * dalpha;
   const float dsigma = -dgauss * G;
@@ -360,7 +364,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         const uint32_t k = lo + kk;
         float v[kSlots];
         const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
-        pair_terms(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v);
+        pair_terms<true>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v);
 #pragma unroll
         for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + threadIdx.x] = v[q];
       }
@@ -378,7 +382,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         const uint32_t k = lo + kk;
         float v[kSlots];
         const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
-        if (pair_terms(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v)) {
+        if (pair_terms<false>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v)) {
 #pragma unroll
           for (int q = 0; q < NS; q++) {
             const float r = row_sum16(v[q]);
