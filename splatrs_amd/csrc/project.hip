@@ -602,26 +602,22 @@ __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restric
   for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + i] = out[j];
 }
 
-// The per-Gaussian tail of a train step in one pass: the Gaussian's slots summed over its
-// entries in entry order (src/rasterize.rs:480-503), the densification statistics
-// (src/densify.rs:36-60, when accumulating), project_backward_one (src/project.rs:335-474)
-// and Adam on all 59 parameters (src/adam.rs:77-90).  The screen-space and parameter
-// gradients stay in registers; the arithmetic is that of the separate kernels.
-__global__ void __launch_bounds__(256) grad_adam_kernel(
-    float* __restrict__ params, float* __restrict__ m, float* __restrict__ v, uint32_t n,
-    uint32_t stride, CameraDev cam, float near_plane, float eps2d, int antialias, int degree,
+// The per-Gaussian part of a train step's backward in one pass: the Gaussian's slots
+// summed over its entries in entry order (src/rasterize.rs:480-503), the densification
+// statistics (src/densify.rs:36-60, when accumulating) and project_backward_one
+// (src/project.rs:335-474) into the gradient planes.  The screen-space gradient stays in
+// registers; the arithmetic is that of the separate kernels.  (Adam runs as its own
+// float4 stream: fused in here, the 59 parameters, their gradients and both moments
+// exceed the register file and the pass runs at one wave per SIMD.)
+__global__ void __launch_bounds__(256) grad_stats_kernel(
+    const float* __restrict__ params, float* __restrict__ grads, uint32_t n, uint32_t stride,
+    CameraDev cam, float near_plane, float eps2d, int antialias, int degree,
     const uint32_t* __restrict__ entry_offset, const float* __restrict__ slots,
-    StatsArgs sa, float c1, float c2, float b1, float b2, float eps, splat_learning_rates lr,
-    const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
+    StatsArgs sa, const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (status && status->abort) return;
-  if (dyn) {
-    cam = dyn->cam;
-    c1 = dyn->c1;
-    c2 = dyn->c2;
-    lr = dyn->lr;
-  }
+  if (dyn) cam = dyn->cam;
   float sum[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; k++) sum[k] = 0.0f;
@@ -664,14 +660,7 @@ __global__ void __launch_bounds__(256) grad_adam_kernel(
   float out[kParams];
   project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, &pg, out);
 #pragma unroll
-  for (int j = 0; j < kParams; j++) {
-    const size_t o = (size_t)j * stride + i;
-    float pj = g[j], mj = m[o], vj = v[o];
-    adam_update(pj, out[j], mj, vj, plane_lr(j, lr), b1, b2, c1, c2, eps);
-    params[o] = pj;
-    m[o] = mj;
-    v[o] = vj;
-  }
+  for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + i] = out[j];
 }
 
 // Batched: each scene Gaussian sums its per-view gradients over the scene's pairs in step
@@ -773,15 +762,13 @@ void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stri
          opts.antialias, opts.sh_degree, dproj, grads, dyn);
 }
 
-void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size_t stride,
-               const splat_camera& cam, const splat_proj_opts& opts,
-               const uint32_t* entry_offset, const float* slots, const StatsArgs& sa,
-               float c1, float c2, const splat_adam_state& st, const splat_learning_rates& lr,
-               const StepDyn* dyn, const StepStatus* status) {
-  launch(ctx, "grad_adam", grad_adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params, m,
-         v, (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
-         opts.antialias, opts.sh_degree, entry_offset, slots, sa, c1, c2, st.beta1, st.beta2,
-         st.eps, lr, dyn, status);
+void grad_stats(splat_ctx* ctx, const float* params, float* grads, size_t n, size_t stride,
+                const splat_camera& cam, const splat_proj_opts& opts,
+                const uint32_t* entry_offset, const float* slots, const StatsArgs& sa,
+                const StepDyn* dyn, const StepStatus* status) {
+  launch(ctx, "grad_stats", grad_stats_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params,
+         grads, (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
+         opts.antialias, opts.sh_degree, entry_offset, slots, sa, dyn, status);
 }
 
 void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,
