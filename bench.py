@@ -48,7 +48,7 @@ def algorithmic_bytes(name, N, V, E, P):
     return table.get(name)
 
 
-def learned_compare(iters, log=print):
+def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_thresholds=()):
     """Full training with densification on the ellipsoid scene (300k surface points, 100
     cameras 1297x840 f = 1100 on a radius-6 hemisphere, targets from a 2M-Gaussian ground
     truth, `iters` iterations, densify every 100 from 500 to 5000 with threshold 2e-4,
@@ -78,37 +78,53 @@ def learned_compare(iters, log=print):
     cfg = dict(iterations=iters, densify=1, densify_from_iter=500, densify_until_iter=until,
                densification_interval=100, opacity_reset_interval=3000,
                densify_grad_threshold=0.0002, sh_degree=3, sh_degree_interval=1000,
-               position_lr_max_steps=iters, log_interval=0, seed=0)
+               position_lr_max_steps=iters, log_interval=500, seed=0)
     scorer = abi.Scorer(ctx, seed=0)
 
-    def run(mode):
+    def run(mode, threshold=0.5, grad_threshold=0.0002):
         tr = abi.Trainer(ctx, init)
         for c, v in zip(cams, views):
             tr.add_view(c, v)
-        tr.attach_scorer(scorer, rec.learn_config(
-            mode=mode, samples_per_event=32768, max_events=64, outcome_delay=500,
-            eval_views=8, fit=rec.scorer_fit_config(epochs=2, batch=4096, lr=1e-3, seed=1)))
+        if mode:
+            tr.attach_scorer(scorer, rec.learn_config(
+                mode=mode, samples_per_event=32768, max_events=64, outcome_delay=500,
+                eval_views=8, threshold=threshold,
+                fit=rec.scorer_fit_config(epochs=2, batch=4096, lr=1e-3, seed=1)))
+        tot = {"cloned": 0, "split": 0, "pruned": 0}
+
+        def prog(pr):
+            tot["cloned"] += pr.cloned
+            tot["split"] += pr.split
+            tot["pruned"] += pr.pruned
         t = time.perf_counter()
-        tr.train(extent, abi.TrainConfig.default(**cfg))
+        tr.train(extent, abi.TrainConfig.default(**dict(cfg, densify_grad_threshold=grad_threshold)),
+                 on_progress=prog)
         tr.sync()
         sec = time.perf_counter() - t
         psnr = float(np.mean([abi.psnr(tr.render(i, 3), v.astype(np.float32) / np.float32(255))
                               for i, v in enumerate(views)]))
-        st = tr.learn_stats()
-        out = {"psnr_db": round(psnr, 3), "gaussians": int(tr.info()[0]),
+        st = tr.learn_stats() if mode else np.zeros((), rec.LEARN_STATS)
+        out = {"psnr_db": round(psnr, 3), "gaussians": int(tr.info()[0]), **tot,
                "it_per_s": round(iters / sec, 2), "seconds": round(sec, 2),
                "samples": int(st["resolved"]), "positives": int(st["positives"]),
                "fit_updates": int(st["updates"]), "fit_loss": round(float(st["last_loss"]), 4)}
         tr.close()
-        log(f"mode {mode}: {out}")
+        log(f"mode {mode} threshold {threshold} grad threshold {grad_threshold}: {out}")
         return out
-    plain = run(1)
-    learned = run(2)
+    plain = run(1)  # the reference's decisions; the scorer is trained on their outcomes
+    frontier = [dict(policy="learned", threshold=th, **run(2, th)) for th in learned_thresholds]
+    frontier += [dict(policy="plain", grad_threshold=gt, **run(0, grad_threshold=gt))
+                 for gt in plain_thresholds]
+    learned = frontier[0]
     ctx.close()
+    grown = lambda r: r["cloned"] + r["split"]
     return {"workload": "300k points on 20 ellipsoids -> densify, 100 cams 1297x840, "
                         f"{iters} iterations", "plain": plain, "learned": learned,
             "psnr_delta_db": round(learned["psnr_db"] - plain["psnr_db"], 3),
-            "gaussian_ratio": round(learned["gaussians"] / max(plain["gaussians"], 1), 4)}
+            "gaussian_ratio": round(learned["gaussians"] / max(plain["gaussians"], 1), 4),
+            "densified_ratio": round(grown(learned) / max(grown(plain), 1), 4),
+            "frontier": [{k: r[k] for k in ("policy", "threshold", "grad_threshold", "psnr_db",
+                                             "gaussians") if k in r} for r in frontier]}
 
 
 def batched_bench(steps, log=print):
@@ -214,6 +230,10 @@ def main():
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain-vs-learned densification comparison")
     ap.add_argument("--learned-iters", type=int, default=7000)
+    ap.add_argument("--learned-thresholds", default="0.2",
+                    help="scorer decision thresholds of the learned runs (comma separated)")
+    ap.add_argument("--plain-thresholds", default="",
+                    help="extra plain runs at these densify gradient thresholds")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched many-scene configuration")
     ap.add_argument("--batched-steps", type=int, default=10)
@@ -226,7 +246,9 @@ def main():
         return
     if args.learned_only:
         r = learned_compare(args.learned_iters,
-                            log=lambda m: print(m, file=sys.stderr, flush=True))
+                            log=lambda m: print(m, file=sys.stderr, flush=True),
+                            learned_thresholds=[float(x) for x in args.learned_thresholds.split(",")],
+                            plain_thresholds=[float(x) for x in args.plain_thresholds.split(",") if x])
         print(json.dumps(r), flush=True)
         return
     if args.calibrate_copy:
@@ -392,7 +414,10 @@ def main():
         ctx.close()
         err = lambda m: print(m, file=sys.stderr, flush=True)
         for key, fn in (("batched", lambda: batched_bench(args.batched_steps, err)),
-                        ("learned", lambda: learned_compare(args.learned_iters, err))):
+                        ("learned", lambda: learned_compare(
+                            args.learned_iters, err,
+                            [float(x) for x in args.learned_thresholds.split(",")],
+                            [float(x) for x in args.plain_thresholds.split(",") if x]))):
             try:
                 out[key] = fn()
             except Exception as e:  # the headline line is printed regardless

@@ -519,7 +519,8 @@ int splat_scorer_fit(splat_scorer* s, const float* features, const uint8_t* labe
  * mode 1 (learn): the reference's decisions are applied unchanged; a sample of them is
  *   logged at each densification event with the 12 features, labelled `outcome_delay`
  *   iterations later (a clone/split counts as positive while any descendant is alive with
- *   opacity >= useful_opacity),
+ *   an opacity of at least useful_ratio times its parent's at the decision and at least
+ *   useful_opacity),
  *   weighted by the PSNR change over that window, and the scorer is fit online on every
  *   labelled sample.
  * mode 2 (learned): the scorer's logits decide clone / split / prune at each event; the
@@ -538,8 +539,12 @@ typedef struct {
   float threshold;             /* mode 2: clone/split probability threshold */
   float prune_threshold;       /* mode 2: prune probability threshold */
   uint32_t eval_views;         /* views whose mean PSNR weights the samples */
-  float useful_opacity;        /* a descendant counts as useful at this opacity or above
-                                  (any survivor after an opacity reset in the window) */
+  float useful_opacity;        /* floor of the opacity a descendant needs to count as useful
+                                  (any survivor counts after an opacity reset in the
+                                  window) */
+  float useful_ratio;          /* ... and at least this fraction of its parent's opacity at
+                                  the decision */
+  uint32_t pad_;
   splat_scorer_fit_config fit; /* each online update */
 } splat_learn_config;
 

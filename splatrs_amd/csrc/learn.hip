@@ -9,8 +9,9 @@
 //   * mode 1 (learn): the reference's decisions are applied unchanged, and a sample of them
 //     is logged (features, decision bits, parent tag).  Once `outcome_delay` iterations have
 //     passed, each logged clone/split is labelled by whether any descendant of it is still
-//     alive with opacity >= useful_opacity (a densification whose offspring were pruned or
-//     faded is labelled negative);
+//     alive with an opacity of at least useful_ratio times its parent's at the decision (and
+//     useful_opacity): a densification whose offspring were pruned or faded is labelled
+//     negative;
 //     samples are weighted by the PSNR change over the same window, and the scorer is fit
 //     on every labelled sample so far;
 //   * mode 2 (learned): the scorer's logits decide clone / split / prune; the reference's
@@ -127,22 +128,29 @@ __global__ void __launch_bounds__(256) log_kernel(
   sample_of[i] = id;
 }
 
-// Useful descendants of the samples [b, b + c): every live row whose opacity logit is at
-// least min_logit walks up its lineage (sample ids decrease towards older decisions) and
-// credits the samples of this event it meets.
+// Useful descendants of the samples [b, b + c): every live row walks up its lineage
+// (sample ids decrease towards older decisions) and credits the samples of this event it
+// meets when its opacity is at least the larger of min_opacity and `ratio` times the
+// opacity the logged Gaussian had when it was densified (feature 6) -- a clone or child the
+// optimiser faded well below its parent did not earn its place.  any_survivor credits every live descendant
+// (used when an opacity reset fell inside the window).
 __global__ void __launch_bounds__(256) alive_kernel(const uint32_t* __restrict__ origin,
                                                     uint32_t n,
-                                                    const float* __restrict__ opacity,
-                                                    float min_logit,
+                                                    const float* __restrict__ opacity_logit,
+                                                    float min_opacity, float ratio,
+                                                    int any_survivor,
+                                                    const float* __restrict__ s_feats,
                                                     const uint32_t* __restrict__ s_parent,
                                                     uint32_t b, uint32_t c,
                                                     uint32_t* __restrict__ alive) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (opacity[i] < min_logit) return;
+  const float op = 1.0f / (1.0f + splat_expf(-opacity_logit[i]));
   uint32_t s = origin[i];
   for (int depth = 0; depth < 64 && s != kNoSample && s >= b; depth++) {
-    if (s < b + c) atomicAdd(&alive[s - b], 1u);
+    if (s < b + c &&
+        (any_survivor || op >= rmax(min_opacity, ratio * s_feats[(size_t)s * kMlpFeatures + 6])))
+      atomicAdd(&alive[s - b], 1u);
     s = s_parent[s];
   }
 }
@@ -299,13 +307,13 @@ void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float exte
     if (e.count) {
       uint32_t* alive = (uint32_t*)L.alive.reserve((size_t)e.count * 4 + 4);
       fill_u32(ctx, alive, e.count, 0u);
-      // a descendant counts while it is visible enough (opacity >= useful_opacity); after
-      // an opacity reset inside the window every survivor counts
-      const bool reset_inside = last_reset > e.iteration;
-      const float min_logit = reset_inside ? -INFINITY : logit(L.cfg.useful_opacity);
+      // a descendant counts while its opacity reached its parent's at the decision (and
+      // useful_opacity); after an opacity reset inside the window every survivor counts
+      const int reset_inside = last_reset > e.iteration ? 1 : 0;
       launch(ctx, "learn_alive", alive_kernel, dim3(blocks_for(t->n, 256)), dim3(256), 0,
              (const uint32_t*)t->origin.as<uint32_t>(), (uint32_t)t->n,
-             (const float*)(t->params.as<float>() + 10 * t->stride), min_logit,
+             (const float*)(t->params.as<float>() + 10 * t->stride), L.cfg.useful_opacity,
+             L.cfg.useful_ratio, reset_inside, (const float*)L.feats.as<float>(),
              (const uint32_t*)L.parent.as<uint32_t>(), e.base, e.count, alive);
       const float wpos = rclamp(1.0f + (psnr - e.psnr), 0.25f, 4.0f);
       uint32_t* cnt = (uint32_t*)L.counter.reserve(16);
@@ -344,6 +352,7 @@ void splat_learn_config_default(splat_learn_config* c) {
   c->prune_threshold = 0.5f;
   c->eval_views = 8;
   c->useful_opacity = 0.05f;
+  c->useful_ratio = 0.5f;
   splat_scorer_fit_config_default(&c->fit);
 }
 
@@ -359,7 +368,7 @@ int splat_trainer_attach_scorer(splat_trainer* t, splat_scorer* s, const splat_l
     SPLAT_REQUIRE(cfg && (cfg->mode == 1 || cfg->mode == 2), "mode must be 1 (learn) or 2 (learned)");
     SPLAT_REQUIRE(cfg->threshold > 0.0f && cfg->threshold < 1.0f && cfg->prune_threshold > 0.0f &&
                       cfg->prune_threshold < 1.0f && cfg->useful_opacity > 0.0f &&
-                      cfg->useful_opacity < 1.0f,
+                      cfg->useful_opacity < 1.0f && cfg->useful_ratio >= 0.0f,
                   "thresholds must lie in (0, 1)");
     SPLAT_REQUIRE(s->ctx == t->ctx, "scorer and trainer must share a context");
     auto L = std::make_unique<LearnState>();

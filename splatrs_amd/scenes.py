@@ -197,10 +197,10 @@ def sample_surface(rng, ell, n):
 
 def surface_color(ell, idx, pts):
     """Each ellipsoid's base colour with a smooth modulation and a fine 3D checker
-    (cells of ~0.08 units, a few pixels to tens of pixels on screen), so the surfaces carry
+    (cells of 1/32 unit, a few pixels on screen at distance 6), so the surfaces carry
     detail the initial point cloud cannot represent."""
     smooth = 0.12 * np.sin(4.0 * pts[:, 0] + 3.0 * pts[:, 1]) * np.cos(5.0 * pts[:, 2])
-    cells = np.floor(pts * 12.0).astype(np.int64)
+    cells = np.floor(pts * 32.0).astype(np.int64)
     checker = np.where((cells.sum(axis=1) & 1) == 0, 0.3, -0.3)
     rot = (idx % 3)[:, None] == np.arange(3)[None, :]   # the checker tints one channel
     col = ell["color"][idx] + smooth[:, None] + checker[:, None] * (0.5 + 0.5 * rot)
