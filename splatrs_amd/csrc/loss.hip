@@ -322,33 +322,54 @@ __global__ void blur_v_kernel(const float* __restrict__ tmp, uint32_t W, uint32_
 
 // loss = (1 - lambda) * (sum |x - y| * inv_n) + lambda * (1 - sum SSIM / n), from the
 // per-block partial sums (summed in binary64), written to *out without a host round trip.
-__global__ void loss_value_kernel(const float2* __restrict__ partial, uint32_t nb, float lambda,
-                                  float n, float* __restrict__ out,
-                                  const StepDyn* __restrict__ dyn) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (dyn) out = dyn->loss_out;
-  if (!out) return;
+// Sum of the per-block partials in binary64 by one 256-lane workgroup (lane-strided sums,
+// then a fixed tree), so the result does not depend on timing.
+__device__ __forceinline__ void sum_partials(const float2* __restrict__ partial, uint32_t nb,
+                                             double* a_out, double* b_out) {
+  __shared__ double sa[256], sb[256];
   double a = 0, b = 0;
-  for (uint32_t i = 0; i < nb; i++) {
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
     a += partial[i].x;
     b += partial[i].y;
   }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  *a_out = sa[0];
+  *b_out = sb[0];
+}
+
+__global__ void __launch_bounds__(256) loss_value_kernel(const float2* __restrict__ partial,
+                                                         uint32_t nb, float lambda, float n,
+                                                         float* __restrict__ out,
+                                                         const StepDyn* __restrict__ dyn) {
+  if (dyn) out = dyn->loss_out;
+  if (!out) return;
+  double a, b;
+  sum_partials(partial, nb, &a, &b);
+  if (threadIdx.x != 0) return;
   const float inv_n = 1.0f / n;
   const float l1 = (float)b * inv_n;
   const float ssim = (float)a / n;
   *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
 }
 
-// Per-pair loss values of a batched step (each pair's partials summed in block order).
-__global__ void loss_values_batch_kernel(const float2* __restrict__ partial, BatchDev bd,
-                                         float lambda, float* __restrict__ out) {
-  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= bd.npairs) return;
-  double a = 0, b = 0;
-  for (uint32_t i = bd.loss_block_base[v]; i < bd.loss_block_base[v + 1]; i++) {
-    a += partial[i].x;
-    b += partial[i].y;
-  }
+// Per-pair loss values of a batched step: one workgroup per pair.
+__global__ void __launch_bounds__(256) loss_values_batch_kernel(const float2* __restrict__ partial,
+                                                                BatchDev bd, float lambda,
+                                                                float* __restrict__ out) {
+  const uint32_t v = blockIdx.x;
+  double a, b;
+  sum_partials(partial + bd.loss_block_base[v], bd.loss_block_base[v + 1] - bd.loss_block_base[v],
+               &a, &b);
+  if (threadIdx.x != 0) return;
   const float n = (float)((size_t)bd.width[v] * bd.height[v] * 3);
   const float inv_n = 1.0f / n;
   const float l1 = (float)b * inv_n;
@@ -379,15 +400,15 @@ void loss_forward_backward_batch(splat_ctx* ctx, const float* pred, const BatchD
          pred, (const void*)nullptr, 0u, 0u, taps, (const float*)d_mu, (const float*)d_exx,
          (const float*)d_exy, 0.0f, 0.0f, lambda, dimage, (const StepDyn*)nullptr, bd);
   if (losses)
-    launch(ctx, "loss_value", loss_values_batch_kernel, dim3(blocks_for(bd.npairs, 64)),
-           dim3(64), 0, (const float2*)partial, bd, lambda, losses);
+    launch(ctx, "loss_value", loss_values_batch_kernel, dim3(bd.npairs), dim3(256), 0,
+           (const float2*)partial, bd, lambda, losses);
 }
 
 void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
                    float lambda, float* out_dev, const StepDyn* dyn) {
   const size_t n = (size_t)width * height * 3;
   const unsigned nb = blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
-  launch(ctx, "loss_value", loss_value_kernel, dim3(1), dim3(64), 0,
+  launch(ctx, "loss_value", loss_value_kernel, dim3(1), dim3(256), 0,
          reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, out_dev, dyn);
 }
 
