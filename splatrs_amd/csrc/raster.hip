@@ -268,13 +268,39 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
   return true;
 }
 
-// Sum over each 16-lane row of the wave with DPP row shifts (ALU-rate cross-lane moves);
-// lane 16 r + 15 ends with the sum of row r.
-__device__ __forceinline__ float row_sum16(float v) {
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x112, 0xf, 0xf, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x114, 0xf, 0xf, false));
-  v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x118, 0xf, 0xf, false));
+// The value of lane (l ^ m) within each 16-lane row, by a DPP control whose partner map is
+// that xor: row_ror:8 (m = 8), row_half_mirror (7), quad_perm [2,3,0,1] (2) and
+// [1,0,3,2] (1).  The compiler folds the move into the add that consumes it.
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+constexpr int kDppXor8 = 0x128, kDppXor7 = 0x141, kDppXor2 = 0x4e, kDppXor1 = 0xb1;
+
+// Reduce-scatter of 8 values over each 16-lane row: afterwards lanes 2m and 2m+1 of the row
+// hold the row's sum of v[m].  Each step pairs a lane with lane ^ {8, 7, 2, 1}; the two
+// partners keep complementary halves of their values (chosen by bit 3, 2, 1 of the lane)
+// and add the half the other sends.  The four masks span all 16 lanes, so every sum covers
+// the row; the order of the additions is fixed.
+__device__ __forceinline__ float row_scatter8(const float v[8], uint32_t lane) {
+  const bool b3 = (lane & 8u) != 0, b2 = (lane & 4u) != 0, b1 = (lane & 2u) != 0;
+  float a[4], b[2];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    a[q] = (b3 ? v[q + 4] : v[q]) + dpp_row<kDppXor8>(b3 ? v[q] : v[q + 4]);
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+    b[q] = (b2 ? a[q + 2] : a[q]) + dpp_row<kDppXor7>(b2 ? a[q] : a[q + 2]);
+  const float c = (b1 ? b[1] : b[0]) + dpp_row<kDppXor2>(b1 ? b[0] : b[1]);
+  return c + dpp_row<kDppXor1>(c);
+}
+
+// Sum over each 16-lane row by a rotate butterfly: every lane ends with its row's sum.
+__device__ __forceinline__ float row_allsum16(float v) {
+  v += dpp_row<0x121>(v);  // row_ror:1
+  v += dpp_row<0x122>(v);  // row_ror:2
+  v += dpp_row<0x124>(v);  // row_ror:4
+  v += dpp_row<0x128>(v);  // row_ror:8
   return v;
 }
 
@@ -371,9 +397,14 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     } else {
       // each wave walks, back to front, only the entries that reach its rows and precede
       // its last contributor (B <= 64: one ballot covers the batch)
-      constexpr int NS = ABS ? kSlots : 9;
-      const bool row_end = (lane & 15) == 15;
-      const int part = wave * 4 + (lane >> 4);
+      // Row sums: slots 0-7 by one reduce-scatter (even lane 2m holds slot m), slot 8 (and
+      // the AbsGS slots 9, 10) by a butterfly read from lane 15 (13, 11); one LDS store per
+      // writer lane.
+      const uint32_t l16 = lane & 15u;
+      const bool even = (l16 & 1u) == 0;
+      const bool writer = even | (l16 == 15) | (ABS & ((l16 == 13) | (l16 == 11)));
+      const uint32_t wslot = even ? (l16 >> 1) : (l16 == 15 ? 8u : (l16 == 13 ? 9u : 10u));
+      const uint32_t woff = wslot * 16 + wave * 4 + (lane >> 4);
       uint64_t todo = __ballot(lane < cnt && lo + lane < wave_last &&
                                wave_touches(srect[min(lane, cnt - 1)], wave));
       while (todo) {
@@ -383,11 +414,16 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         float v[kSlots];
         const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
         if (pair_terms<false>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v)) {
-#pragma unroll
-          for (int q = 0; q < NS; q++) {
-            const float r = row_sum16(v[q]);
-            if (row_end) red[(kk * kSlots + q) * 16 + part] = r;
+          float r = row_scatter8(v, lane);
+          const float e8 = row_allsum16(v[8]);
+          r = l16 == 15 ? e8 : r;
+          if (ABS) {
+            const float e9 = row_allsum16(v[9]);
+            const float e10 = row_allsum16(v[10]);
+            r = l16 == 13 ? e9 : r;
+            r = l16 == 11 ? e10 : r;
           }
+          if (writer) red[kk * kSlots * 16 + woff] = r;
         }
       }
     }
