@@ -10,7 +10,7 @@
 // written to the entry's slots (src/rasterize.rs:449-454); a second kernel sums each
 // Gaussian's slots in entry order (src/rasterize.rs:480-503).  ORDERED=true sums the
 // pixel terms sequentially in row-major order -- the reference's order, so slots and
-// gradients are bitwise identical to it; ORDERED=false uses wave butterflies.
+// gradients are bitwise identical to it; ORDERED=false uses DPP row reductions.
 #include "internal.hpp"
 
 namespace splat {
@@ -87,10 +87,20 @@ __device__ __forceinline__ uint32_t entry_rect(float4 r0, float4 r1, float4 r2, 
   return x0 | (x1 << 8) | (y0 << 16) | (y1 << 24);
 }
 
-// Whether a packed rectangle has pixels in the 4 tile rows of wave w (rows 4w .. 4w+3).
+// Pixel of thread t in the 16x16 tile: wave w = t / 64 covers the 8x8 block
+// (w % 2, w / 2) and lane l its pixel (l % 8, l / 8).  Square blocks let the rectangle
+// test below cull in both directions (10% fewer (entry, wave) pairs than 4x16 strips).
+__device__ __forceinline__ void tile_pixel(uint32_t t, uint32_t* i, uint32_t* j) {
+  const uint32_t w = t >> 6, l = t & 63u;
+  *i = (w & 1u) * 8u + (l & 7u);
+  *j = (w >> 1) * 8u + (l >> 3);
+}
+
+// Whether a packed rectangle has pixels in the 8x8 block of wave w.
 __device__ __forceinline__ bool wave_touches(uint32_t rc, uint32_t w) {
   const uint32_t x0 = rc & 0xffu, x1 = (rc >> 8) & 0xffu, y0 = (rc >> 16) & 0xffu, y1 = rc >> 24;
-  return (x0 < x1) & (y0 < 4 * w + 4) & (y1 > 4 * w);
+  const uint32_t bx = (w & 1u) * 8u, by = (w >> 1) * 8u;
+  return (x0 < x1) & (y0 < y1) & (x0 < bx + 8u) & (x1 > bx) & (y0 < by + 8u) & (y1 > by);
 }
 
 __device__ __forceinline__ bool in_rect(uint32_t rc, uint32_t i, uint32_t j) {
@@ -133,7 +143,8 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
   if (tile >= num_tiles) break;
   const TileGeom tg = tile_geom<BATCH>(tile, tiles_x, W, H, bd);
   const TileInfo ti = tg.ti;
-  const uint32_t i = threadIdx.x % kTile, j = threadIdx.x / kTile;
+  uint32_t i, j;
+  tile_pixel(threadIdx.x, &i, &j);
   const bool valid = i < ti.w && j < ti.h;
   const float px = (float)(ti.gx0 + i) + 0.5f;
   const float py = (float)(ti.gy0 + j) + 0.5f;
@@ -155,8 +166,8 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
     __syncthreads();
     // Predicated form of the per-pixel walk (src/rasterize.rs:262-290): every decision of
     // evaluate() and the T_MIN test is a lane predicate; only the exp is skipped when no
-    // lane of the wave needs it.  Each wave (4 pixel rows) walks, in order, only the
-    // entries whose pixel rectangle reaches its rows (a ballot over the batch).
+    // lane of the wave needs it.  Each wave (an 8x8 pixel block) walks, in order, only the
+    // entries whose pixel rectangle reaches its block (a ballot over the batch).
     for (uint32_t c0k = 0; c0k < cnt; c0k += 64) {
     const uint32_t kl = c0k + (threadIdx.x & 63);
     uint64_t todo = __ballot(kl < cnt && wave_touches(srect[min(kl, cnt - 1)], threadIdx.x >> 6));
@@ -209,10 +220,8 @@ struct PixBack {
 // Terms of one (entry, pixel) pair, src/rasterize.rs:404-445, in predicated form: `act`
 // says the pixel is inside the entry's rectangle and before its last contributor.
 // Returns whether any lane of the wave contributes; v[] holds the slot terms (zero for
-// lanes that do not), and the pixel state advances only on contributing lanes.
-// EXACT: the reference's correctly rounded divisions (ordered mode, bitwise); otherwise
-// one hardware reciprocal (1 ulp) replaces both divisions by 1 - alpha.
-template <bool EXACT>
+// lanes that do not), and the pixel state advances only on contributing lanes.  The
+// reference's expressions and correctly rounded divisions: the ordered backward is bitwise.
 __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, float4 r1, float4 r2,
                                            float px, float py, const uint64_t* tab,
                                            float v[kSlots]) {
@@ -228,8 +237,7 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
   const float og = o * G;
   const bool p2 = p1 & !(og < kAlphaMin);
   const float alpha = rmin(og, kAlphaMax);
-  const float inv = EXACT ? 0.0f : __builtin_amdgcn_rcpf(1.0f - alpha);
-  const float t = EXACT ? s.t / (1.0f - alpha) : s.t * inv;
+  const float t = s.t / (1.0f - alpha);
   const float a0 = s.last_alpha * s.lc0 + (1.0f - s.last_alpha) * s.acc0;
   const float a1 = s.last_alpha * s.lc1 + (1.0f - s.last_alpha) * s.acc1;
   const float a2 = s.last_alpha * s.lc2 + (1.0f - s.last_alpha) * s.acc2;
@@ -238,7 +246,7 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
   dalpha += (r2.z - a1) * s.d1;
   dalpha += (r2.w - a2) * s.d2;
   dalpha *= t;
-  dalpha += (EXACT ? -s.t_final / (1.0f - alpha) : -s.t_final * inv) * s.bg_dot;
+  dalpha += (-s.t_final / (1.0f - alpha)) * s.bg_dot;
   const bool p3 = p2 & (og < kAlphaMax);
   const float dgauss = o * dalpha;
   const float dsigma = -dgauss * G;
@@ -265,6 +273,65 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
     s.lc2 = r2.w;
     s.last_alpha = alpha;
   }
+  return true;
+}
+
+// pair_terms in the tree-reduced form, rearranged for fewer instructions.  Equal up to
+// rounding; the decisions (sigma range, alpha_min, alpha_max) are the exact ones, so the
+// walk visits the same pairs as the forward:
+//  * one hardware reciprocal (1 ulp) stands in for both divisions by 1 - alpha;
+//  * the accumulated colour behind the next contributor is updated eagerly,
+//    A <- A + alpha (c - A), the reference's last_alpha * last_color + (1 - last_alpha) * acc
+//    (src/rasterize.rs:420-428) evaluated one entry earlier;
+//  * with q = dL/dsigma the slots 0-4 become the per-pixel products q dx, q dy, q dx^2,
+//    q dx dy, q dy^2; the entry's conic turns the sums into dL/dmean and 0.5 scales the
+//    diagonal conic slots once per entry, when the batch's slots are written;
+//  * products with a zero factor stand in for the per-slot selects.
+// The pixel state uses acc0..2 for A; lc and last_alpha are unused.
+template <bool ABS>
+__device__ __forceinline__ bool pair_terms_fast(PixBack& s, bool act, float4 r0, float4 r1,
+                                                float4 r2, float px, float py,
+                                                const uint64_t* tab, float v[kSlots]) {
+  const float dx = px - r0.x;
+  const float dy = py - r0.y;
+  const float sigma = 0.5f * (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;
+  const bool p1 = act & !((sigma < 0.0f) | (sigma > r1.z));
+  if (!__any(p1)) return false;
+  const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
+  const float o = r1.y;
+  const float og = o * G;
+  const bool p2 = p1 & !(og < kAlphaMin);
+  const bool p3 = p2 & (og < kAlphaMax);
+  const float alpha = rmin(og, kAlphaMax);
+  const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
+  const float t = s.t * inv;
+  const float e0 = r2.y - s.acc0, e1 = r2.z - s.acc1, e2 = r2.w - s.acc2;
+  float dalpha = e0 * s.d0;
+  dalpha = fmaf(e1, s.d1, dalpha);
+  dalpha = fmaf(e2, s.d2, dalpha);
+  dalpha = fmaf(dalpha, t, -(s.t_final * inv) * s.bg_dot);
+  const float q = p3 ? -(o * dalpha) * G : 0.0f;
+  const float w = p2 ? alpha * t : 0.0f;
+  const float qdx = q * dx, qdy = q * dy;
+  v[0] = qdx;
+  v[1] = qdy;
+  v[2] = qdx * dx;
+  v[3] = qdx * dy;
+  v[4] = qdy * dy;
+  v[5] = p3 ? G * dalpha : 0.0f;
+  v[6] = w * s.d0;
+  v[7] = w * s.d1;
+  v[8] = w * s.d2;
+  if (ABS) {
+    v[9] = fabsf(fmaf(r0.z, qdx, r0.w * qdy));
+    v[10] = fabsf(fmaf(r0.w, qdx, r1.x * qdy));
+  } else {
+    v[9] = v[10] = 0.0f;
+  }
+  s.t = p2 ? t : s.t;
+  s.acc0 = p2 ? fmaf(alpha, e0, s.acc0) : s.acc0;
+  s.acc1 = p2 ? fmaf(alpha, e1, s.acc1) : s.acc1;
+  s.acc2 = p2 ? fmaf(alpha, e2, s.acc2) : s.acc2;
   return true;
 }
 
@@ -331,7 +398,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   if (threadIdx.x == 0) smax_last = 0;
   const TileGeom tg = tile_geom<BATCH>(tile, tiles_x, W, H, bd);
   const TileInfo ti = tg.ti;
-  const uint32_t i = threadIdx.x % kTile, j = threadIdx.x / kTile;
+  uint32_t i, j;
+  tile_pixel(threadIdx.x, &i, &j);
   const bool valid = i < ti.w && j < ti.h;
   const float px = (float)(ti.gx0 + i) + 0.5f;
   const float py = (float)(ti.gy0 + j) + 0.5f;
@@ -390,14 +458,14 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         const uint32_t k = lo + kk;
         float v[kSlots];
         const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
-        pair_terms<true>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v);
+        pair_terms(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v);
 #pragma unroll
-        for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + threadIdx.x] = v[q];
+        for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + j * kTile + i] = v[q];
       }
     } else {
-      // each wave walks, back to front, only the entries that reach its rows and precede
-      // its last contributor (B <= 64: one ballot covers the batch)
-      // Row sums: slots 0-7 by one reduce-scatter (even lane 2m holds slot m), slot 8 (and
+      // each wave walks, back to front, only the entries that reach its pixel block and
+      // precede its last contributor (B <= 64: one ballot covers the batch).
+      // Sums over each 16-lane row: slots 0-7 by one reduce-scatter (even lane 2m holds slot m), slot 8 (and
       // the AbsGS slots 9, 10) by a butterfly read from lane 15 (13, 11); one LDS store per
       // writer lane.
       const uint32_t l16 = lane & 15u;
@@ -413,7 +481,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         const uint32_t k = lo + kk;
         float v[kSlots];
         const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
-        if (pair_terms<false>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v)) {
+        if (pair_terms_fast<ABS>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v)) {
           float r = row_scatter8(v, lane);
           const float e8 = row_allsum16(v[8]);
           r = l16 == 15 ? e8 : r;
@@ -435,10 +503,23 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       if (ORDERED) {
         const float* p = &red[(kk * kSlots + slot) * 256];
         for (int t = 0; t < 256; t++) acc += p[t];
-      } else if (ABS || slot < 9) {
-        const float* p = &red[(kk * kSlots + slot) * 16];
+      } else {
+        // pair_terms_fast's products: slots 0, 1 combine both position sums with the conic
+        const auto part_sum = [&](uint32_t sl) {
+          const float* p = &red[(kk * kSlots + sl) * 16];
+          float a = 0.0f;
 #pragma unroll
-        for (int t = 0; t < 16; t++) acc += p[t];
+          for (int t = 0; t < 16; t++) a += p[t];
+          return a;
+        };
+        if (slot < 2) {
+          const float sx = part_sum(0), sy = part_sum(1);
+          const float4 a = s0[kk];
+          acc = slot == 0 ? -fmaf(a.z, sx, a.w * sy) : -fmaf(a.w, sx, s1[kk].x * sy);
+        } else if (ABS || slot < 9) {
+          acc = part_sum(slot);
+          if (slot == 2 || slot == 4) acc *= 0.5f;
+        }
       }
       slots[(size_t)sorted_entry[start + lo + kk] * kSlots + slot] = acc;
     }

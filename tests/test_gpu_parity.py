@@ -99,6 +99,11 @@ def full_pipeline(ctx, g, cam, opts, bg, ordered=True, check_bwd=True, seed=2):
         if ordered:
             same = np.mean(gg[f] == go[f])
             say(f"backward {f}: bitwise-equal fraction {same:.6f}")
+        else:
+            a, b = gg[f].astype(np.float64), go[f].astype(np.float64)
+            rel = np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-30)
+            say(f"backward {f}: median relative error {np.median(rel):.2e}, fraction beyond "
+                f"1e-4 before the floor {np.mean(rel > 1e-4):.2e}")
         grad_close(gg[f], go[f], f)
     grad_close(ag, ao, "abs")
     pbg = ctx.project_backward_all(g, cam, opts, go)
@@ -291,3 +296,9 @@ def test_full_size_1m_1080p(ctx):
     cam = scenes.hd_camera()
     opts = rec.proj_opts(3)
     full_pipeline(ctx, g, cam, opts, (0.0, 0.0, 0.0))
+
+
+def test_full_size_1m_1080p_tree_reduced_backward(ctx):
+    say("1M Gaussians, 1920x1080, tree-reduced backward (the benchmark's mode)")
+    full_pipeline(ctx, scenes.large_scene(0), scenes.hd_camera(), rec.proj_opts(3),
+                  (0.0, 0.0, 0.0), ordered=False)
