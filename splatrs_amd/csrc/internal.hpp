@@ -289,6 +289,18 @@ __device__ __forceinline__ float plane_lr(int j, const splat_learning_rates& lr)
   return lr.sh_rest;
 }
 
+// Streaming 16-byte loads and stores with the non-temporal hint: data touched once per
+// pass (the Adam planes) should not displace reusable lines from the caches.
+typedef float float4_vec __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_stream(const float* p) {
+  const float4_vec t = __builtin_nontemporal_load(reinterpret_cast<const float4_vec*>(p));
+  return make_float4(t.x, t.y, t.z, t.w);
+}
+__device__ __forceinline__ void store_stream(float* p, float4 x) {
+  const float4_vec t = {x.x, x.y, x.z, x.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<float4_vec*>(p));
+}
+
 // Adam's update of one scalar, src/adam.rs:44-61.
 __device__ __forceinline__ void adam_update(float& p, float g, float& m, float& v, float lr,
                                             float b1, float b2, float c1, float c2, float eps) {

@@ -31,17 +31,17 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ params,
   const float rate = plane_lr(j, lr);
   const size_t base = (size_t)j * stride + i0;
   if (i0 + 4 <= n) {
-    float4 P = *reinterpret_cast<float4*>(params + base);
-    const float4 G = *reinterpret_cast<const float4*>(grads + base);
-    float4 M = *reinterpret_cast<float4*>(m + base);
-    float4 V = *reinterpret_cast<float4*>(v + base);
+    float4 P = load_stream(params + base);
+    const float4 G = load_stream(grads + base);
+    float4 M = load_stream(m + base);
+    float4 V = load_stream(v + base);
     adam_update(P.x, G.x, M.x, V.x, rate, b1, b2, c1, c2, eps);
     adam_update(P.y, G.y, M.y, V.y, rate, b1, b2, c1, c2, eps);
     adam_update(P.z, G.z, M.z, V.z, rate, b1, b2, c1, c2, eps);
     adam_update(P.w, G.w, M.w, V.w, rate, b1, b2, c1, c2, eps);
-    *reinterpret_cast<float4*>(params + base) = P;
-    *reinterpret_cast<float4*>(m + base) = M;
-    *reinterpret_cast<float4*>(v + base) = V;
+    store_stream(params + base, P);
+    store_stream(m + base, M);
+    store_stream(v + base, V);
   } else {
     for (uint32_t k = 0; i0 + k < n; k++)
       adam_update(params[base + k], grads[base + k], m[base + k], v[base + k], rate, b1, b2, c1,

@@ -294,8 +294,10 @@ __device__ __forceinline__ void project_row(const float* __restrict__ params, si
   const int ncoef = (degree + 1) * (degree + 1);
 #pragma unroll
   for (int j = 0; j < kParams; j++) {
-    // SH coefficients beyond the active degree are never read by the forward pass.
-    g[j] = (j < 11 + 3 * ncoef) ? params[(size_t)j * stride + pi] : 0.0f;
+    // SH coefficients beyond the active degree are never read by the forward pass.  The
+    // parameter planes are streamed once per pass: non-temporal loads.
+    g[j] = (j < 11 + 3 * ncoef) ? __builtin_nontemporal_load(params + (size_t)j * stride + pi)
+                                : 0.0f;
   }
   Core k;
   bool ok = core(g, cam, near_plane, eps2d, antialias != 0, degree, k);
@@ -614,19 +616,33 @@ __global__ void __launch_bounds__(256) grad_stats_kernel(
     CameraDev cam, float near_plane, float eps2d, int antialias, int degree,
     const uint32_t* __restrict__ entry_offset, const float* __restrict__ slots,
     StatsArgs sa, const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
+  constexpr uint32_t kChunk = 1024;  // entries staged per pass (45 KB of LDS)
+  __shared__ float sl[kChunk * kSlots];
+  if (status && status->abort) return;  // uniform: every lane reads the same flag
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (status && status->abort) return;
   if (dyn) cam = dyn->cam;
   float sum[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; k++) sum[k] = 0.0f;
-  const uint32_t e0 = entry_offset[i], e1 = entry_offset[i + 1];
-  for (uint32_t e = e0; e < e1; e++) {
-    const float* q = slots + (size_t)e * kSlots;
+  // The workgroup's rows own one contiguous span of entries (entry_offset is a scan), so
+  // their slots are staged through LDS with coalesced loads; each lane then sums its own
+  // entries in entry order.
+  const uint32_t ib = blockIdx.x * blockDim.x, ie = min(ib + blockDim.x, n);
+  const uint32_t span0 = entry_offset[ib], span1 = entry_offset[ie];
+  const uint32_t e0 = i < n ? entry_offset[i] : 0u, e1 = i < n ? entry_offset[i + 1] : 0u;
+  for (uint32_t c0 = span0; c0 < span1; c0 += kChunk) {
+    const uint32_t c1 = min(c0 + kChunk, span1);
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < (c1 - c0) * kSlots; q += blockDim.x)
+      sl[q] = slots[(size_t)c0 * kSlots + q];
+    __syncthreads();
+    for (uint32_t e = max(e0, c0); e < min(e1, c1); e++) {
+      const float* q = sl + (e - c0) * kSlots;
 #pragma unroll
-    for (int k = 0; k < kSlots; k++) sum[k] += q[k];
+      for (int k = 0; k < kSlots; k++) sum[k] += q[k];
+    }
   }
+  if (i >= n) return;
   splat_proj_grad pg;
   pg.mean2d[0] = sum[0];
   pg.mean2d[1] = sum[1];
@@ -656,11 +672,11 @@ __global__ void __launch_bounds__(256) grad_stats_kernel(
   }
   float g[kParams];
 #pragma unroll
-  for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
+  for (int j = 0; j < kParams; j++) g[j] = __builtin_nontemporal_load(params + (size_t)j * stride + i);
   float out[kParams];
   project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, &pg, out);
 #pragma unroll
-  for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + i] = out[j];
+  for (int j = 0; j < kParams; j++) __builtin_nontemporal_store(out[j], grads + (size_t)j * stride + i);
 }
 
 // Batched: each scene Gaussian sums its per-view gradients over the scene's pairs in step
