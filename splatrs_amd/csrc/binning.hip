@@ -18,17 +18,46 @@ namespace {
 constexpr int kSmallCap = 1024;   // entries per tile sorted by the 256-lane kernel
 constexpr int kBigCap = 16384;    // entries per tile sorted in LDS by the 1024-lane kernel
 
-__global__ void __launch_bounds__(256) sum_u64_kernel(const uint32_t* __restrict__ in, size_t n,
-                                                      unsigned long long* total) {
-  __shared__ unsigned long long sh[4];
-  unsigned long long s = 0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    s += in[i];
-  for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+// Last g in [lo, hi) with off[g] <= e, given off[lo] <= e < off[hi] (Gaussians with zero
+// entries are skipped: they share their offset with the next Gaussian).
+template <class Off>
+__device__ __forceinline__ uint32_t search_offsets(Off off, uint32_t lo, uint32_t hi, uint32_t e) {
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off(mid) <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+constexpr uint32_t kKeygenSpan = 2048;  // Gaussian offsets staged per keygen workgroup
+
+// Gaussian of entry e for a workgroup of consecutive entries (call from every lane; e may
+// be past e_total).  Two lanes search entry_offset for the workgroup's first and last
+// entry; the offsets in between are staged in LDS and each lane searches there.  Spans of
+// more than kKeygenSpan Gaussians (long runs of culled ones) search global memory.
+__device__ __forceinline__ uint32_t entry_gaussian_of(uint32_t e, uint32_t e_total,
+                                                      const uint32_t* __restrict__ entry_offset,
+                                                      uint32_t n, uint32_t* s_off, uint32_t* s_g) {
+  const uint32_t eb = blockIdx.x * blockDim.x;
+  if (eb >= e_total) return 0u;  // uniform: no entry of this workgroup exists
+  const auto global_off = [&](uint32_t g) { return entry_offset[g]; };
+  if (threadIdx.x < 2) {
+    const uint32_t key = threadIdx.x == 0 ? eb : min(eb + blockDim.x, e_total) - 1;
+    s_g[threadIdx.x] = search_offsets(global_off, 0u, n, key);
+  }
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(total, sh[0] + sh[1] + sh[2] + sh[3]);
+  const uint32_t g0 = s_g[0], g1 = s_g[1];
+  const bool staged = g1 - g0 + 1 <= kKeygenSpan;
+  if (staged)
+    for (uint32_t q = threadIdx.x; q <= g1 - g0 + 1; q += blockDim.x) s_off[q] = entry_offset[g0 + q];
+  __syncthreads();
+  if (e >= e_total) return 0u;
+  if (staged)
+    return g0 + search_offsets([&](uint32_t q) { return s_off[q]; }, 0u, g1 - g0 + 1, e);
+  return search_offsets(global_off, g0, g1 + 1, e);
 }
 
 __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict__ entry_offset,
@@ -40,22 +69,15 @@ __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ entry_slot,
                                                      uint32_t* __restrict__ tile_count,
                                                      const StepStatus* __restrict__ status) {
+  __shared__ uint32_t s_off[kKeygenSpan + 1];
+  __shared__ uint32_t s_g[2];
   if (status) {
     if (status->abort) return;
     e_total = status->entries;
   }
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = entry_gaussian_of(e, e_total, entry_offset, n, s_off, s_g);
   if (e >= e_total) return;
-  // last g with entry_offset[g] <= e (Gaussians with zero entries are skipped)
-  uint32_t lo = 0, hi = n;  // entry_offset[lo] <= e < entry_offset[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (entry_offset[mid] <= e)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const uint32_t g = lo;
   const uint32_t local = e - entry_offset[g];
   const uint2 rc = rect[g];
   const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
@@ -77,19 +99,13 @@ __global__ void __launch_bounds__(256) keygen_batch_kernel(const uint32_t* __res
                                                            uint32_t* __restrict__ entry_slot,
                                                            uint32_t* __restrict__ tile_count,
                                                            const StepStatus* __restrict__ status) {
+  __shared__ uint32_t s_off[kKeygenSpan + 1];
+  __shared__ uint32_t s_g[2];
   if (status->abort) return;
   const uint32_t e_total = status->entries;
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = entry_gaussian_of(e, e_total, entry_offset, n, s_off, s_g);
   if (e >= e_total) return;
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (entry_offset[mid] <= e)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const uint32_t g = lo;
   const uint32_t v = find_segment(bd.work_base, bd.npairs, g);
   const uint32_t local = e - entry_offset[g];
   const uint2 rc = rect[g];
@@ -261,9 +277,7 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
 
   // 1. entry offsets; the 64-bit total guards the u32 offsets (src/rasterize.rs:74)
   fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
-  launch(ctx, "bin_total", sum_u64_kernel, dim3(std::min<unsigned>(blocks_for(n, 256), 1024u)),
-         dim3(256), 0, (const uint32_t*)pb.ntiles, n, total64);
-  exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small);
+  exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, nullptr, total64);
   unsigned long long e64 = 0;
   SPLAT_HIP(hipMemcpyAsync(&e64, total64, 8, hipMemcpyDeviceToHost, ctx->stream));
   SPLAT_HIP(hipStreamSynchronize(ctx->stream));
@@ -324,9 +338,7 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
                       const BatchDev* bd) {
   const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
   fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
-  launch(ctx, "bin_total", sum_u64_kernel, dim3(std::min<unsigned>(blocks_for(n, 256), 1024u)),
-         dim3(256), 0, (const uint32_t*)pb.ntiles, n, total64);
-  exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums);
+  exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64);
   bin_guard(ctx, total64, capacity, status);
   fill_u32(ctx, bb.tile_count, T, 0);
   fill_u32(ctx, small + 1, 1, 0);

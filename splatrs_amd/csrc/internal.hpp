@@ -383,9 +383,11 @@ void planes_to_aos(splat_ctx* ctx, const float* planes, size_t n, size_t stride,
 
 // scan.hip
 // `sums` (scan_temp_bytes(n) bytes) defaults to a context scratch buffer; graph-captured
-// code passes its own so the captured pointer stays valid.
+// code passes its own so the captured pointer stays valid.  `total64` (optional, zeroed by
+// the caller) receives the exact 64-bit sum of the input.
 void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
-                        uint32_t* total_dev, uint32_t* sums = nullptr);
+                        uint32_t* total_dev, uint32_t* sums = nullptr,
+                        unsigned long long* total64 = nullptr);
 size_t scan_temp_bytes(size_t n);
 // Fill kernel (used instead of hipMemset so the same code can sit inside a HIP graph).
 void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v);

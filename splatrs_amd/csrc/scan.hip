@@ -39,18 +39,76 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
   return v;
 }
 
+// Sum of one 64-bit value per thread over the workgroup, added to *total (one atomic per
+// workgroup; *total starts at zero).
+template <int THREADS>
+__device__ __forceinline__ void block_add_u64(unsigned long long v, unsigned long long* sh,
+                                              unsigned long long* total) {
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < THREADS / 64; w++) t += sh[w];
+    atomicAdd(total, t);
+  }
+}
+
+// total64 (optional): the exact 64-bit sum of the input, for callers that must detect a
+// total beyond the 32-bit offsets.
 __global__ void __launch_bounds__(kScanThreads) scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                    size_t n,
-                                                                   uint32_t* __restrict__ block_sums) {
+                                                                   uint32_t* __restrict__ block_sums,
+                                                                   unsigned long long* total64) {
   __shared__ uint32_t sh[kScanThreads / 64 + 1];
+  __shared__ unsigned long long sh64[kScanThreads / 64];
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
   uint32_t s = 0;
+  unsigned long long s64 = 0;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++)
-    if (base + k < n) s += in[base + k];
+    if (base + k < n) {
+      s += in[base + k];
+      s64 += in[base + k];
+    }
   uint32_t total;
   block_inclusive_scan<kScanThreads>(s, sh, &total);
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+  if (total64) block_add_u64<kScanThreads>(s64, sh64, total64);
+}
+
+// The whole scan in one workgroup (n <= kTopThreads * kTopItems): out[n] and *total_out
+// receive the total.
+__global__ void __launch_bounds__(kTopThreads) scan_single_kernel(const uint32_t* __restrict__ in,
+                                                                  uint32_t n,
+                                                                  uint32_t* __restrict__ out,
+                                                                  uint32_t* __restrict__ total_out,
+                                                                  unsigned long long* total64) {
+  __shared__ uint32_t sh[kTopThreads / 64 + 1];
+  __shared__ unsigned long long sh64[kTopThreads / 64];
+  uint32_t vals[kTopItems];
+  uint32_t s = 0;
+  unsigned long long s64 = 0;
+  const uint32_t base = threadIdx.x * kTopItems;
+#pragma unroll
+  for (int k = 0; k < kTopItems; k++) {
+    vals[k] = (base + k < n) ? in[base + k] : 0u;
+    s += vals[k];
+    s64 += vals[k];
+  }
+  uint32_t total;
+  const uint32_t incl = block_inclusive_scan<kTopThreads>(s, sh, &total);
+  uint32_t run = incl - s;
+#pragma unroll
+  for (int k = 0; k < kTopItems; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += vals[k];
+  }
+  if (threadIdx.x == 0) {
+    out[n] = total;
+    if (total_out) *total_out = total;
+  }
+  if (total64) block_add_u64<kTopThreads>(s64, sh64, total64);
 }
 
 __global__ void __launch_bounds__(kTopThreads) scan_top_kernel(uint32_t* __restrict__ sums, uint32_t nb,
@@ -116,11 +174,17 @@ size_t scan_temp_bytes(size_t n) {
 }
 
 void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
-                        uint32_t* total_dev, uint32_t* sums) {
-  const unsigned nb = blocks_for(n == 0 ? 1 : n, kScanTile);
+                        uint32_t* total_dev, uint32_t* sums, unsigned long long* total64) {
+  if (n <= (size_t)kTopThreads * kTopItems) {
+    launch(ctx, "scan_single", scan_single_kernel, dim3(1), dim3(kTopThreads), 0, in,
+           (uint32_t)n, out, total_dev, total64);
+    return;
+  }
+  const unsigned nb = blocks_for(n, kScanTile);
   SPLAT_REQUIRE(nb <= (unsigned)(kTopThreads * kTopItems), "scan: input too large");
   if (!sums) sums = ctx->buf<uint32_t>("scan_block_sums", nb + 1);
-  launch(ctx, "scan_reduce", scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, in, n, sums);
+  launch(ctx, "scan_reduce", scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, in, n, sums,
+         total64);
   launch(ctx, "scan_top", scan_top_kernel, dim3(1), dim3(kTopThreads), 0, sums, nb, total_dev);
   launch(ctx, "scan_apply", scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, in, n,
          (const uint32_t*)sums, out, (const uint32_t*)total_dev);
