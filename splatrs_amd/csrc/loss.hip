@@ -11,6 +11,8 @@
 // in the reference's order (horizontal: k = 0..10; vertical: out += kv * line, k =
 // 0..10); out-of-image taps add +0.0, which is exact, so every intermediate equals the
 // reference's value bit for bit.
+#include <utility>
+
 #include "internal.hpp"
 
 namespace splat {
@@ -31,35 +33,36 @@ __device__ __forceinline__ float load_target(const void* t, size_t idx) {
 }
 
 // Horizontal 11-tap blur at flat row position q (channel stride 3) from an LDS row
-// segment whose element 0 is row position q0 - kHalo.
-__device__ __forceinline__ float hblur(const float* seg, int q, int q0, int width3, const Taps& k) {
+// segment whose element 0 is row position q0 - kHalo.  The segment holds +0.0 outside the
+// image, which is what the reference's zero padding adds.
+__device__ __forceinline__ float hblur(const float* seg, int q, int q0, const Taps& k) {
   float acc = 0.0f;
 #pragma unroll
-  for (int t = 0; t < 11; t++) {
-    const int qs = q + (t - 5) * 3;
-    const float v = (qs >= 0 && qs < width3) ? seg[qs - q0 + kHalo] : 0.0f;
-    acc += k.k[t] * v;
-  }
+  for (int t = 0; t < 11; t++) acc += k.k[t] * seg[q + (t - 5) * 3 - q0 + kHalo];
   return acc;
 }
 
-template <int M>
-__device__ __forceinline__ void shift_in(float (&h)[M][11], const float (&nv)[M]) {
-#pragma unroll
-  for (int m = 0; m < M; m++) {
-#pragma unroll
-    for (int t = 0; t < 10; t++) h[m][t] = h[m][t + 1];
-    h[m][10] = nv[m];
-  }
+// f(integral_constant<int, 0>) ... f(integral_constant<int, N - 1>), in order.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
+// The vertical window is a ring of 11 horizontally blurred rows per map: the row loop is
+// unrolled 11 times (static_for) so that row step p writes slot p and the taps read the slots oldest
+// first, (p + 1 + t) % 11 for t = 0..10, as compile-time register names (no shifting).
 template <int M>
-__device__ __forceinline__ void vblur(const float (&h)[M][11], const Taps& k, float (&out)[M]) {
+__device__ __forceinline__ void vblur_ring(const float (&h)[M][11], int p, const Taps& k,
+                                           float (&out)[M]) {
 #pragma unroll
   for (int m = 0; m < M; m++) {
     float acc = 0.0f;
 #pragma unroll
-    for (int t = 0; t < 11; t++) acc += k.k[t] * h[m][t];
+    for (int t = 0; t < 11; t++) acc += k.k[t] * h[m][(p + 1 + t) % 11];
     out[m] = acc;
   }
 }
@@ -144,29 +147,56 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
 #pragma unroll
     for (int t = 0; t < 11; t++) h[m][t] = 0.0f;
   float ssum = 0.0f, l1sum = 0.0f;
+  // Software pipeline: the global loads of row rr + 1 (segment elements threadIdx.x and
+  // threadIdx.x + 256) and of output row rr - 4's x, y are issued before row rr is
+  // blurred, and land in LDS at the next step.
+  float fx[2], fy[2], ox = 0.0f, oy = 0.0f;
+  const auto fetch = [&](int r) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int t = threadIdx.x + k * 256;
+      const int qq = q0 - kHalo + t;
+      const bool in = r >= 0 && r < (int)H && t < kStrip + 2 * kHalo && qq >= 0 && qq < width3;
+      const size_t idx = (size_t)(in ? r : 0) * width3 + (in ? qq : 0);
+      fx[k] = in ? xin[idx] : 0.0f;
+      fy[k] = in ? load_target<U8>(yin, idx) : 0.0f;
+    }
+    const int orow = r - 5;
+    if (orow >= ybeg && orow < yend && col_ok) {
+      const size_t idx = (size_t)orow * width3 + q;
+      ox = xin[idx];
+      oy = load_target<U8>(yin, idx);
+    }
+  };
+  fetch(ybeg - 5);
   // Row rr enters the window; output row rr - 5 is then complete.
-  for (int rr = ybeg - 5; rr < yend + 5; rr++) {
+  for (int rr0 = ybeg - 5; rr0 < yend + 5; rr0 += 11) {
+  static_for<11>([&](auto step) {
+    constexpr int p = decltype(step)::value;
+    const int rr = rr0 + p;
+    if (rr >= yend + 5) return;  // uniform over the workgroup
     float nv[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     __syncthreads();
-    if (rr >= 0 && rr < (int)H) {
-      for (int t = threadIdx.x; t < kStrip + 2 * kHalo; t += blockDim.x) {
-        const int qq = q0 - kHalo + t;
-        const bool in = qq >= 0 && qq < width3;
-        const size_t idx = (size_t)rr * width3 + qq;
-        sx[t] = in ? xin[idx] : 0.0f;
-        sy[t] = in ? load_target<U8>(yin, idx) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int t = threadIdx.x + k * 256;
+      if (t < kStrip + 2 * kHalo) {
+        sx[t] = fx[k];
+        sy[t] = fy[k];
       }
     }
+    const float xo = ox, yo = oy;  // output row rr - 5
     __syncthreads();
+    fetch(rr + 1);
     if (rr >= 0 && rr < (int)H && col_ok) {
-      // x, y, x^2, y^2, xy blurred horizontally (products formed per element first)
+      // x, y, x^2, y^2, xy blurred horizontally (products formed per element first; the
+      // segment's +0.0 outside the image stands for the zero padding)
       float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
 #pragma unroll
       for (int t = 0; t < 11; t++) {
-        const int qs = q + (t - 5) * 3;
-        const bool in = qs >= 0 && qs < width3;
-        const float xv = in ? sx[qs - q0 + kHalo] : 0.0f;
-        const float yv = in ? sy[qs - q0 + kHalo] : 0.0f;
+        const int o = q + (t - 5) * 3 - q0 + kHalo;
+        const float xv = sx[o];
+        const float yv = sy[o];
         const float kk = taps.k[t];
         a0 += kk * xv;
         a1 += kk * yv;
@@ -180,11 +210,12 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
       nv[3] = a3;
       nv[4] = a4;
     }
-    shift_in<5>(h, nv);
+#pragma unroll
+    for (int m = 0; m < 5; m++) h[m][p] = nv[m];
     const int out_row = rr - 5;
     if (out_row >= ybeg && out_row < yend && col_ok) {
       float mu[5];
-      vblur<5>(h, taps, mu);
+      vblur_ring<5>(h, p, taps, mu);
       const float mx = mu[0], my = mu[1];
       const float sx2 = mu[2] - mx * mx;
       const float sy2 = mu[3] - my * my;
@@ -203,8 +234,9 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
       d_exx[idx] = -num * d1 / (den * den);
       d_exy[idx] = 2.0f * n1 / den;
       ssum += s;
-      l1sum += fabsf(xin[idx] - load_target<U8>(yin, idx));
+      l1sum += fabsf(xo - yo);
     }
+  });
   }
   const float bs = block_sum(ssum, red);
   const float bl = block_sum(l1sum, red);
@@ -250,33 +282,59 @@ __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict_
   for (int m = 0; m < 3; m++)
 #pragma unroll
     for (int t = 0; t < 11; t++) h[m][t] = 0.0f;
-  for (int rr = ybeg - 5; rr < yend + 5; rr++) {
+  // Software pipeline as in ssim_stats_kernel: row rr + 1 and output row rr - 4's x, y
+  // are fetched while row rr is blurred.
+  float f0[2], f1[2], f2[2], ox = 0.0f, oy = 0.0f;
+  const auto fetch = [&](int r) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int t = threadIdx.x + k * 256;
+      const int qq = q0 - kHalo + t;
+      const bool in = r >= 0 && r < (int)H && t < kStrip + 2 * kHalo && qq >= 0 && qq < width3;
+      const size_t idx = (size_t)(in ? r : 0) * width3 + (in ? qq : 0);
+      f0[k] = in ? d_mu[idx] : 0.0f;
+      f1[k] = in ? d_exx[idx] : 0.0f;
+      f2[k] = in ? d_exy[idx] : 0.0f;
+    }
+    const int orow = r - 5;
+    if (orow >= ybeg && orow < yend && col_ok) {
+      const size_t idx = (size_t)orow * width3 + q;
+      ox = xin[idx];
+      oy = load_target<U8>(yin, idx);
+    }
+  };
+  fetch(ybeg - 5);
+  for (int rr0 = ybeg - 5; rr0 < yend + 5; rr0 += 11) {
+  static_for<11>([&](auto step) {
+    constexpr int p = decltype(step)::value;
+    const int rr = rr0 + p;
+    if (rr >= yend + 5) return;  // uniform over the workgroup
     float nv[3] = {0.0f, 0.0f, 0.0f};
     __syncthreads();
-    if (rr >= 0 && rr < (int)H) {
-      for (int t = threadIdx.x; t < kStrip + 2 * kHalo; t += blockDim.x) {
-        const int qq = q0 - kHalo + t;
-        const bool in = qq >= 0 && qq < width3;
-        const size_t idx = (size_t)rr * width3 + qq;
-        s0[t] = in ? d_mu[idx] : 0.0f;
-        s1[t] = in ? d_exx[idx] : 0.0f;
-        s2[t] = in ? d_exy[idx] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int t = threadIdx.x + k * 256;
+      if (t < kStrip + 2 * kHalo) {
+        s0[t] = f0[k];
+        s1[t] = f1[k];
+        s2[t] = f2[k];
       }
     }
+    const float xv = ox, yv = oy;  // output row rr - 5
     __syncthreads();
+    fetch(rr + 1);
     if (rr >= 0 && rr < (int)H && col_ok) {
-      nv[0] = hblur(s0, q, q0, width3, taps);
-      nv[1] = hblur(s1, q, q0, width3, taps);
-      nv[2] = hblur(s2, q, q0, width3, taps);
+      nv[0] = hblur(s0, q, q0, taps);
+      nv[1] = hblur(s1, q, q0, taps);
+      nv[2] = hblur(s2, q, q0, taps);
     }
-    shift_in<3>(h, nv);
+#pragma unroll
+    for (int m = 0; m < 3; m++) h[m][p] = nv[m];
     const int out_row = rr - 5;
     if (out_row >= ybeg && out_row < yend && col_ok) {
       float b[3];
-      vblur<3>(h, taps, b);
+      vblur_ring<3>(h, p, taps, b);
       const size_t idx = (size_t)out_row * width3 + q;
-      const float xv = xin[idx];
-      const float yv = load_target<U8>(yin, idx);
       const float g = (b[0] + 2.0f * xv * b[1] + yv * b[2]) * inv_n;
       if (MODE == 1) {
         grad[idx] = g;
@@ -286,6 +344,7 @@ __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict_
         grad[idx] = w1 * sign - lambda * g;
       }
     }
+  });
   }
 }
 
