@@ -43,7 +43,8 @@ def algorithmic_bytes(name, N, V, E, P):
         "bin_scatter": 28 * E,
         "bin_sort_small": 24 * E,
         "stats": 24 * N,
-        "grad_stats": 44 * E + 472 * N,    # slots + params read; gradient planes written
+        # slots + params + both moments read; params and moments written (no gradient planes)
+        "grad_adam": 44 * E + 1416 * N,
     }
     return table.get(name)
 

@@ -365,12 +365,14 @@ struct StatsArgs {
   uint32_t* strong;
   float strong_threshold;
 };
-// Fused backward tail of a train step: slot reduce + statistics + project backward into
-// the gradient planes.
-void grad_stats(splat_ctx* ctx, const float* params, float* grads, size_t n, size_t stride,
-                const splat_camera& cam, const splat_proj_opts& opts,
-                const uint32_t* entry_offset, const float* slots, const StatsArgs& sa,
-                const StepDyn* dyn = nullptr, const StepStatus* status = nullptr);
+// Fused backward tail and optimizer of a train step: slot reduce + statistics + project
+// backward + Adam in one pass (no gradient planes); c1, c2, lr come from dyn when it is
+// given.
+void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size_t stride,
+               const splat_camera& cam, const splat_proj_opts& opts, const uint32_t* entry_offset,
+               const float* slots, const StatsArgs& sa, float c1, float c2,
+               const splat_adam_state& st, const splat_learning_rates& lr,
+               const StepDyn* dyn = nullptr, const StepStatus* status = nullptr);
 // Batched: projections of every (scene, view) pair (total `work` rows), and the summed
 // parameter gradients of every scene Gaussian (total `rows` plane rows).
 void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,

@@ -433,14 +433,14 @@ __global__ void proj_unpack_kernel(const splat_proj* __restrict__ in, uint32_t n
 }
 
 // project_backward_one, src/project.rs:335-474.
-// Gradient of one Gaussian (parameters g) for one view into out (zeroed here).
+// Gradient of one Gaussian (parameters g) for one view into out[j * ostride] (zeroed here).
 __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev& cam,
                                                 float near_plane, float eps2d, int antialias,
                                                 int degree,
                                                 const splat_proj_grad* __restrict__ dproj_row,
-                                                float* out) {
+                                                float* out, int ostride = 1) {
 #pragma unroll
-  for (int j = 0; j < kParams; j++) out[j] = 0.0f;
+  for (int j = 0; j < kParams; j++) out[(j) * ostride] = 0.0f;
   Core k;
   if (core(g, cam, near_plane, eps2d, antialias != 0, degree, k)) {
     const splat_proj_grad dp = *dproj_row;
@@ -455,7 +455,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
           if (kk < nco) {
-            out[11 + 3 * kk + ch] = dc * k.basis[kk];
+            out[(11 + 3 * kk + ch) * ostride] = dc * k.basis[kk];
             const float w = dc * g[11 + 3 * kk + ch];
             ddir.x += bgrad[kk].x * w;
             ddir.y += bgrad[kk].y * w;
@@ -471,7 +471,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
 
     const float do_raw = dp.opacity * k.comp;
     const float dcomp = dp.opacity * k.o_raw;
-    out[10] = do_raw * k.o_raw * (1.0f - k.o_raw);
+    out[(10) * ostride] = do_raw * k.o_raw * (1.0f - k.o_raw);
 
     const float a = k.a, b = k.b, c = k.c, det = k.det;
     const float da_ = dp.conic[0], db_ = dp.conic[1], dc_ = dp.conic[2];
@@ -532,9 +532,9 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
     dtz += dp.mean2d[1] * (-fy * k.t.y * rz2);
     const V3 dtv{dtx, dty, dtz};
     // rot^T * dt (rows of the transpose are the columns of rot)
-    out[0] = dot3(V3{cam.rot[0], cam.rot[3], cam.rot[6]}, dtv) + dmean_color.x;
-    out[1] = dot3(V3{cam.rot[1], cam.rot[4], cam.rot[7]}, dtv) + dmean_color.y;
-    out[2] = dot3(V3{cam.rot[2], cam.rot[5], cam.rot[8]}, dtv) + dmean_color.z;
+    out[(0) * ostride] = dot3(V3{cam.rot[0], cam.rot[3], cam.rot[6]}, dtv) + dmean_color.x;
+    out[(1) * ostride] = dot3(V3{cam.rot[1], cam.rot[4], cam.rot[7]}, dtv) + dmean_color.y;
+    out[(2) * ostride] = dot3(V3{cam.rot[2], cam.rot[5], cam.rot[8]}, dtv) + dmean_color.z;
 
     M3 dm;
     for (int ii = 0; ii < 3; ii++)
@@ -551,7 +551,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
         ds[col] += dm.m[r][col] * k.rq.m[r][col];
         dr.m[r][col] = dm.m[r][col] * sv[col];
       }
-    for (int ii = 0; ii < 3; ii++) out[3 + ii] = ds[ii] * sv[ii];
+    for (int ii = 0; ii < 3; ii++) out[(3 + ii) * ostride] = ds[ii] * sv[ii];
     // quat_to_mat_backward, src/math.rs:336-372.
     const float w = k.q[0], x = k.q[1], y = k.q[2], z = k.q[3];
     const float Dw[3][3] = {{0.0f, -2.0f * z, 2.0f * y},
@@ -580,7 +580,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
     if (nn > 0.0f) {
       const float u[4] = {raw[0] / nn, raw[1] / nn, raw[2] / nn, raw[3] / nn};
       const float dot = u[0] * dq[0] + u[1] * dq[1] + u[2] * dq[2] + u[3] * dq[3];
-      for (int t = 0; t < 4; t++) out[6 + t] = (dq[t] - u[t] * dot) / nn;
+      for (int t = 0; t < 4; t++) out[(6 + t) * ostride] = (dq[t] - u[t] * dot) / nn;
     }
   }
 }
@@ -604,45 +604,53 @@ __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restric
   for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + i] = out[j];
 }
 
-// The per-Gaussian part of a train step's backward in one pass: the Gaussian's slots
-// summed over its entries in entry order (src/rasterize.rs:480-503), the densification
-// statistics (src/densify.rs:36-60, when accumulating) and project_backward_one
-// (src/project.rs:335-474) into the gradient planes.  The screen-space gradient stays in
-// registers; the arithmetic is that of the separate kernels.  (Adam runs as its own
-// float4 stream: fused in here, the 59 parameters, their gradients and both moments
-// exceed the register file and the pass runs at one wave per SIMD.)
-__global__ void __launch_bounds__(256) grad_stats_kernel(
-    const float* __restrict__ params, float* __restrict__ grads, uint32_t n, uint32_t stride,
-    CameraDev cam, float near_plane, float eps2d, int antialias, int degree,
-    const uint32_t* __restrict__ entry_offset, const float* __restrict__ slots,
-    StatsArgs sa, const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
-  constexpr uint32_t kChunk = 1024;  // entries staged per pass (45 KB of LDS)
-  __shared__ float sl[kChunk * kSlots];
+// The per-Gaussian part of a train step's backward and the optimizer in one pass, one
+// lane per Gaussian row: the row's slots summed over its entries in entry order
+// (src/rasterize.rs:480-503; the workgroup's rows own one contiguous span of entries,
+// staged through LDS with coalesced loads), the densification statistics
+// (src/densify.rs:36-60, when accumulating) and project_backward_one (src/project.rs:
+// 335-474), whose 59 gradients go to LDS (the staging buffer, reused, [plane][256 rows])
+// instead of gradient planes in HBM.  After a barrier the workgroup runs Adam
+// (src/adam.rs:44-61) over its 59 x 256 parameter and moment values as float4 groups of 4
+// consecutive rows (plane-major, so each wave covers 1 KB of one plane), the gradients
+// read back from LDS.  The arithmetic is that of the separate host-tier kernels.
+__global__ void __launch_bounds__(256) grad_adam_kernel(
+    float* __restrict__ params, float* __restrict__ m, float* __restrict__ v, uint32_t n,
+    uint32_t stride, CameraDev cam, float near_plane, float eps2d, int antialias, int degree,
+    const uint32_t* __restrict__ entry_offset, const float* __restrict__ slots, StatsArgs sa,
+    float c1, float c2, float b1, float b2, float eps, splat_learning_rates lr,
+    const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
+  constexpr uint32_t kChunk = 1024;  // entries staged per pass
+  constexpr uint32_t kShared = kChunk * kSlots > kParams * 256 ? kChunk * kSlots : kParams * 256;
+  __shared__ __align__(16) float sh[kShared];
   if (status && status->abort) return;  // uniform: every lane reads the same flag
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (dyn) cam = dyn->cam;
+  if (dyn) {
+    cam = dyn->cam;
+    c1 = dyn->c1;
+    c2 = dyn->c2;
+    lr = dyn->lr;
+  }
   float sum[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; k++) sum[k] = 0.0f;
-  // The workgroup's rows own one contiguous span of entries (entry_offset is a scan), so
-  // their slots are staged through LDS with coalesced loads; each lane then sums its own
-  // entries in entry order.
   const uint32_t ib = blockIdx.x * blockDim.x, ie = min(ib + blockDim.x, n);
   const uint32_t span0 = entry_offset[ib], span1 = entry_offset[ie];
   const uint32_t e0 = i < n ? entry_offset[i] : 0u, e1 = i < n ? entry_offset[i + 1] : 0u;
   for (uint32_t c0 = span0; c0 < span1; c0 += kChunk) {
-    const uint32_t c1 = min(c0 + kChunk, span1);
+    const uint32_t c1e = min(c0 + kChunk, span1);
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (c1 - c0) * kSlots; q += blockDim.x)
-      sl[q] = slots[(size_t)c0 * kSlots + q];
+    for (uint32_t q = threadIdx.x; q < (c1e - c0) * kSlots; q += blockDim.x)
+      sh[q] = slots[(size_t)c0 * kSlots + q];
     __syncthreads();
-    for (uint32_t e = max(e0, c0); e < min(e1, c1); e++) {
-      const float* q = sl + (e - c0) * kSlots;
+    for (uint32_t e = max(e0, c0); e < min(e1, c1e); e++) {
+      const float* q = sh + (e - c0) * kSlots;
 #pragma unroll
       for (int k = 0; k < kSlots; k++) sum[k] += q[k];
     }
   }
-  if (i >= n) return;
+  __syncthreads();  // the staging buffer now holds gradients
+  if (i < n) {
   splat_proj_grad pg;
   pg.mean2d[0] = sum[0];
   pg.mean2d[1] = sum[1];
@@ -672,11 +680,36 @@ __global__ void __launch_bounds__(256) grad_stats_kernel(
   }
   float g[kParams];
 #pragma unroll
-  for (int j = 0; j < kParams; j++) g[j] = __builtin_nontemporal_load(params + (size_t)j * stride + i);
-  float out[kParams];
-  project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, &pg, out);
-#pragma unroll
-  for (int j = 0; j < kParams; j++) __builtin_nontemporal_store(out[j], grads + (size_t)j * stride + i);
+  for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
+  project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, &pg, sh + threadIdx.x, 256);
+  }
+  __syncthreads();  // every row's gradients are in LDS
+  const uint32_t rows = ie - ib;
+  constexpr uint32_t kGroups = kParams * 64;  // float4 groups of the workgroup
+#pragma unroll 2
+  for (uint32_t q = threadIdx.x; q < kGroups; q += blockDim.x) {
+    const uint32_t j = q >> 6, r = (q & 63u) * 4u;
+    if (r >= rows) continue;
+    const float rate = plane_lr((int)j, lr);
+    const size_t at = (size_t)j * stride + ib + r;
+    const float* gq = sh + j * 256 + r;
+    if (r + 4 <= rows) {
+      float4 P = load_stream(params + at);
+      float4 M = load_stream(m + at);
+      float4 V = load_stream(v + at);
+      const float4 G = *reinterpret_cast<const float4*>(gq);
+      adam_update(P.x, G.x, M.x, V.x, rate, b1, b2, c1, c2, eps);
+      adam_update(P.y, G.y, M.y, V.y, rate, b1, b2, c1, c2, eps);
+      adam_update(P.z, G.z, M.z, V.z, rate, b1, b2, c1, c2, eps);
+      adam_update(P.w, G.w, M.w, V.w, rate, b1, b2, c1, c2, eps);
+      store_stream(params + at, P);
+      store_stream(m + at, M);
+      store_stream(v + at, V);
+    } else {
+      for (uint32_t k = 0; r + k < rows; k++)
+        adam_update(params[at + k], gq[k], m[at + k], v[at + k], rate, b1, b2, c1, c2, eps);
+    }
+  }
 }
 
 // Batched: each scene Gaussian sums its per-view gradients over the scene's pairs in step
@@ -778,13 +811,15 @@ void project_backward(splat_ctx* ctx, const float* params, size_t n, size_t stri
          opts.antialias, opts.sh_degree, dproj, grads, dyn);
 }
 
-void grad_stats(splat_ctx* ctx, const float* params, float* grads, size_t n, size_t stride,
-                const splat_camera& cam, const splat_proj_opts& opts,
-                const uint32_t* entry_offset, const float* slots, const StatsArgs& sa,
-                const StepDyn* dyn, const StepStatus* status) {
-  launch(ctx, "grad_stats", grad_stats_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params,
-         grads, (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
-         opts.antialias, opts.sh_degree, entry_offset, slots, sa, dyn, status);
+void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size_t stride,
+               const splat_camera& cam, const splat_proj_opts& opts, const uint32_t* entry_offset,
+               const float* slots, const StatsArgs& sa, float c1, float c2,
+               const splat_adam_state& st, const splat_learning_rates& lr, const StepDyn* dyn,
+               const StepStatus* status) {
+  launch(ctx, "grad_adam", grad_adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params, m,
+         v, (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
+         opts.antialias, opts.sh_degree, entry_offset, slots, sa, c1, c2, st.beta1, st.beta2,
+         st.eps, lr, dyn, status);
 }
 
 void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,

@@ -69,7 +69,6 @@ void alloc_model(splat_trainer* t, size_t cap) {
   t->params.reserve(bytes);
   t->m.reserve(bytes);
   t->v.reserve(bytes);
-  t->grads.reserve(bytes);
   t->grad_accum.reserve((cap + 1) * 4);
   t->denom.reserve((cap + 1) * 4);
   t->max_radii.reserve((cap + 1) * 4);
@@ -209,7 +208,6 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   P.params = t->params.as<float>();
   P.m = t->m.as<float>();
   P.v = t->v.as<float>();
-  P.grads = t->grads.as<float>();
   P.grad_accum = t->grad_accum.as<float>();
   P.denom = t->denom.as<uint32_t>();
   P.max_radii = t->max_radii.as<uint32_t>();
@@ -301,10 +299,8 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   splat_adam_state ast{0.9f, 0.999f, 1e-15f, 0, 0};
   splat_learning_rates lr0;
   std::memset(&lr0, 0, sizeof(lr0));
-  grad_stats(ctx, P.params, P.grads, n, P.stride, cam0, opts, P.bb.entry_offset, P.slots, sa,
-             P.dyn, P.status);
-  adam_planes(ctx, P.params, P.grads, P.m, P.v, n, P.stride, 1.0f, 1.0f, ast, lr0, P.dyn,
-              P.status);
+  grad_adam(ctx, P.params, P.m, P.v, n, P.stride, cam0, opts, P.bb.entry_offset, P.slots, sa,
+            1.0f, 1.0f, ast, lr0, P.dyn, P.status);
   launch(ctx, "step_report", step_report_kernel, dim3(1), dim3(64), 0, (const StepDyn*)P.dyn,
          (const StepStatus*)P.status, P.reports);
 }
@@ -518,7 +514,6 @@ void trainer_densify(splat_trainer* t, const splat_densify_config& cfg, float ex
   }
   t->cap = cap2;
   t->stride = stride2;
-  t->grads.reserve(stride2 * kParams * 4);
   t->grad_accum.reserve((cap2 + 1) * 4);
   t->denom.reserve((cap2 + 1) * 4);
   t->max_radii.reserve((cap2 + 1) * 4);
@@ -687,10 +682,8 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     st.step = t->adam_step;
     const float c1 = 1.0f - powi(st.beta1, (int32_t)st.step);
     const float c2 = 1.0f - powi(st.beta2, (int32_t)st.step);
-    grad_stats(ctx, t->params.as<float>(), t->grads.as<float>(), n, t->stride, tv.cam, opts,
-               bb.entry_offset, slots, sa);
-    adam_planes(ctx, t->params.as<float>(), t->grads.as<float>(), t->m.as<float>(),
-                t->v.as<float>(), n, t->stride, c1, c2, st, p->lr);
+    grad_adam(ctx, t->params.as<float>(), t->m.as<float>(), t->v.as<float>(), n, t->stride,
+              tv.cam, opts, bb.entry_offset, slots, sa, c1, c2, st, p->lr);
 
     if (r) {
       r->num_entries = E;

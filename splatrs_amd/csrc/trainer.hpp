@@ -30,7 +30,7 @@ struct StepPlan {
   int32_t sh_degree, antialias, ordered, accumulate, absgrad, pad0;
   float near_plane, lambda, bg[3], strong_threshold;
   int32_t fwd_batch, fwd_persistent, fwd_per_cu, bwd_batch, bwd_persistent, bwd_per_cu;
-  float *params, *m, *v, *grads, *grad_accum;
+  float *params, *m, *v, *grad_accum;
   uint32_t *denom, *max_radii;
   float* max_grad;   // extra statistics for the learned scorer (null when not tracked)
   uint32_t* strong;
@@ -58,7 +58,7 @@ struct splat_trainer {
   int device = 0;
   size_t n = 0, cap = 0, stride = 0;
   uint64_t adam_step = 0;
-  DevBuf params, m, v, grads;           // planes [59][stride]
+  DevBuf params, m, v;                  // planes [59][stride]
   DevBuf grad_accum, denom, max_radii;  // statistics, capacity cap
   std::vector<std::unique_ptr<TrainerView>> views;
   std::map<std::string, DevBuf> store;  // per-view buffers (projection, binning, images)
