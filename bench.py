@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "train iters/s @1M Gaussians 1080p (plain vs batched+learned) at PSNR parity"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_GINST = 256 * 2.4  # CUs x GHz: wave64 VALU instructions per second (G)
 
 
 def algorithmic_bytes(name, N, V, E, P):
@@ -354,15 +355,25 @@ def main():
     # HBM bytes per launch measured by the rocprofv3 FETCH_SIZE / WRITE_SIZE passes on this
     # workload (profiles/), when they cover this kernel.
     traffic = None
+    valu = None
     tpath = os.path.join(ROOT, "profiles", "hbm_traffic_1M_1080p.json")
     if args.gaussians == 1_000_000 and os.path.exists(tpath):
         t = json.load(open(tpath))["kernels"].get(roof_name)
         if t:
             traffic = int(t["read_bytes"] + t["write_bytes"])
+            if "valu_insts" in t:
+                # The compositing kernels are bound by instruction issue, not by HBM: wave64
+                # VALU instructions per launch (rocprofv3 SQ_INSTS_VALU on this workload) per
+                # second of this run, against one instruction per CU per clock (256 CUs at
+                # 2.4 GHz, the non-packed FP32 rate of the 157.3 TFLOP/s peak).
+                rate = t["valu_insts"] / mean_s / 1e9
+                valu = {"insts_per_launch": int(t["valu_insts"]), "achieved": round(rate, 1),
+                        "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
+                        "frac": round(rate / VALU_PEAK_GINST, 3)}
     roofline = {"kernel": roof_name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": int(b),
-                "mean_ms": kernels[roof_name]["mean_ms"]}
+                "mean_ms": kernels[roof_name]["mean_ms"], "valu_issue": valu}
 
     cpu = None
     if args.cpu_steps > 0:
