@@ -215,7 +215,9 @@ BatchBufs reserve_bufs(splat_batch* b) {
   x.bb.tiles_x = (uint32_t)T;
   x.bb.tiles_y = 1;
   x.bb.entry_offset = (uint32_t*)st["entry_offset"].reserve(w * 4);
-  x.bb.tile_count = (uint32_t*)st["tile_count"].reserve(T * 4 + 4);
+  x.bb.tile_count = (uint32_t*)st["tile_count"].reserve(kTileSub * T * 4 + 4);
+  x.bb.tile_sub = (uint32_t*)st["tile_sub"].reserve((kTileSub + 1) * T * 4 + 4);
+  x.bb.block_marks = (uint32_t*)st["block_marks"].reserve((b->entry_cap / 256 + 2) * 4);
   x.bb.tile_start = (uint32_t*)st["tile_start"].reserve(T * 4 + 8);
   x.bb.entry_gaussian = (uint32_t*)st["entry_gaussian"].reserve(ec * 4);
   x.bb.entry_tile = (uint32_t*)st["entry_tile"].reserve(ec * 4);

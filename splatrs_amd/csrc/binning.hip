@@ -4,8 +4,11 @@
 // index breaks ties), so any correct sort yields its order.  Here:
 //   1. entry_offset = exclusive scan of each Gaussian's tile count (projection wrote it);
 //   2. keygen (one lane per entry): Gaussian by binary search in entry_offset, tile from
-//      the rectangle, and an atomic per-tile counter that gives the entry a slot;
-//   3. tile_start = exclusive scan of the per-tile counts;
+//      the rectangle, and an atomic counter that gives the entry a slot.  Each tile has
+//      kSub counters, one per workgroup index mod kSub (workgroups are dealt to the XCDs
+//      in turn, so a counter set stays in one XCD's cache lines), laid out [kSub][T];
+//   3. per tile, the counters' offsets within the tile and its total; tile_start =
+//      exclusive scan of the totals;
 //   4. scatter (depth bits << 32 | entry) into its tile's bucket;
 //   5. sort each bucket by that 64-bit value: bitonic in LDS (tiles up to 1024 entries in
 //      256-lane workgroups; up to 16384 in 1024-lane workgroups with 128 KB of LDS) and
@@ -16,6 +19,14 @@ namespace splat {
 namespace {
 
 constexpr int kSmallCap = 1024;   // entries per tile sorted by the 256-lane kernel
+constexpr uint32_t kKeygenThreads = 256;
+constexpr uint32_t kKeygenPerLane = 1;
+constexpr uint32_t kKeygenEntries = kKeygenThreads * kKeygenPerLane;  // per keygen workgroup
+// (More entries per lane do not help: the step's 2.1M returning counter atomics bound the
+// pass, not the per-workgroup offset lookup.)
+
+// Counter set of the keygen workgroup that handles entry e.
+__device__ __forceinline__ uint32_t entry_sub(uint32_t e) { return (e / kKeygenEntries) % kTileSub; }
 constexpr int kBigCap = 16384;    // entries per tile sorted in LDS by the 1024-lane kernel
 
 // Last g in [lo, hi) with off[g] <= e, given off[lo] <= e < off[hi] (Gaussians with zero
@@ -34,30 +45,51 @@ __device__ __forceinline__ uint32_t search_offsets(Off off, uint32_t lo, uint32_
 
 constexpr uint32_t kKeygenSpan = 2048;  // Gaussian offsets staged per keygen workgroup
 
-// Gaussian of entry e for a workgroup of consecutive entries (call from every lane; e may
-// be past e_total).  Two lanes search entry_offset for the workgroup's first and last
-// entry; the offsets in between are staged in LDS and each lane searches there.  Spans of
-// more than kKeygenSpan Gaussians (long runs of culled ones) search global memory.
-__device__ __forceinline__ uint32_t entry_gaussian_of(uint32_t e, uint32_t e_total,
-                                                      const uint32_t* __restrict__ entry_offset,
-                                                      uint32_t n, uint32_t* s_off, uint32_t* s_g) {
-  const uint32_t eb = blockIdx.x * blockDim.x;
-  if (eb >= e_total) return 0u;  // uniform: no entry of this workgroup exists
-  const auto global_off = [&](uint32_t g) { return entry_offset[g]; };
-  if (threadIdx.x < 2) {
-    const uint32_t key = threadIdx.x == 0 ? eb : min(eb + blockDim.x, e_total) - 1;
-    s_g[threadIdx.x] = search_offsets(global_off, 0u, n, key);
-  }
+// marks[b] = the Gaussian that holds entry b * kKeygenEntries (the first entry of keygen
+// workgroup b), written by that Gaussian.
+__global__ void __launch_bounds__(256) block_marks_kernel(const uint32_t* __restrict__ entry_offset,
+                                                          uint32_t n, uint32_t* __restrict__ marks,
+                                                          const StepStatus* __restrict__ status) {
+  if (status && status->abort) return;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const uint32_t lo = entry_offset[g], hi = entry_offset[g + 1];
+  for (uint32_t b = (lo + kKeygenEntries - 1) / kKeygenEntries; b * kKeygenEntries < hi; b++)
+    marks[b] = g;
+}
+
+// The Gaussians of one keygen workgroup's entries [b * kKeygenEntries, ...): they lie in
+// [g0, ghi) with entry_offset[g0] <= every entry < entry_offset[ghi].  Their offsets are
+// staged in LDS (call from every lane) unless the span exceeds kKeygenSpan Gaussians (long
+// runs of culled ones); then lookups search global memory.
+struct KeygenSpan {
+  uint32_t g0, ghi;
+  bool staged;
+};
+
+__device__ __forceinline__ KeygenSpan keygen_span(uint32_t e_total,
+                                                  const uint32_t* __restrict__ entry_offset,
+                                                  uint32_t n, const uint32_t* __restrict__ marks,
+                                                  uint32_t* s_off) {
+  const uint32_t b = blockIdx.x;
+  const uint32_t nb = (e_total + kKeygenEntries - 1) / kKeygenEntries;
+  KeygenSpan sp;
+  sp.g0 = marks[b];
+  sp.ghi = b + 1 < nb ? marks[b + 1] + 1 : n;
+  sp.staged = sp.ghi - sp.g0 <= kKeygenSpan;
+  if (sp.staged)
+    for (uint32_t q = threadIdx.x; q <= sp.ghi - sp.g0; q += blockDim.x)
+      s_off[q] = entry_offset[sp.g0 + q];
   __syncthreads();
-  const uint32_t g0 = s_g[0], g1 = s_g[1];
-  const bool staged = g1 - g0 + 1 <= kKeygenSpan;
-  if (staged)
-    for (uint32_t q = threadIdx.x; q <= g1 - g0 + 1; q += blockDim.x) s_off[q] = entry_offset[g0 + q];
-  __syncthreads();
-  if (e >= e_total) return 0u;
-  if (staged)
-    return g0 + search_offsets([&](uint32_t q) { return s_off[q]; }, 0u, g1 - g0 + 1, e);
-  return search_offsets(global_off, g0, g1 + 1, e);
+  return sp;
+}
+
+__device__ __forceinline__ uint32_t keygen_find(const KeygenSpan& sp, uint32_t e,
+                                                const uint32_t* __restrict__ entry_offset,
+                                                const uint32_t* s_off) {
+  if (sp.staged)
+    return sp.g0 + search_offsets([&](uint32_t q) { return s_off[q]; }, 0u, sp.ghi - sp.g0, e);
+  return search_offsets([&](uint32_t g) { return entry_offset[g]; }, sp.g0, sp.ghi, e);
 }
 
 __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict__ entry_offset,
@@ -68,25 +100,32 @@ __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ entry_tile,
                                                      uint32_t* __restrict__ entry_slot,
                                                      uint32_t* __restrict__ tile_count,
+                                                     uint32_t num_tiles,
+                                                     const uint32_t* __restrict__ marks,
                                                      const StepStatus* __restrict__ status) {
   __shared__ uint32_t s_off[kKeygenSpan + 1];
-  __shared__ uint32_t s_g[2];
   if (status) {
     if (status->abort) return;
     e_total = status->entries;
   }
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t g = entry_gaussian_of(e, e_total, entry_offset, n, s_off, s_g);
-  if (e >= e_total) return;
-  const uint32_t local = e - entry_offset[g];
-  const uint2 rc = rect[g];
-  const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
-  const uint32_t w = x1 - x0;
-  const uint32_t ty = y0 + local / w, tx = x0 + local % w;
-  const uint32_t tile = ty * tiles_x + tx;
-  entry_gaussian[e] = g;
-  entry_tile[e] = tile;
-  entry_slot[e] = atomicAdd(&tile_count[tile], 1u);
+  const uint32_t eb = blockIdx.x * kKeygenEntries;
+  if (eb >= e_total) return;  // uniform: no entry of this workgroup exists
+  const KeygenSpan sp = keygen_span(e_total, entry_offset, n, marks, s_off);
+#pragma unroll
+  for (uint32_t k = 0; k < kKeygenPerLane; k++) {
+    const uint32_t e = eb + k * kKeygenThreads + threadIdx.x;
+    if (e >= e_total) break;
+    const uint32_t g = keygen_find(sp, e, entry_offset, s_off);
+    const uint32_t local = e - entry_offset[g];
+    const uint2 rc = rect[g];
+    const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
+    const uint32_t w = x1 - x0;
+    const uint32_t ty = y0 + local / w, tx = x0 + local % w;
+    const uint32_t tile = ty * tiles_x + tx;
+    entry_gaussian[e] = g;
+    entry_tile[e] = tile;
+    entry_slot[e] = atomicAdd(&tile_count[entry_sub(e) * num_tiles + tile], 1u);
+  }
 }
 
 // keygen over the global tile space of a batched step: the row's pair gives its tile grid
@@ -98,24 +137,48 @@ __global__ void __launch_bounds__(256) keygen_batch_kernel(const uint32_t* __res
                                                            uint32_t* __restrict__ entry_tile,
                                                            uint32_t* __restrict__ entry_slot,
                                                            uint32_t* __restrict__ tile_count,
+                                                           uint32_t num_tiles,
+                                                           const uint32_t* __restrict__ marks,
                                                            const StepStatus* __restrict__ status) {
   __shared__ uint32_t s_off[kKeygenSpan + 1];
-  __shared__ uint32_t s_g[2];
   if (status->abort) return;
   const uint32_t e_total = status->entries;
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t g = entry_gaussian_of(e, e_total, entry_offset, n, s_off, s_g);
-  if (e >= e_total) return;
-  const uint32_t v = find_segment(bd.work_base, bd.npairs, g);
-  const uint32_t local = e - entry_offset[g];
-  const uint2 rc = rect[g];
-  const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
-  const uint32_t w = x1 - x0;
-  const uint32_t ty = y0 + local / w, tx = x0 + local % w;
-  const uint32_t tile = bd.tile_base[v] + ty * bd.tiles_x[v] + tx;
-  entry_gaussian[e] = g;
-  entry_tile[e] = tile;
-  entry_slot[e] = atomicAdd(&tile_count[tile], 1u);
+  const uint32_t eb = blockIdx.x * kKeygenEntries;
+  if (eb >= e_total) return;  // uniform: no entry of this workgroup exists
+  const KeygenSpan sp = keygen_span(e_total, entry_offset, n, marks, s_off);
+#pragma unroll
+  for (uint32_t k = 0; k < kKeygenPerLane; k++) {
+    const uint32_t e = eb + k * kKeygenThreads + threadIdx.x;
+    if (e >= e_total) break;
+    const uint32_t g = keygen_find(sp, e, entry_offset, s_off);
+    const uint32_t v = find_segment(bd.work_base, bd.npairs, g);
+    const uint32_t local = e - entry_offset[g];
+    const uint2 rc = rect[g];
+    const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
+    const uint32_t w = x1 - x0;
+    const uint32_t ty = y0 + local / w, tx = x0 + local % w;
+    const uint32_t tile = bd.tile_base[v] + ty * bd.tiles_x[v] + tx;
+    entry_gaussian[e] = g;
+    entry_tile[e] = tile;
+    entry_slot[e] = atomicAdd(&tile_count[entry_sub(e) * num_tiles + tile], 1u);
+  }
+}
+
+// Per tile: each counter set's offset within the tile and the tile's total (the input of
+// the tile_start scan).
+__global__ void __launch_bounds__(256) tile_totals_kernel(uint32_t num_tiles,
+                                                          const uint32_t* __restrict__ tile_count,
+                                                          uint32_t* __restrict__ tile_sub,
+                                                          uint32_t* __restrict__ tile_total) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= num_tiles) return;
+  uint32_t acc = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kTileSub; k++) {
+    tile_sub[k * num_tiles + t] = acc;
+    acc += tile_count[k * num_tiles + t];
+  }
+  tile_total[t] = acc;
 }
 
 __global__ void __launch_bounds__(256) scatter_kernel(uint32_t e_total,
@@ -123,6 +186,8 @@ __global__ void __launch_bounds__(256) scatter_kernel(uint32_t e_total,
                                                       const uint32_t* __restrict__ entry_tile,
                                                       const uint32_t* __restrict__ entry_slot,
                                                       const uint32_t* __restrict__ tile_start,
+                                                      const uint32_t* __restrict__ tile_sub,
+                                                      uint32_t num_tiles,
                                                       const float* __restrict__ depth,
                                                       uint64_t* __restrict__ bucket,
                                                       const StepStatus* __restrict__ status) {
@@ -132,7 +197,8 @@ __global__ void __launch_bounds__(256) scatter_kernel(uint32_t e_total,
   }
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= e_total) return;
-  const uint32_t pos = tile_start[entry_tile[e]] + entry_slot[e];
+  const uint32_t tile = entry_tile[e];
+  const uint32_t pos = tile_start[tile] + tile_sub[entry_sub(e) * num_tiles + tile] + entry_slot[e];
   bucket[pos] = ((uint64_t)f2bits(depth[entry_gaussian[e]]) << 32) | e;
 }
 
@@ -260,6 +326,15 @@ __global__ void __launch_bounds__(1024) sort_big_kernel(const uint32_t* __restri
   }
 }
 
+// Per-counter offsets and tile totals, then tile_start = exclusive scan of the totals.
+void tile_starts(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint32_t* total_dev,
+                 uint32_t* scan_sums) {
+  uint32_t* totals = bb.tile_sub + kTileSub * T;
+  launch(ctx, "bin_tile_totals", tile_totals_kernel, dim3(blocks_for(T, 256)), dim3(256), 0,
+         (uint32_t)T, (const uint32_t*)bb.tile_count, bb.tile_sub, totals);
+  exclusive_scan_u32(ctx, totals, bb.tile_start, T, total_dev, scan_sums);
+}
+
 }  // namespace
 
 size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uint32_t tiles_y,
@@ -269,7 +344,8 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   bb.tiles_x = tiles_x;
   bb.tiles_y = tiles_y;
   bb.entry_offset = (uint32_t*)store["entry_offset"].reserve((n + 1) * 4);
-  bb.tile_count = (uint32_t*)store["tile_count"].reserve(T * 4);
+  bb.tile_count = (uint32_t*)store["tile_count"].reserve(kTileSub * T * 4);
+  bb.tile_sub = (uint32_t*)store["tile_sub"].reserve((kTileSub + 1) * T * 4);
   bb.tile_start = (uint32_t*)store["tile_start"].reserve((T + 1) * 4);
   uint32_t* small = (uint32_t*)store["bin_small"].reserve(64);  // [0] E, [1] big count, [2] T
   unsigned long long* total64 = (unsigned long long*)store["bin_total64"].reserve(8);
@@ -301,20 +377,26 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   bb.sorted_entry = store["sorted_entry"].as<uint32_t>();
   bb.sorted_gauss = store["sorted_gauss"].as<uint32_t>();
 
+  bb.block_marks = (uint32_t*)store["block_marks"].reserve((E / kKeygenEntries + 2) * 4);
   // 2. keygen + per-tile counts
-  fill_u32(ctx, bb.tile_count, T, 0);
+  fill_u32(ctx, bb.tile_count, kTileSub * T, 0);
   fill_u32(ctx, small + 1, 1, 0);
-  launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(E, 256)), dim3(256), 0,
-         (const uint32_t*)bb.entry_offset, (uint32_t)n, (uint32_t)E, (const uint2*)pb.rect,
-         tiles_x, bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
+  launch(ctx, "bin_marks", block_marks_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
+         (const StepStatus*)nullptr);
+  launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(E, kKeygenEntries)),
+         dim3(kKeygenThreads), 0, (const uint32_t*)bb.entry_offset, (uint32_t)n, (uint32_t)E,
+         (const uint2*)pb.rect, tiles_x, bb.entry_gaussian, bb.entry_tile, bb.entry_slot,
+         bb.tile_count, (uint32_t)T, (const uint32_t*)bb.block_marks,
          (const StepStatus*)nullptr);
   // 3. tile starts
-  exclusive_scan_u32(ctx, bb.tile_count, bb.tile_start, T, small + 2);
+  tile_starts(ctx, bb, T, small + 2, nullptr);
   // 4. scatter into buckets
   launch(ctx, "bin_scatter", scatter_kernel, dim3(blocks_for(E, 256)), dim3(256), 0,
          (uint32_t)E, (const uint32_t*)bb.entry_gaussian, (const uint32_t*)bb.entry_tile,
          (const uint32_t*)bb.entry_slot, (const uint32_t*)bb.tile_start,
-         (const float*)pb.depth, bb.bucket, (const StepStatus*)nullptr);
+         (const uint32_t*)bb.tile_sub, (uint32_t)T, (const float*)pb.depth, bb.bucket,
+         (const StepStatus*)nullptr);
   // 5. per-tile sorts
   launch(ctx, "bin_sort_small", sort_small_kernel, dim3((unsigned)T), dim3(256), 0, (uint32_t)T,
          (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
@@ -340,24 +422,30 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
   fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
   exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64);
   bin_guard(ctx, total64, capacity, status);
-  fill_u32(ctx, bb.tile_count, T, 0);
+  fill_u32(ctx, bb.tile_count, kTileSub * T, 0);
   fill_u32(ctx, small + 1, 1, 0);
   // The entry kernels cover the capacity and read the step's count from `status`.
+  launch(ctx, "bin_marks", block_marks_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
+         (const StepStatus*)status);
   if (bd)
-    launch(ctx, "bin_keygen", keygen_batch_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0,
-           (const uint32_t*)bb.entry_offset, (uint32_t)n, *bd, (const uint2*)pb.rect,
-           bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
+    launch(ctx, "bin_keygen", keygen_batch_kernel, dim3(blocks_for(capacity, kKeygenEntries)),
+           dim3(kKeygenThreads), 0, (const uint32_t*)bb.entry_offset, (uint32_t)n, *bd,
+           (const uint2*)pb.rect, bb.entry_gaussian, bb.entry_tile, bb.entry_slot,
+           bb.tile_count, (uint32_t)T, (const uint32_t*)bb.block_marks,
            (const StepStatus*)status);
   else
-    launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0,
-           (const uint32_t*)bb.entry_offset, (uint32_t)n, 0u, (const uint2*)pb.rect, bb.tiles_x,
-           bb.entry_gaussian, bb.entry_tile, bb.entry_slot, bb.tile_count,
+    launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(capacity, kKeygenEntries)),
+           dim3(kKeygenThreads), 0, (const uint32_t*)bb.entry_offset, (uint32_t)n, 0u,
+           (const uint2*)pb.rect, bb.tiles_x, bb.entry_gaussian, bb.entry_tile, bb.entry_slot,
+           bb.tile_count, (uint32_t)T, (const uint32_t*)bb.block_marks,
            (const StepStatus*)status);
-  exclusive_scan_u32(ctx, bb.tile_count, bb.tile_start, T, small + 2, scan_sums);
+  tile_starts(ctx, bb, T, small + 2, scan_sums);
   launch(ctx, "bin_scatter", scatter_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0, 0u,
          (const uint32_t*)bb.entry_gaussian, (const uint32_t*)bb.entry_tile,
-         (const uint32_t*)bb.entry_slot, (const uint32_t*)bb.tile_start, (const float*)pb.depth,
-         bb.bucket, (const StepStatus*)status);
+         (const uint32_t*)bb.entry_slot, (const uint32_t*)bb.tile_start,
+         (const uint32_t*)bb.tile_sub, (uint32_t)T, (const float*)pb.depth, bb.bucket,
+         (const StepStatus*)status);
   launch(ctx, "bin_sort_small", sort_small_kernel, dim3((unsigned)T), dim3(256), 0, (uint32_t)T,
          (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
          bb.sorted_entry, bb.sorted_gauss, (uint64_t*)nullptr, big_list, small + 1);

@@ -324,13 +324,20 @@ struct ProjBuffers {
 };
 
 // Binning of one view.
+constexpr uint32_t kTileSub = 8;  // counter sets per tile in the binning's counting pass
+
 struct BinBuffers {
   uint32_t* entry_offset;    // n + 1 (exclusive scan of ntiles)
   uint32_t* entry_gaussian;  // E
   uint32_t* entry_tile;      // E
   uint32_t* entry_slot;      // E (position within the tile, from the counting pass)
-  uint32_t* tile_count;      // T
+  uint32_t* tile_count;      // kTileSub * T: per-tile counters, one set per keygen workgroup
+                             // index mod kTileSub, laid out [kTileSub][T]
+  uint32_t* tile_sub;        // (kTileSub + 1) * T: each counter's offset in its tile
+                             // [kTileSub][T], then the tile totals [T]
   uint32_t* tile_start;      // T + 1
+  uint32_t* block_marks;     // >= capacity / 256 + 2: Gaussian of each keygen workgroup's
+                             // first entry (a workgroup handles 256 entries)
   uint64_t* bucket;          // E: (depth bits << 32 | entry), grouped by tile
   uint32_t* sorted_entry;    // E: entry ids in (tile, depth, entry) order
   uint32_t* sorted_gauss;    // E: their Gaussian ids

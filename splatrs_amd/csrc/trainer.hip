@@ -228,7 +228,9 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   bb.tiles_x = tx;
   bb.tiles_y = ty;
   bb.entry_offset = (uint32_t*)st["entry_offset"].reserve((t->cap + 1) * 4);
-  bb.tile_count = (uint32_t*)st["tile_count"].reserve(T * 4);
+  bb.tile_count = (uint32_t*)st["tile_count"].reserve(kTileSub * T * 4);
+  bb.tile_sub = (uint32_t*)st["tile_sub"].reserve((kTileSub + 1) * T * 4);
+  bb.block_marks = (uint32_t*)st["block_marks"].reserve((ec / 256 + 2) * 4);
   bb.tile_start = (uint32_t*)st["tile_start"].reserve((T + 1) * 4);
   bb.entry_gaussian = (uint32_t*)st["entry_gaussian"].reserve(ec * 4);
   bb.entry_tile = (uint32_t*)st["entry_tile"].reserve(ec * 4);
