@@ -609,19 +609,24 @@ __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restric
 // (src/rasterize.rs:480-503; the workgroup's rows own one contiguous span of entries,
 // staged through LDS with coalesced loads), the densification statistics
 // (src/densify.rs:36-60, when accumulating) and project_backward_one (src/project.rs:
-// 335-474), whose 59 gradients go to LDS (the staging buffer, reused, [plane][256 rows])
+// 335-474), whose 59 gradients go to LDS (the staging buffer, reused, [plane][rows])
 // instead of gradient planes in HBM.  After a barrier the workgroup runs Adam
-// (src/adam.rs:44-61) over its 59 x 256 parameter and moment values as float4 groups of 4
-// consecutive rows (plane-major, so each wave covers 1 KB of one plane), the gradients
-// read back from LDS.  The arithmetic is that of the separate host-tier kernels.
-__global__ void __launch_bounds__(256) grad_adam_kernel(
+// (src/adam.rs:44-61) over its 59 x kGradAdamRows parameter and moment values as float4
+// groups of 4 consecutive rows (plane-major), the gradients read back from LDS.  The
+// arithmetic is that of the separate host-tier kernels.  128 rows per workgroup (30 KB of
+// LDS) let 2.5 waves per SIMD run at the kernel's 165 VGPRs; 256 rows (60 KB) allowed 2
+// and ran 6% slower.
+constexpr uint32_t kGradAdamRows = 128;  // Gaussian rows per grad_adam workgroup
+
+__global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
     float* __restrict__ params, float* __restrict__ m, float* __restrict__ v, uint32_t n,
     uint32_t stride, CameraDev cam, float near_plane, float eps2d, int antialias, int degree,
     const uint32_t* __restrict__ entry_offset, const float* __restrict__ slots, StatsArgs sa,
     float c1, float c2, float b1, float b2, float eps, splat_learning_rates lr,
     const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
-  constexpr uint32_t kChunk = 1024;  // entries staged per pass
-  constexpr uint32_t kShared = kChunk * kSlots > kParams * 256 ? kChunk * kSlots : kParams * 256;
+  constexpr uint32_t R = kGradAdamRows;
+  constexpr uint32_t kChunk = 4 * R;  // entries staged per pass
+  constexpr uint32_t kShared = kChunk * kSlots > kParams * R ? kChunk * kSlots : kParams * R;
   __shared__ __align__(16) float sh[kShared];
   if (status && status->abort) return;  // uniform: every lane reads the same flag
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -681,18 +686,19 @@ __global__ void __launch_bounds__(256) grad_adam_kernel(
   float g[kParams];
 #pragma unroll
   for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
-  project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, &pg, sh + threadIdx.x, 256);
+  project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, &pg, sh + threadIdx.x, R);
   }
   __syncthreads();  // every row's gradients are in LDS
   const uint32_t rows = ie - ib;
-  constexpr uint32_t kGroups = kParams * 64;  // float4 groups of the workgroup
+  constexpr uint32_t kPerPlane = R / 4;  // float4 groups per plane
+  constexpr uint32_t kGroups = kParams * kPerPlane;  // float4 groups of the workgroup
 #pragma unroll 2
   for (uint32_t q = threadIdx.x; q < kGroups; q += blockDim.x) {
-    const uint32_t j = q >> 6, r = (q & 63u) * 4u;
+    const uint32_t j = q / kPerPlane, r = (q % kPerPlane) * 4u;
     if (r >= rows) continue;
     const float rate = plane_lr((int)j, lr);
     const size_t at = (size_t)j * stride + ib + r;
-    const float* gq = sh + j * 256 + r;
+    const float* gq = sh + j * R + r;
     if (r + 4 <= rows) {
       float4 P = load_stream(params + at);
       float4 M = load_stream(m + at);
@@ -816,7 +822,8 @@ void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size
                const float* slots, const StatsArgs& sa, float c1, float c2,
                const splat_adam_state& st, const splat_learning_rates& lr, const StepDyn* dyn,
                const StepStatus* status) {
-  launch(ctx, "grad_adam", grad_adam_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, params, m,
+  launch(ctx, "grad_adam", grad_adam_kernel, dim3(blocks_for(n, kGradAdamRows)),
+         dim3(kGradAdamRows), 0, params, m,
          v, (uint32_t)n, (uint32_t)stride, camera_dev(cam), opts.near_plane, opts.eps2d,
          opts.antialias, opts.sh_degree, entry_offset, slots, sa, c1, c2, st.beta1, st.beta2,
          st.eps, lr, dyn, status);
