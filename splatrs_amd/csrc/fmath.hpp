@@ -93,7 +93,10 @@ SPLAT_HD float expf_tab(float x, const uint64_t* tab) {
 }
 
 // expf_tab for x <= 0 or NaN without branches: glibc's special cases reduce to the
-// underflow cut (x < log(0x1p-150) -> +0, which also covers -inf); NaN propagates.
+// underflow cut (x < log(0x1p-150) -> +0, which also covers -inf); NaN propagates.  The
+// cubic is evaluated with binary64 FMAs: over every binary32 x <= 0 (2.14e9 inputs) the
+// rounded result equals that of the separate multiplies and adds of expf_tab (checked
+// exhaustively against both; the two agree with glibc's expf except at one input).
 SPLAT_HD float expf_nonpos_tab(float x, const uint64_t* tab) {
   const double inv_ln2_n = 0x1.71547652b82fep+0 * 32.0;
   const double shift = 0x1.8p+52;
@@ -107,10 +110,10 @@ SPLAT_HD float expf_nonpos_tab(float x, const uint64_t* tab) {
   const double r = z - kd;
   const uint64_t t = tab[ki % 32] + (ki << 47);
   const double s = bits2d(t);
-  const double zz = c0 * r + c1;
+  const double zz = __builtin_fma(c0, r, c1);
   const double r2 = r * r;
-  double y = c2 * r + 1.0;
-  y = zz * r2 + y;
+  double y = __builtin_fma(c2, r, 1.0);
+  y = __builtin_fma(zz, r2, y);
   y = y * s;
   return x < -0x1.9fe368p6f ? 0.0f : (float)y;
 }
