@@ -165,6 +165,12 @@ void* splat_ctx_stream(splat_ctx* ctx);
  * reset.  `names` receives up to `cap` NUL-separated names in one buffer of `names_len`
  * bytes.  Returns the number of kernels in *count. */
 int splat_set_timing(splat_ctx* ctx, int enabled);
+/* Algorithm of splat_bin (no counterpart in the reference, whose bin() sorts all keys with
+ * par_sort_unstable): 0 (default) counts entries per tile and sorts each tile's depth keys,
+ * 1 sorts the rows by depth and then the entries by tile with stable radix sorts (the
+ * batched engine always bins this way).  Both give the reference's (tile, depth, entry)
+ * order. */
+int splat_set_bin_method(splat_ctx* ctx, int method);
 int splat_reset_timing(splat_ctx* ctx);
 int splat_kernel_times(splat_ctx* ctx, char* names, size_t names_len, uint64_t* launches,
                        double* total_ms, uint32_t cap, uint32_t* count);

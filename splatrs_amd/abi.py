@@ -135,6 +135,7 @@ _SIGS = {
     "splat_sync": (i32, [P]),
     "splat_ctx_stream": (P, [P]),
     "splat_set_timing": (i32, [P, i32]),
+    "splat_set_bin_method": (i32, [P, i32]),
     "splat_reset_timing": (i32, [P]),
     "splat_kernel_times": (i32, [P, P, C.c_size_t, P, P, u32, P]),
     "splat_project_all": (i32, [P, P, u64, P, P, P]),
@@ -273,6 +274,10 @@ class Context:
 
     def reset_timing(self):
         self.check(lib().splat_reset_timing(self._h), "reset_timing")
+
+    def set_bin_method(self, method):
+        """0: per-tile counting and sorts (default); 1: radix sorts (same result)."""
+        self.check(lib().splat_set_bin_method(self._h, int(method)), "set_bin_method")
 
     def kernel_times(self):
         cap = 256

@@ -202,6 +202,13 @@ int splat_set_timing(splat_ctx* ctx, int enabled) {
   });
 }
 
+int splat_set_bin_method(splat_ctx* ctx, int method) {
+  return guard(ctx, [&] {
+    SPLAT_REQUIRE(method == 0 || method == 1, "bin method must be 0 or 1");
+    ctx->bin_method = method;
+  });
+}
+
 int splat_reset_timing(splat_ctx* ctx) {
   return guard(ctx, [&] {
     ctx->drain_timing();

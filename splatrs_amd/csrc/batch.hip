@@ -199,6 +199,7 @@ struct BatchBufs {
   splat_proj_grad* pgrad;
   StepStatus* status;
   BatchDyn* dyn;
+  RadixScratch rs;
 };
 
 BatchBufs reserve_bufs(splat_batch* b) {
@@ -215,20 +216,21 @@ BatchBufs reserve_bufs(splat_batch* b) {
   x.bb.tiles_x = (uint32_t)T;
   x.bb.tiles_y = 1;
   x.bb.entry_offset = (uint32_t*)st["entry_offset"].reserve(w * 4);
-  x.bb.tile_count = (uint32_t*)st["tile_count"].reserve(kTileSub * T * 4 + 4);
-  x.bb.tile_sub = (uint32_t*)st["tile_sub"].reserve((kTileSub + 1) * T * 4 + 4);
+  // Batched steps bin by radix sorts (dense scenes put thousands of entries in a tile):
+  // no per-tile counters, buckets or per-tile sorts.
+  x.bb.tile_count = x.bb.tile_sub = x.bb.entry_tile = x.bb.entry_slot = nullptr;
+  x.bb.bucket = nullptr;
   x.bb.block_marks = (uint32_t*)st["block_marks"].reserve((b->entry_cap / 256 + 2) * 4);
   x.bb.tile_start = (uint32_t*)st["tile_start"].reserve(T * 4 + 8);
   x.bb.entry_gaussian = (uint32_t*)st["entry_gaussian"].reserve(ec * 4);
-  x.bb.entry_tile = (uint32_t*)st["entry_tile"].reserve(ec * 4);
-  x.bb.entry_slot = (uint32_t*)st["entry_slot"].reserve(ec * 4);
-  x.bb.bucket = (uint64_t*)st["bucket"].reserve(ec * 8);
   x.bb.sorted_entry = (uint32_t*)st["sorted_entry"].reserve(ec * 4);
   x.bb.sorted_gauss = (uint32_t*)st["sorted_gauss"].reserve(ec * 4);
   x.small = (uint32_t*)st["small"].reserve(64);
   x.total64 = (unsigned long long*)st["total64"].reserve(8);
-  x.big_list = (uint32_t*)st["big_list"].reserve(T * 4 + 4);
-  x.scan_sums = (uint32_t*)st["scan_sums"].reserve(scan_temp_bytes(std::max(w, T)));
+  x.big_list = nullptr;
+  x.rs = radix_scratch(st, w, ec);
+  x.scan_sums = (uint32_t*)st["scan_sums"].reserve(
+      scan_temp_bytes(std::max({w, T, radix_count_words(std::max(w, ec))})));
   x.counter = (uint32_t*)st["counter"].reserve(16);
   x.image = (float*)st["image"].reserve(px * 12 + 4);
   x.final_t = (float*)st["final_t"].reserve(px * 4 + 4);
@@ -250,8 +252,8 @@ void enqueue(splat_batch* b, const BatchBufs& x, const splat_batch_step_params& 
   const BatchDev& bd = b->bd;
   project_forward_batch(ctx, b->params.as<float>(), b->stride, bd, b->work, p.near_plane,
                         p.antialias, p.sh_degree, x.pb);
-  bin_build_device(ctx, x.pb, b->work, x.bb, b->entry_cap, x.small, x.total64, x.big_list,
-                   x.scan_sums, x.status, &bd);
+  bin_build_radix_device(ctx, x.pb, b->work, x.bb, b->entry_cap, x.rs, x.small, x.total64,
+                         x.scan_sums, x.status, &bd);
   raster_forward(ctx, x.pb, x.bb, 0, 0, p.background, x.image, x.final_t, x.n_contrib, fwd, &bd,
                  x.counter);
   loss_forward_backward_batch(ctx, x.image, bd, b->loss_blocks_total, b->pixels * 3,
@@ -492,8 +494,8 @@ int splat_batch_tune(splat_batch* b, const uint32_t* views, uint32_t nviews,
       clear_status(b, x.status);
       project_forward_batch(ctx, b->params.as<float>(), b->stride, b->bd, b->work, p->near_plane,
                             p->antialias, p->sh_degree, x.pb);
-      bin_build_device(ctx, x.pb, b->work, x.bb, b->entry_cap, x.small, x.total64, x.big_list,
-                       x.scan_sums, x.status, &b->bd);
+      bin_build_radix_device(ctx, x.pb, b->work, x.bb, b->entry_cap, x.rs, x.small, x.total64,
+                             x.scan_sums, x.status, &b->bd);
       const StepStatus st = read_status(b, x.status);
       if (!st.abort) break;
       grow_entries(b, st.total);

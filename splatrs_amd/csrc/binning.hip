@@ -326,6 +326,96 @@ __global__ void __launch_bounds__(1024) sort_big_kernel(const uint32_t* __restri
   }
 }
 
+// ---- radix binning ---------------------------------------------------------------------
+
+// Depth keys of the rows (rows without entries sort last) and their indices.
+__global__ void __launch_bounds__(256) depth_keys_kernel(const float* __restrict__ depth,
+                                                         const uint32_t* __restrict__ ntiles,
+                                                         uint32_t n, uint32_t* __restrict__ keys,
+                                                         uint32_t* __restrict__ vals,
+                                                         const StepStatus* __restrict__ status) {
+  if (status && status->abort) return;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  keys[g] = ntiles[g] ? f2bits(depth[g]) : 0xffffffffu;  // visible depths are finite > 0
+  vals[g] = g;
+}
+
+// Entry counts of the rows in depth order.
+__global__ void __launch_bounds__(256) ordered_counts_kernel(const uint32_t* __restrict__ order,
+                                                             const uint32_t* __restrict__ ntiles,
+                                                             uint32_t n, uint32_t* __restrict__ cnt,
+                                                             const StepStatus* __restrict__ status) {
+  if (status && status->abort) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = ntiles[order[i]];
+}
+
+// Position j of the depth-ordered entry list: the row (soff, marks: its offsets in depth
+// order), the tile (row-major over the row's rectangle, src/rasterize.rs:86-99) and the
+// reference's entry id (the row's Gaussian-order offset + the local index).
+template <bool BATCH>
+__global__ void __launch_bounds__(256) keygen_ordered_kernel(
+    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ order, uint32_t n,
+    uint32_t e_total, const uint32_t* __restrict__ entry_offset, const uint2* __restrict__ rect,
+    uint32_t tiles_x, BatchDev bd, const uint32_t* __restrict__ marks,
+    uint32_t* __restrict__ tkeys, uint32_t* __restrict__ tvals,
+    uint32_t* __restrict__ entry_gaussian, const StepStatus* __restrict__ status) {
+  __shared__ uint32_t s_off[kKeygenSpan + 1];
+  if (status) {
+    if (status->abort) return;
+    e_total = status->entries;
+  }
+  const uint32_t eb = blockIdx.x * kKeygenEntries;
+  if (eb >= e_total) return;  // uniform: no entry of this workgroup exists
+  const KeygenSpan sp = keygen_span(e_total, soff, n, marks, s_off);
+  const uint32_t j = eb + threadIdx.x;
+  if (j >= e_total) return;
+  const uint32_t i = keygen_find(sp, j, soff, s_off);
+  const uint32_t g = order[i];
+  const uint32_t local = j - soff[i];
+  const uint2 rc = rect[g];
+  const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
+  const uint32_t w = x1 - x0;
+  const uint32_t ty = y0 + local / w, tx = x0 + local % w;
+  uint32_t tile;
+  if (BATCH) {
+    const uint32_t v = find_segment(bd.work_base, bd.npairs, g);
+    tile = bd.tile_base[v] + ty * bd.tiles_x[v] + tx;
+  } else {
+    tile = ty * tiles_x + tx;
+  }
+  const uint32_t e = entry_offset[g] + local;
+  tkeys[j] = tile;
+  tvals[j] = e;
+  entry_gaussian[e] = g;
+}
+
+// The tile-sorted entries: sorted lists, (optional) keys and tile ranges.  Position j
+// starts every tile in (tile[j - 1], tile[j]]; the last position also ends the tiles after
+// its own, so each tile_start word is written exactly once when there are entries.
+__global__ void __launch_bounds__(256) radix_finalize_kernel(
+    const uint32_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals, uint32_t e_total,
+    uint32_t num_tiles, const uint32_t* __restrict__ entry_gaussian,
+    const float* __restrict__ depth, uint32_t* __restrict__ sorted_entry,
+    uint32_t* __restrict__ sorted_gauss, uint32_t* __restrict__ tile_start,
+    uint64_t* __restrict__ keys_out, const StepStatus* __restrict__ status) {
+  if (status) {
+    if (status->abort) return;
+    e_total = status->entries;
+  }
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= e_total) return;
+  const uint32_t t = tkeys[j], e = tvals[j];
+  const uint32_t g = entry_gaussian[e];
+  sorted_entry[j] = e;
+  sorted_gauss[j] = g;
+  if (keys_out) keys_out[j] = ((uint64_t)t << 32) | f2bits(depth[g]);
+  for (uint32_t u = j == 0 ? 0u : tkeys[j - 1] + 1u; u <= t; u++) tile_start[u] = j;
+  if (j + 1 == e_total)
+    for (uint32_t u = t + 1; u <= num_tiles; u++) tile_start[u] = e_total;
+}
+
 // Per-counter offsets and tile totals, then tile_start = exclusive scan of the totals.
 void tile_starts(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint32_t* total_dev,
                  uint32_t* scan_sums) {
@@ -376,6 +466,15 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   bb.bucket = store["bucket"].as<uint64_t>();
   bb.sorted_entry = store["sorted_entry"].as<uint32_t>();
   bb.sorted_gauss = store["sorted_gauss"].as<uint32_t>();
+  if (ctx->bin_method == 1) {  // radix sorts (same result)
+    bb.block_marks = (uint32_t*)store["block_marks"].reserve((E / kKeygenEntries + 2) * 4);
+    const RadixScratch rs = radix_scratch(store, n, E);
+    uint32_t* sums = (uint32_t*)store["radix_scan_sums"].reserve(
+        scan_temp_bytes(radix_count_words(std::max<size_t>(n + 1, E))));
+    bin_build_radix_device(ctx, pb, n, bb, E, rs, small, total64, sums, nullptr, nullptr,
+                           keys_out, (uint32_t)E, true);
+    return E;
+  }
 
   bb.block_marks = (uint32_t*)store["block_marks"].reserve((E / kKeygenEntries + 2) * 4);
   // 2. keygen + per-tile counts
@@ -453,6 +552,75 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
          (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
          bb.sorted_entry, bb.sorted_gauss, (uint64_t*)nullptr, (const uint32_t*)big_list,
          (const uint32_t*)(small + 1));
+}
+
+
+RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_t capacity) {
+  RadixScratch rs;
+  for (int k = 0; k < 2; k++) {
+    const std::string sk = std::to_string(k);
+    rs.rk[k] = (uint32_t*)store["radix_rk" + sk].reserve((n + 1) * 4);
+    rs.rv[k] = (uint32_t*)store["radix_rv" + sk].reserve((n + 1) * 4);
+    rs.ek[k] = (uint32_t*)store["radix_ek" + sk].reserve((capacity + 1) * 4);
+    rs.ev[k] = (uint32_t*)store["radix_ev" + sk].reserve((capacity + 1) * 4);
+  }
+  rs.soff = (uint32_t*)store["radix_soff"].reserve((n + 1) * 4);
+  rs.counts = (uint32_t*)store["radix_counts"].reserve(radix_count_words(std::max(n, capacity)) * 4);
+  return rs;
+}
+
+void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
+                            size_t capacity, const RadixScratch& rs, uint32_t* small,
+                            unsigned long long* total64, uint32_t* scan_sums, StepStatus* status,
+                            const BatchDev* bd, uint64_t* keys_out, uint32_t e_host,
+                            bool offsets_done) {
+  const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
+  if (!offsets_done) {
+    fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
+    exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64);
+    if (status) bin_guard(ctx, total64, capacity, status);
+  }
+  const uint32_t* n_entries = status ? &status->entries : nullptr;
+  // 1. rows by depth
+  launch(ctx, "bin_depth_keys", depth_keys_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (const float*)pb.depth, (const uint32_t*)pb.ntiles, (uint32_t)n, rs.rk[0], rs.rv[0],
+         (const StepStatus*)status);
+  const int c = radix_sort_pairs(ctx, rs.rk[0], rs.rv[0], rs.rk[1], rs.rv[1], n, (uint32_t)n,
+                                 nullptr, 32, rs.counts, scan_sums, status);
+  const uint32_t* order = rs.rv[c];
+  // 2. the rows' entry offsets in depth order
+  launch(ctx, "bin_ordered_counts", ordered_counts_kernel, dim3(blocks_for(n, 256)), dim3(256),
+         0, order, (const uint32_t*)pb.ntiles, (uint32_t)n, rs.rk[1 - c],
+         (const StepStatus*)status);
+  exclusive_scan_u32(ctx, rs.rk[1 - c], rs.soff, n, small + 4, scan_sums);
+  launch(ctx, "bin_marks", block_marks_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         (const uint32_t*)rs.soff, (uint32_t)n, bb.block_marks, (const StepStatus*)status);
+  // 3. entries in depth order, keyed by tile
+  const unsigned kg = blocks_for(status ? capacity : e_host, kKeygenEntries);
+  if (bd)
+    launch(ctx, "bin_keygen", keygen_ordered_kernel<true>, dim3(kg), dim3(kKeygenThreads), 0,
+           (const uint32_t*)rs.soff, order, (uint32_t)n, e_host, (const uint32_t*)bb.entry_offset,
+           (const uint2*)pb.rect, bb.tiles_x, *bd, (const uint32_t*)bb.block_marks, rs.ek[0],
+           rs.ev[0], bb.entry_gaussian, (const StepStatus*)status);
+  else
+    launch(ctx, "bin_keygen", keygen_ordered_kernel<false>, dim3(kg), dim3(kKeygenThreads), 0,
+           (const uint32_t*)rs.soff, order, (uint32_t)n, e_host, (const uint32_t*)bb.entry_offset,
+           (const uint2*)pb.rect, bb.tiles_x, BatchDev{}, (const uint32_t*)bb.block_marks,
+           rs.ek[0], rs.ev[0], bb.entry_gaussian, (const StepStatus*)status);
+  // 4. stable sort by tile
+  uint32_t tile_bits = 0;
+  while (tile_bits < 32 && (T >> tile_bits) > 1) tile_bits++;  // T - 1 < 2^tile_bits
+  if (((size_t)1 << tile_bits) < T) tile_bits++;
+  const int c2 = radix_sort_pairs(ctx, rs.ek[0], rs.ev[0], rs.ek[1], rs.ev[1],
+                                  status ? capacity : e_host, e_host, n_entries, tile_bits,
+                                  rs.counts, scan_sums, status);
+  // 5. lists and tile ranges
+  fill_u32(ctx, bb.tile_start, T + 1, 0u);
+  launch(ctx, "bin_finalize", radix_finalize_kernel,
+         dim3(blocks_for(status ? capacity : e_host, 256)), dim3(256), 0,
+         (const uint32_t*)rs.ek[c2], (const uint32_t*)rs.ev[c2], e_host, (uint32_t)T,
+         (const uint32_t*)bb.entry_gaussian, (const float*)pb.depth, bb.sorted_entry,
+         bb.sorted_gauss, bb.tile_start, keys_out, (const StepStatus*)status);
 }
 
 }  // namespace splat

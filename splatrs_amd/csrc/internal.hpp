@@ -102,6 +102,7 @@ struct splat_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   bool timing = false;
+  int bin_method = 0;  // splat_bin: 0 per-tile counting + sorts, 1 radix sorts
   std::vector<splat::PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
   std::map<std::string, splat::KernelStat> kstats;
@@ -417,6 +418,40 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
                       const BatchDev* bd = nullptr);
 void bin_guard(splat_ctx* ctx, const unsigned long long* total64, size_t capacity,
                StepStatus* status);
+
+// Scratch of the radix binning: (key, value) ping-pong pairs over the rows (depth order)
+// and over the entries (tile order), the rows' offsets in depth order and the radix
+// histograms.
+struct RadixScratch {
+  uint32_t *rk[2], *rv[2];  // rows (n)
+  uint32_t *ek[2], *ev[2];  // entries (capacity)
+  uint32_t* soff;           // n + 1
+  uint32_t* counts;         // radix_count_words(max(n, capacity))
+};
+RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_t capacity);
+// Tile binning by two stable radix sorts: the rows by depth, then their entries (emitted in
+// depth order, no per-tile counters) by tile.  Each Gaussian has at most one entry per
+// tile and entry ids grow with the Gaussian index, so the result is the (tile, depth,
+// entry) order of bin_build_device, built from bb's entry_offset, entry_gaussian,
+// block_marks, tile_start, sorted_entry and sorted_gauss.  With `status` (graph steps) the
+// entry count is read on the device; without it `e_host` is the count.  keys_out
+// (optional) receives the sorted (tile << 32 | depth bits) keys.
+void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
+                            size_t capacity, const RadixScratch& rs, uint32_t* small,
+                            unsigned long long* total64, uint32_t* scan_sums, StepStatus* status,
+                            const BatchDev* bd, uint64_t* keys_out = nullptr,
+                            uint32_t e_host = 0, bool offsets_done = false);
+
+// radix.hip
+// Stable LSD radix sort of (key, value) u32 pairs by key bits [0, key_bits), 8 bits per
+// pass, ping-ponging between (k0, v0) and (k1, v1); returns the index (0 or 1) of the pair
+// that holds the result.  The item count is *n_dev when n_dev is given (grids cover
+// `capacity`), else n_host.  counts: radix_count_words(capacity) words; scan_sums:
+// scan_temp_bytes(radix_count_words(capacity)).
+size_t radix_count_words(size_t capacity);
+int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                     size_t capacity, uint32_t n_host, const uint32_t* n_dev, uint32_t key_bits,
+                     uint32_t* counts, uint32_t* scan_sums, const StepStatus* status);
 
 // raster.hip
 // Launch shape of the compositing kernels (results do not depend on it): entries per LDS

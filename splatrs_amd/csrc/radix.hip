@@ -1,0 +1,127 @@
+// Stable LSD radix sort of (key, value) pairs of 32-bit words, 8 key bits per pass.
+//
+// Each pass: an upsweep writes every block's digit histogram, digit-major
+// (counts[digit * nblocks + block]); an exclusive scan of that array gives every
+// (digit, block) its first output position; a downsweep re-reads the block's items in
+// index order (16 rounds of 256) and scatters each to its digit's next position.  Within a
+// round, a lane's rank among the wave's lanes with the same digit comes from ballots over
+// the digit bits (multi-split), and the waves' counts are combined in LDS in wave order, so
+// equal digits keep their input order: the pass is stable and the sort is exact.
+//
+// Item counts may live on the device (graph-captured steps): kernels then read the count
+// through a pointer and cover a grid sized for the capacity.
+#include "internal.hpp"
+
+namespace splat {
+namespace {
+
+constexpr uint32_t kRadixThreads = 256;
+constexpr uint32_t kRadixRounds = 16;
+constexpr uint32_t kRadixItems = kRadixThreads * kRadixRounds;  // items per block
+constexpr uint32_t kRadixBins = 256;
+
+__device__ __forceinline__ uint32_t item_count(uint32_t n, const uint32_t* n_dev) {
+  return n_dev ? *n_dev : n;
+}
+
+__global__ void __launch_bounds__(kRadixThreads) radix_upsweep_kernel(
+    const uint32_t* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ n_dev,
+    uint32_t shift, uint32_t mask, uint32_t nblocks, uint32_t* __restrict__ counts,
+    const StepStatus* __restrict__ status) {
+  __shared__ uint32_t hist[kRadixBins];
+  if (status && status->abort) return;  // uniform
+  n = item_count(n, n_dev);
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kRadixItems;
+  for (uint32_t r = 0; r < kRadixRounds; r++) {
+    const uint32_t i = base + r * kRadixThreads + threadIdx.x;
+    if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  counts[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n,
+    const uint32_t* __restrict__ n_dev, uint32_t shift, uint32_t nbits, uint32_t nblocks,
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, const StepStatus* __restrict__ status) {
+  __shared__ uint32_t run[kRadixBins];            // next output position of each digit
+  __shared__ uint32_t wcnt[kRadixThreads / 64][kRadixBins];  // this round, per wave
+  if (status && status->abort) return;  // uniform
+  n = item_count(n, n_dev);
+  const uint32_t base = blockIdx.x * kRadixItems;
+  if (base >= n) return;  // uniform: the block holds no item
+  const uint32_t t = threadIdx.x, wave = t >> 6;
+  const uint32_t mask = (1u << nbits) - 1u;
+  run[t] = offsets[t * nblocks + blockIdx.x];
+#pragma unroll
+  for (uint32_t w = 0; w < kRadixThreads / 64; w++) wcnt[w][t] = 0;
+  __syncthreads();
+  for (uint32_t r = 0; r < kRadixRounds; r++) {
+    const uint32_t i = base + r * kRadixThreads + t;
+    if (base + r * kRadixThreads >= n) break;  // uniform
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys_in[i] : 0u;
+    const uint32_t val = valid ? vals_in[i] : 0u;
+    const uint32_t d = (key >> shift) & mask;
+    // lanes of this wave holding the same digit
+    uint64_t same = __ballot(valid);
+    for (uint32_t b = 0; b < nbits; b++) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t ones = __ballot(valid && bit);
+      same &= bit ? ones : ~ones;
+    }
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(same >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)same, 0u));
+    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__builtin_popcountll(same);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = run[d] + rank;
+      for (uint32_t w = 0; w < wave; w++) pos += wcnt[w][d];
+      keys_out[pos] = key;
+      vals_out[pos] = val;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kRadixThreads / 64; w++) {
+      add += wcnt[w][t];
+      wcnt[w][t] = 0;
+    }
+    run[t] += add;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+size_t radix_count_words(size_t capacity) {
+  return (size_t)kRadixBins * blocks_for(capacity == 0 ? 1 : capacity, kRadixItems) + 1;
+}
+
+int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                     size_t capacity, uint32_t n_host, const uint32_t* n_dev, uint32_t key_bits,
+                     uint32_t* counts, uint32_t* scan_sums, const StepStatus* status) {
+  const uint32_t nblocks = blocks_for(capacity == 0 ? 1 : capacity, kRadixItems);
+  int cur = 0;
+  uint32_t* keys[2] = {k0, k1};
+  uint32_t* vals[2] = {v0, v1};
+  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+    const uint32_t nbits = std::min(8u, key_bits - shift);
+    launch(ctx, "radix_upsweep", radix_upsweep_kernel, dim3(nblocks), dim3(kRadixThreads), 0,
+           (const uint32_t*)keys[cur], n_host, n_dev, shift, (1u << nbits) - 1u, nblocks, counts,
+           status);
+    // in place; the total lands in the array's last word
+    const size_t len = (size_t)kRadixBins * nblocks;
+    exclusive_scan_u32(ctx, counts, counts, len, counts + len, scan_sums);
+    launch(ctx, "radix_downsweep", radix_downsweep_kernel, dim3(nblocks), dim3(kRadixThreads), 0,
+           (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], n_host, n_dev, shift, nbits,
+           nblocks, (const uint32_t*)counts, keys[1 - cur], vals[1 - cur], status);
+    cur = 1 - cur;
+  }
+  return cur;
+}
+
+}  // namespace splat

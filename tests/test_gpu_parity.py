@@ -302,3 +302,41 @@ def test_full_size_1m_1080p_tree_reduced_backward(ctx):
     say("1M Gaussians, 1920x1080, tree-reduced backward (the benchmark's mode)")
     full_pipeline(ctx, scenes.large_scene(0), scenes.hd_camera(), rec.proj_opts(3),
                   (0.0, 0.0, 0.0), ordered=False)
+
+
+def radix_cases():
+    rng = np.random.default_rng(1)
+    dup = ground_truth(rng, 40)
+    dup = np.concatenate([dup, dup[:20], dup[:5]])  # equal depth keys
+    crowd = np.zeros((20000, 59), np.float32)
+    r2 = np.random.default_rng(20000)
+    crowd[:, 0:3] = r2.normal(0, 0.02, (20000, 3))
+    crowd[:, 3:6] = -0.5
+    crowd[:, 6:10] = r2.standard_normal((20000, 4))
+    crowd[:, 10] = -3.0
+    crowd[:, 11:14] = r2.normal(0.5, 0.2, (20000, 3))
+    crowd_cam = scenes.look_at(64, 64, 60.0, 60.0, (0.0, 0.0, -4.0), (0.0, 0.0, 0.0))
+    g, cam, opts, _ = scene_small()
+    return [("small ring scene", g, cam, opts),
+            ("duplicated Gaussians", dup, ring_cameras(1, 48, 40, 2.4)[0], rec.proj_opts(1)),
+            ("20000 entries per tile", crowd, crowd_cam, rec.proj_opts(0)),
+            ("10k Gaussians 256x256", scenes.small_scene(0), scenes.small_scene_cameras()[3],
+             rec.proj_opts(3)),
+            ("1M Gaussians 1920x1080", scenes.large_scene(0), scenes.hd_camera(), rec.proj_opts(3))]
+
+
+def test_radix_binning_equals_oracle(ctx):
+    """The radix binning (rows by depth, then entries by tile, both stable) gives the
+    reference's (tile, depth, entry) order, bitwise, from tiny to crowded to full size."""
+    ctx.set_bin_method(1)
+    try:
+        for name, g, cam, opts in radix_cases():
+            say(f"radix binning: {name}")
+            w, h = int(cam["width"]), int(cam["height"])
+            pg = ctx.project_all(g, cam, opts)
+            assert_binning_equal(ctx.bin(pg, w, h), O.bin(O.project_all(g, cam, opts), w, h))
+        say("radix binning: empty model")
+        b = ctx.bin(ctx.project_all(np.zeros((0, 59), np.float32), cam, opts), 64, 48)
+        assert b.num_entries == 0 and (b.tile_range == 0).all()
+    finally:
+        ctx.set_bin_method(0)
