@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(256) keygen_ordered_kernel(
     const uint32_t* __restrict__ soff, const uint32_t* __restrict__ order, uint32_t n,
     uint32_t e_total, const uint32_t* __restrict__ entry_offset, const uint2* __restrict__ rect,
     uint32_t tiles_x, BatchDev bd, const uint32_t* __restrict__ marks,
-    uint32_t* __restrict__ tkeys, uint32_t* __restrict__ tvals,
+    uint32_t* __restrict__ tkeys, uint32_t* __restrict__ tvals, uint32_t* __restrict__ tgauss,
     uint32_t* __restrict__ entry_gaussian, const StepStatus* __restrict__ status) {
   __shared__ uint32_t s_off[kKeygenSpan + 1];
   if (status) {
@@ -388,15 +388,16 @@ __global__ void __launch_bounds__(256) keygen_ordered_kernel(
   const uint32_t e = entry_offset[g] + local;
   tkeys[j] = tile;
   tvals[j] = e;
-  entry_gaussian[e] = g;
+  tgauss[j] = g;
+  if (entry_gaussian) entry_gaussian[e] = g;
 }
 
 // The tile-sorted entries: sorted lists, (optional) keys and tile ranges.  Position j
 // starts every tile in (tile[j - 1], tile[j]]; the last position also ends the tiles after
 // its own, so each tile_start word is written exactly once when there are entries.
 __global__ void __launch_bounds__(256) radix_finalize_kernel(
-    const uint32_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals, uint32_t e_total,
-    uint32_t num_tiles, const uint32_t* __restrict__ entry_gaussian,
+    const uint32_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals,
+    const uint32_t* __restrict__ tgauss, uint32_t e_total, uint32_t num_tiles,
     const float* __restrict__ depth, uint32_t* __restrict__ sorted_entry,
     uint32_t* __restrict__ sorted_gauss, uint32_t* __restrict__ tile_start,
     uint64_t* __restrict__ keys_out, const StepStatus* __restrict__ status) {
@@ -406,8 +407,7 @@ __global__ void __launch_bounds__(256) radix_finalize_kernel(
   }
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= e_total) return;
-  const uint32_t t = tkeys[j], e = tvals[j];
-  const uint32_t g = entry_gaussian[e];
+  const uint32_t t = tkeys[j], e = tvals[j], g = tgauss[j];
   sorted_entry[j] = e;
   sorted_gauss[j] = g;
   if (keys_out) keys_out[j] = ((uint64_t)t << 32) | f2bits(depth[g]);
@@ -563,6 +563,7 @@ RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_
     rs.rv[k] = (uint32_t*)store["radix_rv" + sk].reserve((n + 1) * 4);
     rs.ek[k] = (uint32_t*)store["radix_ek" + sk].reserve((capacity + 1) * 4);
     rs.ev[k] = (uint32_t*)store["radix_ev" + sk].reserve((capacity + 1) * 4);
+    rs.eg[k] = (uint32_t*)store["radix_eg" + sk].reserve((capacity + 1) * 4);
   }
   rs.soff = (uint32_t*)store["radix_soff"].reserve((n + 1) * 4);
   rs.counts = (uint32_t*)store["radix_counts"].reserve(radix_count_words(std::max(n, capacity)) * 4);
@@ -601,25 +602,25 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
     launch(ctx, "bin_keygen", keygen_ordered_kernel<true>, dim3(kg), dim3(kKeygenThreads), 0,
            (const uint32_t*)rs.soff, order, (uint32_t)n, e_host, (const uint32_t*)bb.entry_offset,
            (const uint2*)pb.rect, bb.tiles_x, *bd, (const uint32_t*)bb.block_marks, rs.ek[0],
-           rs.ev[0], bb.entry_gaussian, (const StepStatus*)status);
+           rs.ev[0], rs.eg[0], bb.entry_gaussian, (const StepStatus*)status);
   else
     launch(ctx, "bin_keygen", keygen_ordered_kernel<false>, dim3(kg), dim3(kKeygenThreads), 0,
            (const uint32_t*)rs.soff, order, (uint32_t)n, e_host, (const uint32_t*)bb.entry_offset,
            (const uint2*)pb.rect, bb.tiles_x, BatchDev{}, (const uint32_t*)bb.block_marks,
-           rs.ek[0], rs.ev[0], bb.entry_gaussian, (const StepStatus*)status);
+           rs.ek[0], rs.ev[0], rs.eg[0], bb.entry_gaussian, (const StepStatus*)status);
   // 4. stable sort by tile
   uint32_t tile_bits = 0;
   while (tile_bits < 32 && (T >> tile_bits) > 1) tile_bits++;  // T - 1 < 2^tile_bits
   if (((size_t)1 << tile_bits) < T) tile_bits++;
   const int c2 = radix_sort_pairs(ctx, rs.ek[0], rs.ev[0], rs.ek[1], rs.ev[1],
                                   status ? capacity : e_host, e_host, n_entries, tile_bits,
-                                  rs.counts, scan_sums, status);
+                                  rs.counts, scan_sums, status, rs.eg[0], rs.eg[1]);
   // 5. lists and tile ranges
   fill_u32(ctx, bb.tile_start, T + 1, 0u);
   launch(ctx, "bin_finalize", radix_finalize_kernel,
          dim3(blocks_for(status ? capacity : e_host, 256)), dim3(256), 0,
-         (const uint32_t*)rs.ek[c2], (const uint32_t*)rs.ev[c2], e_host, (uint32_t)T,
-         (const uint32_t*)bb.entry_gaussian, (const float*)pb.depth, bb.sorted_entry,
+         (const uint32_t*)rs.ek[c2], (const uint32_t*)rs.ev[c2], (const uint32_t*)rs.eg[c2],
+         e_host, (uint32_t)T, (const float*)pb.depth, bb.sorted_entry,
          bb.sorted_gauss, bb.tile_start, keys_out, (const StepStatus*)status);
 }
 

@@ -424,7 +424,7 @@ void bin_guard(splat_ctx* ctx, const unsigned long long* total64, size_t capacit
 // histograms.
 struct RadixScratch {
   uint32_t *rk[2], *rv[2];  // rows (n)
-  uint32_t *ek[2], *ev[2];  // entries (capacity)
+  uint32_t *ek[2], *ev[2], *eg[2];  // entries (capacity): tile, entry id, Gaussian
   uint32_t* soff;           // n + 1
   uint32_t* counts;         // radix_count_words(max(n, capacity))
 };
@@ -432,10 +432,10 @@ RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_
 // Tile binning by two stable radix sorts: the rows by depth, then their entries (emitted in
 // depth order, no per-tile counters) by tile.  Each Gaussian has at most one entry per
 // tile and entry ids grow with the Gaussian index, so the result is the (tile, depth,
-// entry) order of bin_build_device, built from bb's entry_offset, entry_gaussian,
-// block_marks, tile_start, sorted_entry and sorted_gauss.  With `status` (graph steps) the
-// entry count is read on the device; without it `e_host` is the count.  keys_out
-// (optional) receives the sorted (tile << 32 | depth bits) keys.
+// entry) order of bin_build_device, built from bb's entry_offset, block_marks, tile_start,
+// sorted_entry and sorted_gauss (and entry_gaussian when bb.entry_gaussian is set).  With
+// `status` (graph steps) the entry count is read on the device; without it `e_host` is the
+// count.  keys_out (optional) receives the sorted (tile << 32 | depth bits) keys.
 void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
                             size_t capacity, const RadixScratch& rs, uint32_t* small,
                             unsigned long long* total64, uint32_t* scan_sums, StepStatus* status,
@@ -444,14 +444,15 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
 
 // radix.hip
 // Stable LSD radix sort of (key, value) u32 pairs by key bits [0, key_bits), 8 bits per
-// pass, ping-ponging between (k0, v0) and (k1, v1); returns the index (0 or 1) of the pair
-// that holds the result.  The item count is *n_dev when n_dev is given (grids cover
-// `capacity`), else n_host.  counts: radix_count_words(capacity) words; scan_sums:
-// scan_temp_bytes(radix_count_words(capacity)).
+// pass, ping-ponging between (k0, v0) and (k1, v1) (and a second value between w0 and w1
+// when given); returns the index (0 or 1) of the buffers that hold the result.  The item
+// count is *n_dev when n_dev is given (grids cover `capacity`), else n_host.  counts:
+// radix_count_words(capacity) words; scan_sums: scan_temp_bytes(radix_count_words(capacity)).
 size_t radix_count_words(size_t capacity);
 int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                      size_t capacity, uint32_t n_host, const uint32_t* n_dev, uint32_t key_bits,
-                     uint32_t* counts, uint32_t* scan_sums, const StepStatus* status);
+                     uint32_t* counts, uint32_t* scan_sums, const StepStatus* status,
+                     uint32_t* w0 = nullptr, uint32_t* w1 = nullptr);
 
 // raster.hip
 // Launch shape of the compositing kernels (results do not depend on it): entries per LDS

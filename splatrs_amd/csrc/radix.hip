@@ -1,15 +1,20 @@
-// Stable LSD radix sort of (key, value) pairs of 32-bit words, 8 key bits per pass.
+// Stable LSD radix sort of (key, value[, value2]) tuples of 32-bit words, 8 key bits per
+// pass.
 //
 // Each pass: an upsweep writes every block's digit histogram, digit-major
 // (counts[digit * nblocks + block]); an exclusive scan of that array gives every
-// (digit, block) its first output position; a downsweep re-reads the block's items in
-// index order (16 rounds of 256) and scatters each to its digit's next position.  Within a
-// round, a lane's rank among the wave's lanes with the same digit comes from ballots over
-// the digit bits (multi-split), and the waves' counts are combined in LDS in wave order, so
-// equal digits keep their input order: the pass is stable and the sort is exact.
+// (digit, block) its first output position; a downsweep loads the block's 4096 items
+// (16 per lane, prefetched), ranks them in index order (16 rounds of 256) into a
+// block-local digit order in LDS, and writes them out run by run, so consecutive lanes
+// store consecutive addresses.  Within a round, a lane's rank among the wave's lanes with
+// the same digit comes from ballots over the digit bits (multi-split), and the waves'
+// counts are combined in LDS in wave order, so equal digits keep their input order: the
+// pass is stable and the sort is exact.
 //
 // Item counts may live on the device (graph-captured steps): kernels then read the count
 // through a pointer and cover a grid sized for the capacity.
+#include <utility>
+
 #include "internal.hpp"
 
 namespace splat {
@@ -19,9 +24,21 @@ constexpr uint32_t kRadixThreads = 256;
 constexpr uint32_t kRadixRounds = 16;
 constexpr uint32_t kRadixItems = kRadixThreads * kRadixRounds;  // items per block
 constexpr uint32_t kRadixBins = 256;
+constexpr uint32_t kRadixWaves = kRadixThreads / 64;
 
 __device__ __forceinline__ uint32_t item_count(uint32_t n, const uint32_t* n_dev) {
   return n_dev ? *n_dev : n;
+}
+
+template <class F, int... I>
+__device__ __forceinline__ void rounds_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(integral_constant<0>) ... f(integral_constant<kRadixRounds - 1>): register arrays
+// indexed by the round stay in registers.
+template <class F>
+__device__ __forceinline__ void for_rounds(F&& f) {
+  rounds_impl(f, std::make_integer_sequence<int, kRadixRounds>{});
 }
 
 __global__ void __launch_bounds__(kRadixThreads) radix_upsweep_kernel(
@@ -42,30 +59,61 @@ __global__ void __launch_bounds__(kRadixThreads) radix_upsweep_kernel(
   counts[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
 }
 
+template <bool TWO>
 __global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
-    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t n,
-    const uint32_t* __restrict__ n_dev, uint32_t shift, uint32_t nbits, uint32_t nblocks,
-    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out, const StepStatus* __restrict__ status) {
-  __shared__ uint32_t run[kRadixBins];            // next output position of each digit
-  __shared__ uint32_t wcnt[kRadixThreads / 64][kRadixBins];  // this round, per wave
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    const uint32_t* __restrict__ vals2_in, uint32_t n, const uint32_t* __restrict__ n_dev,
+    uint32_t shift, uint32_t nbits, uint32_t nblocks, const uint32_t* __restrict__ offsets,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out,
+    uint32_t* __restrict__ vals2_out, const StepStatus* __restrict__ status) {
+  __shared__ uint32_t s_key[kRadixItems], s_val[kRadixItems];
+  __shared__ uint32_t s_val2[TWO ? kRadixItems : 1];
+  __shared__ uint32_t gbase[kRadixBins];  // first output position of each digit
+  __shared__ uint32_t lbase[kRadixBins];  // first block-local position of each digit
+  __shared__ uint32_t run[kRadixBins];    // next block-local position of each digit
+  __shared__ uint32_t wcnt[kRadixWaves][kRadixBins];  // this round's counts per wave
+  __shared__ uint32_t wsum[kRadixWaves];
   if (status && status->abort) return;  // uniform
   n = item_count(n, n_dev);
   const uint32_t base = blockIdx.x * kRadixItems;
   if (base >= n) return;  // uniform: the block holds no item
-  const uint32_t t = threadIdx.x, wave = t >> 6;
+  const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
-  run[t] = offsets[t * nblocks + blockIdx.x];
-#pragma unroll
-  for (uint32_t w = 0; w < kRadixThreads / 64; w++) wcnt[w][t] = 0;
-  __syncthreads();
-  for (uint32_t r = 0; r < kRadixRounds; r++) {
+  const uint32_t items = min(kRadixItems, n - base);
+  // items of the block, prefetched
+  uint32_t k[kRadixRounds], v[kRadixRounds], v2[kRadixRounds];
+  for_rounds([&](auto step) {
+    constexpr int r = decltype(step)::value;
     const uint32_t i = base + r * kRadixThreads + t;
-    if (base + r * kRadixThreads >= n) break;  // uniform
     const bool valid = i < n;
-    const uint32_t key = valid ? keys_in[i] : 0u;
-    const uint32_t val = valid ? vals_in[i] : 0u;
-    const uint32_t d = (key >> shift) & mask;
+    k[r] = valid ? keys_in[i] : 0u;
+    v[r] = valid ? vals_in[i] : 0u;
+    if (TWO) v2[r] = valid ? vals2_in[i] : 0u;
+  });
+  // this block's digit counts (consecutive words of the scanned histogram) and their
+  // block-local exclusive scan
+  const uint32_t g0 = offsets[t * nblocks + blockIdx.x];
+  const uint32_t c = offsets[t * nblocks + blockIdx.x + 1] - g0;
+  uint32_t x = c;
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wave] = x;
+#pragma unroll
+  for (uint32_t w = 0; w < kRadixWaves; w++) wcnt[w][t] = 0;
+  __syncthreads();
+  uint32_t lb = x - c;
+  for (uint32_t w = 0; w < wave; w++) lb += wsum[w];
+  gbase[t] = g0;
+  lbase[t] = lb;
+  run[t] = lb;
+  __syncthreads();
+  for_rounds([&](auto step) {
+    constexpr int r = decltype(step)::value;
+    if (base + r * kRadixThreads >= n) return;  // uniform
+    const bool valid = base + r * kRadixThreads + t < n;
+    const uint32_t d = (k[r] >> shift) & mask;
     // lanes of this wave holding the same digit
     uint64_t same = __ballot(valid);
     for (uint32_t b = 0; b < nbits; b++) {
@@ -78,20 +126,30 @@ __global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
     if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__builtin_popcountll(same);
     __syncthreads();
     if (valid) {
-      uint32_t pos = run[d] + rank;
-      for (uint32_t w = 0; w < wave; w++) pos += wcnt[w][d];
-      keys_out[pos] = key;
-      vals_out[pos] = val;
+      uint32_t lp = run[d] + rank;
+      for (uint32_t w = 0; w < wave; w++) lp += wcnt[w][d];
+      s_key[lp] = k[r];
+      s_val[lp] = v[r];
+      if (TWO) s_val2[lp] = v2[r];
     }
     __syncthreads();
     uint32_t add = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kRadixThreads / 64; w++) {
+    for (uint32_t w = 0; w < kRadixWaves; w++) {
       add += wcnt[w][t];
       wcnt[w][t] = 0;
     }
     run[t] += add;
     __syncthreads();
+  });
+  // block-local order -> output, run by run
+  for (uint32_t q = t; q < items; q += kRadixThreads) {
+    const uint32_t key = s_key[q];
+    const uint32_t d = (key >> shift) & mask;
+    const uint32_t pos = gbase[d] + (q - lbase[d]);
+    keys_out[pos] = key;
+    vals_out[pos] = s_val[q];
+    if (TWO) vals2_out[pos] = s_val2[q];
   }
 }
 
@@ -103,11 +161,14 @@ size_t radix_count_words(size_t capacity) {
 
 int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                      size_t capacity, uint32_t n_host, const uint32_t* n_dev, uint32_t key_bits,
-                     uint32_t* counts, uint32_t* scan_sums, const StepStatus* status) {
+                     uint32_t* counts, uint32_t* scan_sums, const StepStatus* status,
+                     uint32_t* w0, uint32_t* w1) {
   const uint32_t nblocks = blocks_for(capacity == 0 ? 1 : capacity, kRadixItems);
   int cur = 0;
   uint32_t* keys[2] = {k0, k1};
   uint32_t* vals[2] = {v0, v1};
+  uint32_t* vals2[2] = {w0, w1};
+  const bool two = w0 != nullptr;
   for (uint32_t shift = 0; shift < key_bits; shift += 8) {
     const uint32_t nbits = std::min(8u, key_bits - shift);
     launch(ctx, "radix_upsweep", radix_upsweep_kernel, dim3(nblocks), dim3(kRadixThreads), 0,
@@ -116,9 +177,16 @@ int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, u
     // in place; the total lands in the array's last word
     const size_t len = (size_t)kRadixBins * nblocks;
     exclusive_scan_u32(ctx, counts, counts, len, counts + len, scan_sums);
-    launch(ctx, "radix_downsweep", radix_downsweep_kernel, dim3(nblocks), dim3(kRadixThreads), 0,
-           (const uint32_t*)keys[cur], (const uint32_t*)vals[cur], n_host, n_dev, shift, nbits,
-           nblocks, (const uint32_t*)counts, keys[1 - cur], vals[1 - cur], status);
+    if (two)
+      launch(ctx, "radix_downsweep", radix_downsweep_kernel<true>, dim3(nblocks),
+             dim3(kRadixThreads), 0, (const uint32_t*)keys[cur], (const uint32_t*)vals[cur],
+             (const uint32_t*)vals2[cur], n_host, n_dev, shift, nbits, nblocks,
+             (const uint32_t*)counts, keys[1 - cur], vals[1 - cur], vals2[1 - cur], status);
+    else
+      launch(ctx, "radix_downsweep", radix_downsweep_kernel<false>, dim3(nblocks),
+             dim3(kRadixThreads), 0, (const uint32_t*)keys[cur], (const uint32_t*)vals[cur],
+             (const uint32_t*)nullptr, n_host, n_dev, shift, nbits, nblocks,
+             (const uint32_t*)counts, keys[1 - cur], vals[1 - cur], (uint32_t*)nullptr, status);
     cur = 1 - cur;
   }
   return cur;
