@@ -349,15 +349,42 @@ constexpr int kDppXor8 = 0x128, kDppXor7 = 0x141, kDppXor2 = 0x4e, kDppXor1 = 0x
 // partners keep complementary halves of their values (chosen by bit 3, 2, 1 of the lane)
 // and add the half the other sends.  The four masks span all 16 lanes, so every sum covers
 // the row; the order of the additions is fixed.
+//
+// The first two steps select by lane bits 3 and 2, which are exactly the DPP banks (lanes
+// 4k .. 4k + 3 of a row): two bank-masked DPP adds, each writing half of the row, give
+// x + x(partner) on one half and y + y(partner) on the other.  The compiler does not form
+// masked DPP adds, so they are written out; the s_nops give the DPP source reads the wait
+// states they need after the VALU writes around the block.
+__device__ __forceinline__ float dpp_halves_xor8(float x, float y) {  // banks 0-1: x, 2-3: y
+  float out;
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_add_f32_dpp %0, %1, %1 row_ror:8 row_mask:0xf bank_mask:0x3\n\t"
+      "v_add_f32_dpp %0, %2, %2 row_ror:8 row_mask:0xf bank_mask:0xc\n\t"
+      "s_nop 1"
+      : "=&v"(out)
+      : "v"(x), "v"(y));
+  return out;
+}
+__device__ __forceinline__ float dpp_halves_xor7(float x, float y) {  // banks 0, 2: x; 1, 3: y
+  float out;
+  asm volatile(
+      "s_nop 1\n\t"
+      "v_add_f32_dpp %0, %1, %1 row_half_mirror row_mask:0xf bank_mask:0x5\n\t"
+      "v_add_f32_dpp %0, %2, %2 row_half_mirror row_mask:0xf bank_mask:0xa\n\t"
+      "s_nop 1"
+      : "=&v"(out)
+      : "v"(x), "v"(y));
+  return out;
+}
+
 __device__ __forceinline__ float row_scatter8(const float v[8], uint32_t lane) {
-  const bool b3 = (lane & 8u) != 0, b2 = (lane & 4u) != 0, b1 = (lane & 2u) != 0;
+  const bool b1 = (lane & 2u) != 0;
   float a[4], b[2];
 #pragma unroll
-  for (int q = 0; q < 4; q++)
-    a[q] = (b3 ? v[q + 4] : v[q]) + dpp_row<kDppXor8>(b3 ? v[q] : v[q + 4]);
+  for (int q = 0; q < 4; q++) a[q] = dpp_halves_xor8(v[q], v[q + 4]);
 #pragma unroll
-  for (int q = 0; q < 2; q++)
-    b[q] = (b2 ? a[q + 2] : a[q]) + dpp_row<kDppXor7>(b2 ? a[q] : a[q + 2]);
+  for (int q = 0; q < 2; q++) b[q] = dpp_halves_xor7(a[q], a[q + 2]);
   const float c = (b1 ? b[1] : b[0]) + dpp_row<kDppXor2>(b1 ? b[0] : b[1]);
   return c + dpp_row<kDppXor1>(c);
 }
