@@ -78,7 +78,7 @@ struct splat_batch {
   size_t rows = 0, stride = 0;
   uint64_t adam_step = 0;
   bool started = false;
-  DevBuf params, m, v, grads;
+  DevBuf params, m, v;
   std::map<std::string, DevBuf> store;
   size_t entry_cap = 0;
   RasterVariant fwd, bwd;  // launch shapes (splat_batch_tune)
@@ -120,6 +120,8 @@ void build_plan(splat_batch* b, const std::vector<uint32_t>& views) {
     pix_base[k] = (uint32_t)pix;
     lbb[k] = (uint32_t)lblocks;
     tiles_x[k] = (W + kTile - 1) / kTile;
+    SPLAT_REQUIRE((size_t)tiles_x[k] * ((H + kTile - 1) / kTile) < kMaxViewTiles,
+                  "view too large for radix binning");
     width[k] = W;
     height[k] = H;
     cams[k] = camera_dev(v.cam);
@@ -261,14 +263,10 @@ void enqueue(splat_batch* b, const BatchBufs& x, const splat_batch_step_params& 
   raster_backward(ctx, x.pb, x.bb, 0, 0, p.background, x.final_t, x.n_contrib, x.dimage,
                   p.ordered_backward != 0, x.slots, false, bwd, &bd, x.counter);
   reduce_slots(ctx, x.bb.entry_offset, x.slots, b->work, x.pgrad, nullptr, x.status);
-  project_backward_batch(ctx, b->params.as<float>(), b->stride, bd, b->rows, p.near_plane,
-                         p.antialias, p.sh_degree, x.pgrad, b->grads.as<float>());
   splat_adam_state ast{0.9f, 0.999f, 1e-15f, 0, 0};
-  splat_learning_rates lr0;
-  std::memset(&lr0, 0, sizeof(lr0));
-  adam_planes(ctx, b->params.as<float>(), b->grads.as<float>(), b->m.as<float>(),
-              b->v.as<float>(), b->rows, b->stride, 1.0f, 1.0f, ast, lr0, &x.dyn->step, x.status,
-              &bd);
+  grad_adam_batch(ctx, b->params.as<float>(), b->m.as<float>(), b->v.as<float>(), b->stride, bd,
+                  b->rows, p.near_plane, p.antialias, p.sh_degree, x.bb.entry_offset, x.pgrad, ast,
+                  &x.dyn->step, x.status);
 }
 
 void start(splat_batch* b) {
@@ -382,7 +380,6 @@ int splat_batch_add_scene(splat_batch* b, const float* gaussians, uint64_t n, ui
     std::swap(b->m.bytes, nm.bytes);
     std::swap(b->v.ptr, nv.ptr);
     std::swap(b->v.bytes, nv.bytes);
-    b->grads.reserve(stride * kParams * 4);
     b->rows = rows;
     b->stride = stride;
     b->store.erase("aos");

@@ -341,25 +341,69 @@ __global__ void __launch_bounds__(256) depth_keys_kernel(const float* __restrict
   vals[g] = g;
 }
 
-// Entry counts of the rows in depth order.
-__global__ void __launch_bounds__(256) ordered_counts_kernel(const uint32_t* __restrict__ order,
-                                                             const uint32_t* __restrict__ ntiles,
-                                                             uint32_t n, uint32_t* __restrict__ cnt,
-                                                             const StepStatus* __restrict__ status) {
-  if (status && status->abort) return;
+constexpr uint32_t kStagedPairs = 1024;  // views whose work-row table is staged in LDS
+
+// Entry counts of the rows in depth order, and each row's record for keygen_ordered:
+// {first tile (row-major over the row's rectangle, src/rasterize.rs:86-99), rectangle
+// width | tile-grid stride << 16, first entry id, Gaussian}.  Rows without entries get no
+// record (no entry position resolves to them).
+template <bool BATCH>
+__global__ void __launch_bounds__(256) ordered_rows_kernel(
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ ntiles,
+    const uint2* __restrict__ rect, const uint32_t* __restrict__ entry_offset, uint32_t n,
+    uint32_t tiles_x, BatchDev bd, uint32_t* __restrict__ cnt, uint4* __restrict__ rows,
+    const StepStatus* __restrict__ status) {
+  __shared__ uint32_t s_wb[BATCH ? kStagedPairs + 1 : 1];  // the views' first work rows
+  if (status && status->abort) return;  // uniform
+  const bool staged = BATCH && bd.npairs <= kStagedPairs;
+  if (staged) {
+    for (uint32_t q = threadIdx.x; q <= bd.npairs; q += blockDim.x) s_wb[q] = bd.work_base[q];
+    __syncthreads();
+  }
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) cnt[i] = ntiles[order[i]];
+  if (i >= n) return;
+  const uint32_t g = order[i];
+  const uint32_t c = ntiles[g];
+  cnt[i] = c;
+  if (c == 0) return;
+  const uint2 rc = rect[g];
+  const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
+  uint32_t first, stride;
+  if (BATCH) {
+    const uint32_t v = find_segment(staged ? s_wb : bd.work_base, bd.npairs, g);
+    stride = bd.tiles_x[v];
+    first = bd.tile_base[v] + y0 * stride + x0;
+  } else {
+    stride = tiles_x;
+    first = y0 * stride + x0;
+  }
+  rows[i] = make_uint4(first, (x1 - x0) | (stride << 16), entry_offset[g], g);
+}
+
+// (x / w, x % w) for 0 < w < 2^16 and x < 2^22 (a rectangle of a view's tile grid, which
+// has fewer than 2^22 tiles): the estimate from the reciprocal (relative error below
+// 2^-22) is the quotient or one less, and one correction step makes it exact.
+__device__ __forceinline__ uint2 divmod_small(uint32_t x, uint32_t w) {
+  uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)w));
+  int32_t r = (int32_t)(x - q * w);
+  if (r < 0) {
+    q--;
+    r += (int32_t)w;
+  } else if (r >= (int32_t)w) {
+    q++;
+    r -= (int32_t)w;
+  }
+  return make_uint2(q, (uint32_t)r);
 }
 
 // Position j of the depth-ordered entry list: the row (soff, marks: its offsets in depth
-// order), the tile (row-major over the row's rectangle, src/rasterize.rs:86-99) and the
-// reference's entry id (the row's Gaussian-order offset + the local index).
-template <bool BATCH>
+// order), whose record gives the entry's tile (the local index row-major over the
+// rectangle) and the reference's entry id (the row's Gaussian-order offset + the local
+// index).
 __global__ void __launch_bounds__(256) keygen_ordered_kernel(
-    const uint32_t* __restrict__ soff, const uint32_t* __restrict__ order, uint32_t n,
-    uint32_t e_total, const uint32_t* __restrict__ entry_offset, const uint2* __restrict__ rect,
-    uint32_t tiles_x, BatchDev bd, const uint32_t* __restrict__ marks,
-    uint32_t* __restrict__ tkeys, uint32_t* __restrict__ tvals, uint32_t* __restrict__ tgauss,
+    const uint32_t* __restrict__ soff, const uint4* __restrict__ rows, uint32_t n,
+    uint32_t e_total, const uint32_t* __restrict__ marks, uint32_t* __restrict__ tkeys,
+    uint32_t* __restrict__ tvals, uint32_t* __restrict__ tgauss,
     uint32_t* __restrict__ entry_gaussian, const StepStatus* __restrict__ status) {
   __shared__ uint32_t s_off[kKeygenSpan + 1];
   if (status) {
@@ -372,24 +416,14 @@ __global__ void __launch_bounds__(256) keygen_ordered_kernel(
   const uint32_t j = eb + threadIdx.x;
   if (j >= e_total) return;
   const uint32_t i = keygen_find(sp, j, soff, s_off);
-  const uint32_t g = order[i];
-  const uint32_t local = j - soff[i];
-  const uint2 rc = rect[g];
-  const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
-  const uint32_t w = x1 - x0;
-  const uint32_t ty = y0 + local / w, tx = x0 + local % w;
-  uint32_t tile;
-  if (BATCH) {
-    const uint32_t v = find_segment(bd.work_base, bd.npairs, g);
-    tile = bd.tile_base[v] + ty * bd.tiles_x[v] + tx;
-  } else {
-    tile = ty * tiles_x + tx;
-  }
-  const uint32_t e = entry_offset[g] + local;
-  tkeys[j] = tile;
+  const uint4 row = rows[i];
+  const uint32_t local = j - (sp.staged ? s_off[i - sp.g0] : soff[i]);
+  const uint2 qr = divmod_small(local, row.y & 0xffffu);
+  const uint32_t e = row.z + local;
+  tkeys[j] = row.x + qr.x * (row.y >> 16) + qr.y;
   tvals[j] = e;
-  tgauss[j] = g;
-  if (entry_gaussian) entry_gaussian[e] = g;
+  tgauss[j] = row.w;
+  if (entry_gaussian) entry_gaussian[e] = row.w;
 }
 
 // The tile-sorted entries: sorted lists, (optional) keys and tile ranges.  Position j
@@ -566,6 +600,7 @@ RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_
     rs.eg[k] = (uint32_t*)store["radix_eg" + sk].reserve((capacity + 1) * 4);
   }
   rs.soff = (uint32_t*)store["radix_soff"].reserve((n + 1) * 4);
+  rs.rows = (uint4*)store["radix_rows"].reserve((n + 1) * 16);
   rs.counts = (uint32_t*)store["radix_counts"].reserve(radix_count_words(std::max(n, capacity)) * 4);
   return rs;
 }
@@ -576,6 +611,7 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
                             const BatchDev* bd, uint64_t* keys_out, uint32_t e_host,
                             bool offsets_done) {
   const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
+  SPLAT_REQUIRE(bd || T < kMaxViewTiles, "image too large for radix binning");
   if (!offsets_done) {
     fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
     exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64);
@@ -589,25 +625,26 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
   const int c = radix_sort_pairs(ctx, rs.rk[0], rs.rv[0], rs.rk[1], rs.rv[1], n, (uint32_t)n,
                                  nullptr, 32, rs.counts, scan_sums, status);
   const uint32_t* order = rs.rv[c];
-  // 2. the rows' entry offsets in depth order
-  launch(ctx, "bin_ordered_counts", ordered_counts_kernel, dim3(blocks_for(n, 256)), dim3(256),
-         0, order, (const uint32_t*)pb.ntiles, (uint32_t)n, rs.rk[1 - c],
-         (const StepStatus*)status);
+  // 2. the rows' entry offsets and records in depth order
+  if (bd)
+    launch(ctx, "bin_ordered_rows", ordered_rows_kernel<true>, dim3(blocks_for(n, 256)),
+           dim3(256), 0, order, (const uint32_t*)pb.ntiles, (const uint2*)pb.rect,
+           (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.tiles_x, *bd, rs.rk[1 - c], rs.rows,
+           (const StepStatus*)status);
+  else
+    launch(ctx, "bin_ordered_rows", ordered_rows_kernel<false>, dim3(blocks_for(n, 256)),
+           dim3(256), 0, order, (const uint32_t*)pb.ntiles, (const uint2*)pb.rect,
+           (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.tiles_x, BatchDev{}, rs.rk[1 - c],
+           rs.rows, (const StepStatus*)status);
   exclusive_scan_u32(ctx, rs.rk[1 - c], rs.soff, n, small + 4, scan_sums);
   launch(ctx, "bin_marks", block_marks_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
          (const uint32_t*)rs.soff, (uint32_t)n, bb.block_marks, (const StepStatus*)status);
   // 3. entries in depth order, keyed by tile
-  const unsigned kg = blocks_for(status ? capacity : e_host, kKeygenEntries);
-  if (bd)
-    launch(ctx, "bin_keygen", keygen_ordered_kernel<true>, dim3(kg), dim3(kKeygenThreads), 0,
-           (const uint32_t*)rs.soff, order, (uint32_t)n, e_host, (const uint32_t*)bb.entry_offset,
-           (const uint2*)pb.rect, bb.tiles_x, *bd, (const uint32_t*)bb.block_marks, rs.ek[0],
-           rs.ev[0], rs.eg[0], bb.entry_gaussian, (const StepStatus*)status);
-  else
-    launch(ctx, "bin_keygen", keygen_ordered_kernel<false>, dim3(kg), dim3(kKeygenThreads), 0,
-           (const uint32_t*)rs.soff, order, (uint32_t)n, e_host, (const uint32_t*)bb.entry_offset,
-           (const uint2*)pb.rect, bb.tiles_x, BatchDev{}, (const uint32_t*)bb.block_marks,
-           rs.ek[0], rs.ev[0], rs.eg[0], bb.entry_gaussian, (const StepStatus*)status);
+  launch(ctx, "bin_keygen", keygen_ordered_kernel,
+         dim3(blocks_for(status ? capacity : e_host, kKeygenEntries)), dim3(kKeygenThreads), 0,
+         (const uint32_t*)rs.soff, (const uint4*)rs.rows, (uint32_t)n, e_host,
+         (const uint32_t*)bb.block_marks, rs.ek[0], rs.ev[0], rs.eg[0], bb.entry_gaussian,
+         (const StepStatus*)status);
   // 4. stable sort by tile
   uint32_t tile_bits = 0;
   while (tile_bits < 32 && (T >> tile_bits) > 1) tile_bits++;  // T - 1 < 2^tile_bits

@@ -381,14 +381,18 @@ void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size
                const float* slots, const StatsArgs& sa, float c1, float c2,
                const splat_adam_state& st, const splat_learning_rates& lr,
                const StepDyn* dyn = nullptr, const StepStatus* status = nullptr);
-// Batched: projections of every (scene, view) pair (total `work` rows), and the summed
-// parameter gradients of every scene Gaussian (total `rows` plane rows).
+// Batched: projections of every (scene, view) pair (total `work` rows), and the backward
+// tail + Adam of every scene Gaussian (total `rows` plane rows, a multiple of 64): the
+// projection backward of each pair's summed slots (dproj, one per work row) summed over
+// the scene's views in step order, then Adam with the scene's position learning rate;
+// c1, c2, lr come from dyn.
 void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,
                            const BatchDev& bd, size_t work, float near_plane, int antialias,
                            int degree, ProjBuffers pb);
-void project_backward_batch(splat_ctx* ctx, const float* params, size_t stride,
-                            const BatchDev& bd, size_t rows, float near_plane, int antialias,
-                            int degree, const splat_proj_grad* dproj, float* grads);
+void grad_adam_batch(splat_ctx* ctx, float* params, float* m, float* v, size_t stride,
+                     const BatchDev& bd, size_t rows, float near_plane, int antialias, int degree,
+                     const uint32_t* entry_offset, const splat_proj_grad* dproj,
+                     const splat_adam_state& st, const StepDyn* dyn, const StepStatus* status);
 void planes_to_aos(splat_ctx* ctx, const float* planes, size_t n, size_t stride, float* aos);
 
 // scan.hip
@@ -426,9 +430,11 @@ struct RadixScratch {
   uint32_t *rk[2], *rv[2];  // rows (n)
   uint32_t *ek[2], *ev[2], *eg[2];  // entries (capacity): tile, entry id, Gaussian
   uint32_t* soff;           // n + 1
+  uint4* rows;              // n + 1 row records of the entry generator
   uint32_t* counts;         // radix_count_words(max(n, capacity))
 };
 RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_t capacity);
+constexpr size_t kMaxViewTiles = size_t(1) << 22;  // tiles of one view for radix binning
 // Tile binning by two stable radix sorts: the rows by depth, then their entries (emitted in
 // depth order, no per-tile counters) by tile.  Each Gaussian has at most one entry per
 // tile and entry ids grow with the Gaussian index, so the result is the (tile, depth,
@@ -521,7 +527,7 @@ void gaussian_kernel_host(float k[11]);
 void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, float* v,
                  size_t n, size_t stride, float c1, float c2, const splat_adam_state& st,
                  const splat_learning_rates& lr, const StepDyn* dyn = nullptr,
-                 const StepStatus* status = nullptr, const BatchDev* bd = nullptr);
+                 const StepStatus* status = nullptr);
 struct DensifyParams {
   float grad_threshold, size_limit, min_opacity, world_limit, shrink;
   int prune_big_world, has_max_screen;

@@ -14,8 +14,7 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ params,
                                                    float c2, float b1, float b2, float eps,
                                                    splat_learning_rates lr,
                                                    const StepDyn* __restrict__ dyn,
-                                                   const StepStatus* __restrict__ status,
-                                                   BatchDev bd) {
+                                                   const StepStatus* __restrict__ status) {
   const int j = blockIdx.y;
   const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (i0 >= n) return;
@@ -25,9 +24,6 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ params,
     c2 = dyn->c2;
     lr = dyn->lr;
   }
-  // batched scenes: each scene's own position learning rate (scene rows start at multiples
-  // of 4, so the four rows of a lane share a scene)
-  if (bd.lr_mean && j < 3) lr.mean = bd.lr_mean[find_segment(bd.scene_off, bd.nscenes, i0)];
   const float rate = plane_lr(j, lr);
   const size_t base = (size_t)j * stride + i0;
   if (i0 + 4 <= n) {
@@ -266,11 +262,10 @@ __global__ void reset_opacity_kernel(float* __restrict__ params, float* __restri
 
 void adam_planes(splat_ctx* ctx, float* params, const float* grads, float* m, float* v, size_t n,
                  size_t stride, float c1, float c2, const splat_adam_state& st,
-                 const splat_learning_rates& lr, const StepDyn* dyn, const StepStatus* status,
-                 const BatchDev* bd) {
+                 const splat_learning_rates& lr, const StepDyn* dyn, const StepStatus* status) {
   launch(ctx, "adam", adam_kernel, dim3(blocks_for((n + 3) / 4, 256), kParams), dim3(256), 0,
          params, grads, m, v, (uint32_t)n, (uint32_t)stride, c1, c2, st.beta1, st.beta2, st.eps,
-         lr, dyn, status, bd ? *bd : BatchDev{});
+         lr, dyn, status);
 }
 
 void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius, const splat_proj_grad* grads,

@@ -3,6 +3,8 @@
 // One lane per Gaussian; the 59 parameter planes are read with unit stride across a wave.
 // The arithmetic restates src/project.rs:137-474 in the reference's evaluation order
 // (compiled without FMA contraction) so the screen records match it bit for bit.
+#include <cstring>
+
 #include "internal.hpp"
 
 namespace splat {
@@ -433,14 +435,24 @@ __global__ void proj_unpack_kernel(const splat_proj* __restrict__ in, uint32_t n
 }
 
 // project_backward_one, src/project.rs:335-474.
-// Gradient of one Gaussian (parameters g) for one view into out[j * ostride] (zeroed here).
+// Gradient of one Gaussian (parameters g) for one view into out[j * ostride]: zeroed and
+// written, or with ACC added to what out holds (each plane receives at most one term).
+template <bool ACC = false>
 __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev& cam,
                                                 float near_plane, float eps2d, int antialias,
                                                 int degree,
                                                 const splat_proj_grad* __restrict__ dproj_row,
                                                 float* out, int ostride = 1) {
+  const auto put = [&](int j, float x) {
+    if (ACC)
+      out[j * ostride] += x;
+    else
+      out[j * ostride] = x;
+  };
+  if (!ACC) {
 #pragma unroll
-  for (int j = 0; j < kParams; j++) out[(j) * ostride] = 0.0f;
+    for (int j = 0; j < kParams; j++) out[j * ostride] = 0.0f;
+  }
   Core k;
   if (core(g, cam, near_plane, eps2d, antialias != 0, degree, k)) {
     const splat_proj_grad dp = *dproj_row;
@@ -455,7 +467,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
           if (kk < nco) {
-            out[(11 + 3 * kk + ch) * ostride] = dc * k.basis[kk];
+            put(11 + 3 * kk + ch, dc * k.basis[kk]);
             const float w = dc * g[11 + 3 * kk + ch];
             ddir.x += bgrad[kk].x * w;
             ddir.y += bgrad[kk].y * w;
@@ -471,7 +483,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
 
     const float do_raw = dp.opacity * k.comp;
     const float dcomp = dp.opacity * k.o_raw;
-    out[(10) * ostride] = do_raw * k.o_raw * (1.0f - k.o_raw);
+    put(10, do_raw * k.o_raw * (1.0f - k.o_raw));
 
     const float a = k.a, b = k.b, c = k.c, det = k.det;
     const float da_ = dp.conic[0], db_ = dp.conic[1], dc_ = dp.conic[2];
@@ -532,9 +544,9 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
     dtz += dp.mean2d[1] * (-fy * k.t.y * rz2);
     const V3 dtv{dtx, dty, dtz};
     // rot^T * dt (rows of the transpose are the columns of rot)
-    out[(0) * ostride] = dot3(V3{cam.rot[0], cam.rot[3], cam.rot[6]}, dtv) + dmean_color.x;
-    out[(1) * ostride] = dot3(V3{cam.rot[1], cam.rot[4], cam.rot[7]}, dtv) + dmean_color.y;
-    out[(2) * ostride] = dot3(V3{cam.rot[2], cam.rot[5], cam.rot[8]}, dtv) + dmean_color.z;
+    put(0, dot3(V3{cam.rot[0], cam.rot[3], cam.rot[6]}, dtv) + dmean_color.x);
+    put(1, dot3(V3{cam.rot[1], cam.rot[4], cam.rot[7]}, dtv) + dmean_color.y);
+    put(2, dot3(V3{cam.rot[2], cam.rot[5], cam.rot[8]}, dtv) + dmean_color.z);
 
     M3 dm;
     for (int ii = 0; ii < 3; ii++)
@@ -551,7 +563,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
         ds[col] += dm.m[r][col] * k.rq.m[r][col];
         dr.m[r][col] = dm.m[r][col] * sv[col];
       }
-    for (int ii = 0; ii < 3; ii++) out[(3 + ii) * ostride] = ds[ii] * sv[ii];
+    for (int ii = 0; ii < 3; ii++) put(3 + ii, ds[ii] * sv[ii]);
     // quat_to_mat_backward, src/math.rs:336-372.
     const float w = k.q[0], x = k.q[1], y = k.q[2], z = k.q[3];
     const float Dw[3][3] = {{0.0f, -2.0f * z, 2.0f * y},
@@ -580,7 +592,7 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
     if (nn > 0.0f) {
       const float u[4] = {raw[0] / nn, raw[1] / nn, raw[2] / nn, raw[3] / nn};
       const float dot = u[0] * dq[0] + u[1] * dq[1] + u[2] * dq[2] + u[3] * dq[3];
-      for (int t = 0; t < 4; t++) out[(6 + t) * ostride] = (dq[t] - u[t] * dot) / nn;
+      for (int t = 0; t < 4; t++) put(6 + t, (dq[t] - u[t] * dot) / nn);
     }
   }
 }
@@ -718,40 +730,78 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
   }
 }
 
-// Batched: each scene Gaussian sums its per-view gradients over the scene's pairs in step
-// order (the first view's gradient is stored, later ones are added to it).  Padding rows
-// between scenes get zero gradients.
-__global__ void __launch_bounds__(256) project_bwd_batch_kernel(const float* __restrict__ params,
-                                                                uint32_t stride, BatchDev bd,
-                                                                uint32_t rows, float near_plane,
-                                                                float eps2d, int antialias,
-                                                                int degree,
-                                                                const splat_proj_grad* __restrict__ dproj,
-                                                                float* __restrict__ grads) {
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= rows) return;
-  const uint32_t s = find_segment(bd.scene_off, bd.nscenes, p);
-  const uint32_t i = p - bd.scene_off[s];
-  const uint32_t k0 = bd.scene_pair_begin[s], k1 = bd.scene_pair_begin[s + 1];
-  if (i >= bd.scene_n[s] || k0 == k1) {
-#pragma unroll
-    for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + p] = 0.0f;
-    return;
+// Batched backward tail and optimizer, one wave of 64 scene rows per workgroup (scene rows
+// start at multiples of 64, so the workgroup's rows share a scene).  For each of the
+// scene's views in step order, project_backward_one (src/project.rs:335-474) adds the
+// view's parameter gradients (from the row's summed slots, dproj) to accumulators in LDS
+// ([plane][row]).  Adam (src/adam.rs:44-61, the scene's own position learning rate) then
+// runs over the workgroup's 59 x 64 values as float4 groups: no gradient planes in HBM.
+// A view none of whose 64 rows has an entry adds nothing and is skipped: an all-zero slot
+// sum gives zero gradients, and Adam's moments and step do not depend on the sign of a
+// zero gradient.
+constexpr uint32_t kBatchRows = 64;  // scene rows per grad_adam_batch workgroup
+
+__global__ void __launch_bounds__(kBatchRows) grad_adam_batch_kernel(
+    float* __restrict__ params, float* __restrict__ m, float* __restrict__ v, uint32_t stride,
+    BatchDev bd, float near_plane, float eps2d, int antialias, int degree,
+    const uint32_t* __restrict__ entry_offset, const splat_proj_grad* __restrict__ dproj,
+    float c1, float c2, float b1, float b2, float eps, splat_learning_rates lr,
+    const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
+  constexpr uint32_t R = kBatchRows;
+  __shared__ __align__(16) float acc[kParams * R];  // gradient sums, [plane][row]
+  if (status && status->abort) return;  // uniform
+  if (dyn) {
+    c1 = dyn->c1;
+    c2 = dyn->c2;
+    lr = dyn->lr;
   }
+  const uint32_t ib = blockIdx.x * R, t = threadIdx.x;
+  const uint32_t s = find_segment(bd.scene_off, bd.nscenes, ib);
+  const uint32_t i0 = ib - bd.scene_off[s], ns = bd.scene_n[s];
+  const uint32_t rows = i0 < ns ? min(R, ns - i0) : 0u;  // padding rows are left alone
+  if (rows == 0) return;  // uniform
+  const bool live = t < rows;
   float g[kParams];
 #pragma unroll
-  for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + p];
+  for (int j = 0; j < kParams; j++) {
+    g[j] = live ? params[(size_t)j * stride + ib + t] : 0.0f;
+    acc[j * R + t] = 0.0f;
+  }
+  const uint32_t k0 = bd.scene_pair_begin[s], k1 = bd.scene_pair_begin[s + 1];
   for (uint32_t k = k0; k < k1; k++) {
-    const uint32_t v = bd.scene_pairs[k];
-    float out[kParams];
-    project_bwd_row(g, bd.cams[v], near_plane, eps2d, antialias, degree,
-                    dproj + bd.work_base[v] + i, out);
-    if (k == k0) {
-#pragma unroll
-      for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + p] = out[j];
+    const uint32_t view = bd.scene_pairs[k];
+    const uint32_t w0 = bd.work_base[view] + i0;
+    if (entry_offset[w0] == entry_offset[w0 + rows]) continue;  // uniform: view sees no row
+    splat_proj_grad pg{};
+    if (live) pg = dproj[w0 + t];
+    project_bwd_row<true>(g, bd.cams[view], near_plane, eps2d, antialias, degree, &pg, acc + t, R);
+  }
+  __syncthreads();
+  lr.mean = bd.lr_mean[s];
+  constexpr uint32_t kPerPlane = R / 4;  // float4 groups per plane
+  constexpr uint32_t kGroups = kParams * kPerPlane;
+#pragma unroll 2
+  for (uint32_t q = t; q < kGroups; q += R) {
+    const uint32_t j = q / kPerPlane, r = (q % kPerPlane) * 4u;
+    if (r >= rows) continue;
+    const float rate = plane_lr((int)j, lr);
+    const size_t at = (size_t)j * stride + ib + r;
+    const float* gq = acc + j * R + r;
+    if (r + 4 <= rows) {
+      float4 P = load_stream(params + at);
+      float4 M = load_stream(m + at);
+      float4 V = load_stream(v + at);
+      const float4 G = *reinterpret_cast<const float4*>(gq);
+      adam_update(P.x, G.x, M.x, V.x, rate, b1, b2, c1, c2, eps);
+      adam_update(P.y, G.y, M.y, V.y, rate, b1, b2, c1, c2, eps);
+      adam_update(P.z, G.z, M.z, V.z, rate, b1, b2, c1, c2, eps);
+      adam_update(P.w, G.w, M.w, V.w, rate, b1, b2, c1, c2, eps);
+      store_stream(params + at, P);
+      store_stream(m + at, M);
+      store_stream(v + at, V);
     } else {
-#pragma unroll
-      for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + p] += out[j];
+      for (uint32_t kk = 0; r + kk < rows; kk++)
+        adam_update(params[at + kk], gq[kk], m[at + kk], v[at + kk], rate, b1, b2, c1, c2, eps);
     }
   }
 }
@@ -836,12 +886,16 @@ void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,
          params, (uint32_t)stride, bd, (uint32_t)work, near_plane, 0.3f, antialias, degree, pb);
 }
 
-void project_backward_batch(splat_ctx* ctx, const float* params, size_t stride,
-                            const BatchDev& bd, size_t rows, float near_plane, int antialias,
-                            int degree, const splat_proj_grad* dproj, float* grads) {
-  launch(ctx, "project_bwd", project_bwd_batch_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0,
-         params, (uint32_t)stride, bd, (uint32_t)rows, near_plane, 0.3f, antialias, degree, dproj,
-         grads);
+void grad_adam_batch(splat_ctx* ctx, float* params, float* m, float* v, size_t stride,
+                     const BatchDev& bd, size_t rows, float near_plane, int antialias, int degree,
+                     const uint32_t* entry_offset, const splat_proj_grad* dproj,
+                     const splat_adam_state& st, const StepDyn* dyn, const StepStatus* status) {
+  SPLAT_REQUIRE(rows % kBatchRows == 0, "batched rows must be a multiple of 64");
+  splat_learning_rates lr0;
+  std::memset(&lr0, 0, sizeof(lr0));
+  launch(ctx, "grad_adam", grad_adam_batch_kernel, dim3((unsigned)(rows / kBatchRows)),
+         dim3(kBatchRows), 0, params, m, v, (uint32_t)stride, bd, near_plane, 0.3f, antialias,
+         degree, entry_offset, dproj, 1.0f, 1.0f, st.beta1, st.beta2, st.eps, lr0, dyn, status);
 }
 
 void aos_to_planes(splat_ctx* ctx, const float* aos, size_t n, float* planes, size_t stride) {

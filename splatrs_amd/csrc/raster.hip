@@ -565,6 +565,8 @@ __global__ void __launch_bounds__(256) reduce_slots_kernel(const uint32_t* __res
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   if (status && status->abort) return;
+  // slots 9 and 10 (the absolute sums) are read only when abs_grads is given
+  const bool abs = abs_grads != nullptr;
   float sum[kSlots];
 #pragma unroll
   for (int k = 0; k < kSlots; k++) sum[k] = 0.0f;
@@ -572,7 +574,11 @@ __global__ void __launch_bounds__(256) reduce_slots_kernel(const uint32_t* __res
   for (uint32_t entry = s; entry < e; entry++) {
     const float* p = slots + (size_t)entry * kSlots;
 #pragma unroll
-    for (int k = 0; k < kSlots; k++) sum[k] += p[k];
+    for (int k = 0; k < 9; k++) sum[k] += p[k];
+    if (abs) {
+      sum[9] += p[9];
+      sum[10] += p[10];
+    }
   }
   splat_proj_grad pg;
   pg.mean2d[0] = sum[0];
