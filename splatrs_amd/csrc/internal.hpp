@@ -403,6 +403,7 @@ void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_
                         uint32_t* total_dev, uint32_t* sums = nullptr,
                         unsigned long long* total64 = nullptr);
 size_t scan_temp_bytes(size_t n);
+size_t scan_max_items();  // largest n exclusive_scan_u32 takes
 // Fill kernel (used instead of hipMemset so the same code can sit inside a HIP graph).
 void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v);
 

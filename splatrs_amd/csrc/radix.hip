@@ -1,5 +1,5 @@
-// Stable LSD radix sort of (key, value[, value2]) tuples of 32-bit words, 8 key bits per
-// pass.
+// Stable LSD radix sort of (key, value[, value2]) tuples of 32-bit words, up to 9 key bits
+// per pass (the fewest passes for the key width, digits as even as possible).
 //
 // Each pass: an upsweep writes every block's digit histogram, digit-major
 // (counts[digit * nblocks + block]); an exclusive scan of that array gives every
@@ -23,7 +23,8 @@ namespace {
 constexpr uint32_t kRadixThreads = 256;
 constexpr uint32_t kRadixRounds = 16;
 constexpr uint32_t kRadixItems = kRadixThreads * kRadixRounds;  // items per block
-constexpr uint32_t kRadixBins = 256;
+constexpr uint32_t kRadixBins = 512;  // digits of the widest pass (9 bits)
+constexpr uint32_t kRadixMaxBits = 9;
 constexpr uint32_t kRadixWaves = kRadixThreads / 64;
 
 __device__ __forceinline__ uint32_t item_count(uint32_t n, const uint32_t* n_dev) {
@@ -48,7 +49,7 @@ __global__ void __launch_bounds__(kRadixThreads) radix_upsweep_kernel(
   __shared__ uint32_t hist[kRadixBins];
   if (status && status->abort) return;  // uniform
   n = item_count(n, n_dev);
-  hist[threadIdx.x] = 0;
+  for (uint32_t d = threadIdx.x; d < kRadixBins; d += kRadixThreads) hist[d] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * kRadixItems;
   for (uint32_t r = 0; r < kRadixRounds; r++) {
@@ -56,7 +57,8 @@ __global__ void __launch_bounds__(kRadixThreads) radix_upsweep_kernel(
     if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
   }
   __syncthreads();
-  counts[threadIdx.x * nblocks + blockIdx.x] = hist[threadIdx.x];
+  for (uint32_t d = threadIdx.x; d <= mask; d += kRadixThreads)
+    counts[d * nblocks + blockIdx.x] = hist[d];
 }
 
 template <bool TWO>
@@ -91,23 +93,39 @@ __global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
     if (TWO) v2[r] = valid ? vals2_in[i] : 0u;
   });
   // this block's digit counts (consecutive words of the scanned histogram) and their
-  // block-local exclusive scan
-  const uint32_t g0 = offsets[t * nblocks + blockIdx.x];
-  const uint32_t c = offsets[t * nblocks + blockIdx.x + 1] - g0;
-  uint32_t x = c;
+  // block-local exclusive scan; thread t holds digits 2t and 2t + 1
+  const uint32_t bins = mask + 1;
+  uint32_t g0[2], c[2];
+#pragma unroll
+  for (uint32_t h = 0; h < 2; h++) {
+    const uint32_t d = 2 * t + h;
+    g0[h] = d < bins ? offsets[d * nblocks + blockIdx.x] : 0u;
+    c[h] = d < bins ? offsets[d * nblocks + blockIdx.x + 1] - g0[h] : 0u;
+  }
+  const uint32_t pair = c[0] + c[1];
+  uint32_t x = pair;
   for (uint32_t o = 1; o < 64; o <<= 1) {
     const uint32_t y = __shfl_up(x, o, 64);
     if (lane >= o) x += y;
   }
   if (lane == 63) wsum[wave] = x;
+  for (uint32_t d = t; d < kRadixBins; d += kRadixThreads) {
 #pragma unroll
-  for (uint32_t w = 0; w < kRadixWaves; w++) wcnt[w][t] = 0;
+    for (uint32_t w = 0; w < kRadixWaves; w++) wcnt[w][d] = 0;
+  }
   __syncthreads();
-  uint32_t lb = x - c;
+  uint32_t lb = x - pair;
   for (uint32_t w = 0; w < wave; w++) lb += wsum[w];
-  gbase[t] = g0;
-  lbase[t] = lb;
-  run[t] = lb;
+#pragma unroll
+  for (uint32_t h = 0; h < 2; h++) {
+    const uint32_t d = 2 * t + h;
+    if (d < bins) {
+      gbase[d] = g0[h];
+      lbase[d] = lb;
+      run[d] = lb;
+    }
+    lb += c[h];
+  }
   __syncthreads();
   for_rounds([&](auto step) {
     constexpr int r = decltype(step)::value;
@@ -133,13 +151,15 @@ __global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
       if (TWO) s_val2[lp] = v2[r];
     }
     __syncthreads();
-    uint32_t add = 0;
+    for (uint32_t d = t; d < bins; d += kRadixThreads) {
+      uint32_t add = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kRadixWaves; w++) {
-      add += wcnt[w][t];
-      wcnt[w][t] = 0;
+      for (uint32_t w = 0; w < kRadixWaves; w++) {
+        add += wcnt[w][d];
+        wcnt[w][d] = 0;
+      }
+      run[d] += add;
     }
-    run[t] += add;
     __syncthreads();
   });
   // block-local order -> output, run by run
@@ -169,13 +189,18 @@ int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, u
   uint32_t* vals[2] = {v0, v1};
   uint32_t* vals2[2] = {w0, w1};
   const bool two = w0 != nullptr;
-  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-    const uint32_t nbits = std::min(8u, key_bits - shift);
+  // 9-bit digits while their (digit, block) counts fit one scan, else 8
+  const uint32_t max_bits =
+      ((size_t)kRadixBins * nblocks + 1 <= scan_max_items()) ? kRadixMaxBits : 8u;
+  const uint32_t passes = (key_bits + max_bits - 1) / max_bits;
+  const uint32_t width = passes ? (key_bits + passes - 1) / passes : 0u;
+  for (uint32_t shift = 0; shift < key_bits; shift += width) {
+    const uint32_t nbits = std::min(width, key_bits - shift);
     launch(ctx, "radix_upsweep", radix_upsweep_kernel, dim3(nblocks), dim3(kRadixThreads), 0,
            (const uint32_t*)keys[cur], n_host, n_dev, shift, (1u << nbits) - 1u, nblocks, counts,
            status);
     // in place; the total lands in the array's last word
-    const size_t len = (size_t)kRadixBins * nblocks;
+    const size_t len = ((size_t)1 << nbits) * nblocks;
     exclusive_scan_u32(ctx, counts, counts, len, counts + len, scan_sums);
     if (two)
       launch(ctx, "radix_downsweep", radix_downsweep_kernel<true>, dim3(nblocks),

@@ -6,8 +6,8 @@ namespace splat {
 namespace {
 
 constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;  // 2048 items per block
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 4096 items per block
 constexpr int kTopThreads = 1024;
 constexpr int kTopItems = 16;  // one workgroup scans up to 16384 block sums
 
@@ -168,6 +168,8 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
 void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
   launch(ctx, "fill", fill_u32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, p, n, v);
 }
+
+size_t scan_max_items() { return (size_t)kTopThreads * kTopItems * kScanTile; }
 
 size_t scan_temp_bytes(size_t n) {
   return (blocks_for(n == 0 ? 1 : n, kScanTile) + 1) * sizeof(uint32_t);

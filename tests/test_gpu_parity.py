@@ -322,7 +322,10 @@ def radix_cases():
             ("20000 entries per tile", crowd, crowd_cam, rec.proj_opts(0)),
             ("10k Gaussians 256x256", scenes.small_scene(0), scenes.small_scene_cameras()[3],
              rec.proj_opts(3)),
-            ("1M Gaussians 1920x1080", scenes.large_scene(0), scenes.hd_camera(), rec.proj_opts(3))]
+            ("1M Gaussians 1920x1080", scenes.large_scene(0), scenes.hd_camera(), rec.proj_opts(3)),
+            # 275 x 275 = 75625 tiles: 17-bit tile keys, sorted in two 9- and 8-bit passes
+            ("100k Gaussians 4400x4400", scenes.small_scene(2, 100_000),
+             scenes.small_scene_cameras(size=4400, f=5200.0)[3], rec.proj_opts(1))]
 
 
 def test_radix_binning_equals_oracle(ctx):
