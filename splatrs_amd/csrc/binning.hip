@@ -426,9 +426,10 @@ __global__ void __launch_bounds__(256) keygen_ordered_kernel(
   if (entry_gaussian) entry_gaussian[e] = row.w;
 }
 
-// The tile-sorted entries: sorted lists, (optional) keys and tile ranges.  Position j
-// starts every tile in (tile[j - 1], tile[j]]; the last position also ends the tiles after
-// its own, so each tile_start word is written exactly once when there are entries.
+// The tile-sorted entries: tile ranges, (optional) keys and, when tvals is given, the
+// sorted lists (otherwise the sort wrote them in place).  Position j starts every tile in
+// (tile[j - 1], tile[j]]; the last position also ends the tiles after its own, so each
+// tile_start word is written exactly once when there are entries.
 __global__ void __launch_bounds__(256) radix_finalize_kernel(
     const uint32_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals,
     const uint32_t* __restrict__ tgauss, uint32_t e_total, uint32_t num_tiles,
@@ -441,10 +442,15 @@ __global__ void __launch_bounds__(256) radix_finalize_kernel(
   }
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= e_total) return;
-  const uint32_t t = tkeys[j], e = tvals[j], g = tgauss[j];
-  sorted_entry[j] = e;
-  sorted_gauss[j] = g;
-  if (keys_out) keys_out[j] = ((uint64_t)t << 32) | f2bits(depth[g]);
+  const uint32_t t = tkeys[j];
+  if (tvals) {
+    sorted_entry[j] = tvals[j];
+    sorted_gauss[j] = tgauss[j];
+  }
+  if (keys_out) {
+    const uint32_t g = tvals ? tgauss[j] : sorted_gauss[j];
+    keys_out[j] = ((uint64_t)t << 32) | f2bits(depth[g]);
+  }
   for (uint32_t u = j == 0 ? 0u : tkeys[j - 1] + 1u; u <= t; u++) tile_start[u] = j;
   if (j + 1 == e_total)
     for (uint32_t u = t + 1; u <= num_tiles; u++) tile_start[u] = e_total;
@@ -649,14 +655,19 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
   uint32_t tile_bits = 0;
   while (tile_bits < 32 && (T >> tile_bits) > 1) tile_bits++;  // T - 1 < 2^tile_bits
   if (((size_t)1 << tile_bits) < T) tile_bits++;
+  // (the last pass writes the lists in place; a single tile needs no pass)
+  const bool direct = tile_bits > 0;
   const int c2 = radix_sort_pairs(ctx, rs.ek[0], rs.ev[0], rs.ek[1], rs.ev[1],
                                   status ? capacity : e_host, e_host, n_entries, tile_bits,
-                                  rs.counts, scan_sums, status, rs.eg[0], rs.eg[1]);
+                                  rs.counts, scan_sums, status, rs.eg[0], rs.eg[1],
+                                  direct ? bb.sorted_entry : nullptr,
+                                  direct ? bb.sorted_gauss : nullptr);
   // 5. lists and tile ranges
   fill_u32(ctx, bb.tile_start, T + 1, 0u);
   launch(ctx, "bin_finalize", radix_finalize_kernel,
          dim3(blocks_for(status ? capacity : e_host, 256)), dim3(256), 0,
-         (const uint32_t*)rs.ek[c2], (const uint32_t*)rs.ev[c2], (const uint32_t*)rs.eg[c2],
+         (const uint32_t*)rs.ek[c2], direct ? nullptr : (const uint32_t*)rs.ev[c2],
+         direct ? nullptr : (const uint32_t*)rs.eg[c2],
          e_host, (uint32_t)T, (const float*)pb.depth, bb.sorted_entry,
          bb.sorted_gauss, bb.tile_start, keys_out, (const StepStatus*)status);
 }

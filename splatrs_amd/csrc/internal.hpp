@@ -450,16 +450,19 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
                             uint32_t e_host = 0, bool offsets_done = false);
 
 // radix.hip
-// Stable LSD radix sort of (key, value) u32 pairs by key bits [0, key_bits), 8 bits per
-// pass, ping-ponging between (k0, v0) and (k1, v1) (and a second value between w0 and w1
-// when given); returns the index (0 or 1) of the buffers that hold the result.  The item
-// count is *n_dev when n_dev is given (grids cover `capacity`), else n_host.  counts:
-// radix_count_words(capacity) words; scan_sums: scan_temp_bytes(radix_count_words(capacity)).
+// Stable LSD radix sort of (key, value) u32 pairs by key bits [0, key_bits), up to 9 bits
+// per pass, ping-ponging between (k0, v0) and (k1, v1) (and a second value between w0 and
+// w1 when given); returns the index (0 or 1) of the buffers that hold the sorted keys.
+// The values end there too, unless v_final (w_final) is given: the last pass writes them
+// there (key_bits > 0).  The item count is *n_dev when n_dev is given (grids cover
+// `capacity`), else n_host.  counts: radix_count_words(capacity) words; scan_sums:
+// scan_temp_bytes(radix_count_words(capacity)).
 size_t radix_count_words(size_t capacity);
 int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                      size_t capacity, uint32_t n_host, const uint32_t* n_dev, uint32_t key_bits,
                      uint32_t* counts, uint32_t* scan_sums, const StepStatus* status,
-                     uint32_t* w0 = nullptr, uint32_t* w1 = nullptr);
+                     uint32_t* w0 = nullptr, uint32_t* w1 = nullptr,
+                     uint32_t* v_final = nullptr, uint32_t* w_final = nullptr);
 
 // raster.hip
 // Launch shape of the compositing kernels (results do not depend on it): entries per LDS

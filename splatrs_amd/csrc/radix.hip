@@ -182,7 +182,7 @@ size_t radix_count_words(size_t capacity) {
 int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                      size_t capacity, uint32_t n_host, const uint32_t* n_dev, uint32_t key_bits,
                      uint32_t* counts, uint32_t* scan_sums, const StepStatus* status,
-                     uint32_t* w0, uint32_t* w1) {
+                     uint32_t* w0, uint32_t* w1, uint32_t* v_final, uint32_t* w_final) {
   const uint32_t nblocks = blocks_for(capacity == 0 ? 1 : capacity, kRadixItems);
   int cur = 0;
   uint32_t* keys[2] = {k0, k1};
@@ -196,6 +196,9 @@ int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, u
   const uint32_t width = passes ? (key_bits + passes - 1) / passes : 0u;
   for (uint32_t shift = 0; shift < key_bits; shift += width) {
     const uint32_t nbits = std::min(width, key_bits - shift);
+    const bool last = shift + nbits >= key_bits;
+    uint32_t* vo = last && v_final ? v_final : vals[1 - cur];
+    uint32_t* wo = last && w_final ? w_final : vals2[1 - cur];
     launch(ctx, "radix_upsweep", radix_upsweep_kernel, dim3(nblocks), dim3(kRadixThreads), 0,
            (const uint32_t*)keys[cur], n_host, n_dev, shift, (1u << nbits) - 1u, nblocks, counts,
            status);
@@ -206,12 +209,12 @@ int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, u
       launch(ctx, "radix_downsweep", radix_downsweep_kernel<true>, dim3(nblocks),
              dim3(kRadixThreads), 0, (const uint32_t*)keys[cur], (const uint32_t*)vals[cur],
              (const uint32_t*)vals2[cur], n_host, n_dev, shift, nbits, nblocks,
-             (const uint32_t*)counts, keys[1 - cur], vals[1 - cur], vals2[1 - cur], status);
+             (const uint32_t*)counts, keys[1 - cur], vo, wo, status);
     else
       launch(ctx, "radix_downsweep", radix_downsweep_kernel<false>, dim3(nblocks),
              dim3(kRadixThreads), 0, (const uint32_t*)keys[cur], (const uint32_t*)vals[cur],
              (const uint32_t*)nullptr, n_host, n_dev, shift, nbits, nblocks,
-             (const uint32_t*)counts, keys[1 - cur], vals[1 - cur], (uint32_t*)nullptr, status);
+             (const uint32_t*)counts, keys[1 - cur], vo, (uint32_t*)nullptr, status);
     cur = 1 - cur;
   }
   return cur;

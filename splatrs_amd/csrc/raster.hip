@@ -555,46 +555,57 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   }
 }
 
-// Sum each Gaussian's slots over its entries in entry order (src/rasterize.rs:480-503).
-__global__ void __launch_bounds__(256) reduce_slots_kernel(const uint32_t* __restrict__ entry_offset,
-                                                           const float* __restrict__ slots,
-                                                           uint32_t n,
-                                                           splat_proj_grad* __restrict__ grads,
-                                                           float* __restrict__ abs_grads,
-                                                           const StepStatus* __restrict__ status) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  if (status && status->abort) return;
-  // slots 9 and 10 (the absolute sums) are read only when abs_grads is given
-  const bool abs = abs_grads != nullptr;
-  float sum[kSlots];
+// Sum each row's slots over its entries in entry order (src/rasterize.rs:480-503).  One
+// wave per 64 / NS consecutive rows, one lane per (row, slot): the rows' entries form one
+// span, staged through LDS in windows with coalesced loads, and each lane adds its slot of
+// its row's entries in order, so every sum keeps the reference's sequential order.  NS is
+// 9, or 11 with the absolute sums (slots 9 and 10, into abs_grads).
+constexpr uint32_t kReduceWindow = 256;  // entries staged per window
+
+template <int NS>
+__global__ void __launch_bounds__(64) reduce_slots_kernel(const uint32_t* __restrict__ entry_offset,
+                                                          const float* __restrict__ slots,
+                                                          uint32_t n,
+                                                          splat_proj_grad* __restrict__ grads,
+                                                          float* __restrict__ abs_grads,
+                                                          const StepStatus* __restrict__ status) {
+  constexpr uint32_t kRows = 64 / NS;
+  __shared__ float sh[kReduceWindow * kSlots];
+  if (status && status->abort) return;  // uniform
+  const uint32_t lane = threadIdx.x;
+  const uint32_t r0 = blockIdx.x * kRows, rl = lane / NS, k = lane % NS;
+  const uint32_t row = r0 + rl;
+  const bool live = rl < kRows && row < n;
+  const uint32_t span0 = entry_offset[r0], span1 = entry_offset[min(r0 + kRows, n)];
+  const uint32_t e0 = live ? entry_offset[row] : 0u, e1 = live ? entry_offset[row + 1] : 0u;
+  float sum = 0.0f;
+  for (uint32_t c0 = span0; c0 < span1; c0 += kReduceWindow) {
+    const uint32_t words = (min(c0 + kReduceWindow, span1) - c0) * kSlots;
+    const float* src = slots + (size_t)c0 * kSlots;
+    __syncthreads();
+    for (uint32_t q0 = 0; q0 < words; q0 += 64 * 8) {
+      float t[8];
 #pragma unroll
-  for (int k = 0; k < kSlots; k++) sum[k] = 0.0f;
-  const uint32_t s = entry_offset[g], e = entry_offset[g + 1];
-  for (uint32_t entry = s; entry < e; entry++) {
-    const float* p = slots + (size_t)entry * kSlots;
+      for (uint32_t u = 0; u < 8; u++) {
+        const uint32_t q = q0 + u * 64 + lane;
+        t[u] = q < words ? src[q] : 0.0f;
+      }
 #pragma unroll
-    for (int k = 0; k < 9; k++) sum[k] += p[k];
-    if (abs) {
-      sum[9] += p[9];
-      sum[10] += p[10];
+      for (uint32_t u = 0; u < 8; u++) {
+        const uint32_t q = q0 + u * 64 + lane;
+        if (q < words) sh[q] = t[u];
+      }
     }
+    __syncthreads();
+    const uint32_t hi = min(e1, c0 + kReduceWindow);
+    for (uint32_t e = max(e0, c0); e < hi; e++) sum += sh[(e - c0) * kSlots + k];
   }
-  splat_proj_grad pg;
-  pg.mean2d[0] = sum[0];
-  pg.mean2d[1] = sum[1];
-  pg.conic[0] = sum[2];
-  pg.conic[1] = sum[3];
-  pg.conic[2] = sum[4];
-  pg.opacity = sum[5];
-  pg.color[0] = sum[6];
-  pg.color[1] = sum[7];
-  pg.color[2] = sum[8];
-  grads[g] = pg;
-  if (abs_grads) {
-    abs_grads[2 * (size_t)g] = sum[9];
-    abs_grads[2 * (size_t)g + 1] = sum[10];
-  }
+  if (!live) return;
+  // splat_proj_grad holds slots 0-8 in slot order
+  if (k < 9)
+    reinterpret_cast<float*>(grads + row)[k] = sum;
+  else
+    abs_grads[2 * (size_t)row + (k - 9)] = sum;
 }
 
 }  // namespace
@@ -818,8 +829,13 @@ void fill_tune_result(const RasterTune& t, splat_tune_result* out) {
 
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
                   splat_proj_grad* grads, float* abs_grads, const StepStatus* status) {
-  launch(ctx, "reduce_slots", reduce_slots_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         entry_offset, slots, (uint32_t)n, grads, abs_grads, status);
+  static_assert(sizeof(splat_proj_grad) == 9 * sizeof(float), "slot order of splat_proj_grad");
+  if (abs_grads)
+    launch(ctx, "reduce_slots", reduce_slots_kernel<11>, dim3(blocks_for(n, 64 / 11)), dim3(64),
+           0, entry_offset, slots, (uint32_t)n, grads, abs_grads, status);
+  else
+    launch(ctx, "reduce_slots", reduce_slots_kernel<9>, dim3(blocks_for(n, 64 / 9)), dim3(64), 0,
+           entry_offset, slots, (uint32_t)n, grads, abs_grads, status);
 }
 
 }  // namespace splat
