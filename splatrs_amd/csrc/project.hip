@@ -435,24 +435,15 @@ __global__ void proj_unpack_kernel(const splat_proj* __restrict__ in, uint32_t n
 }
 
 // project_backward_one, src/project.rs:335-474.
-// Gradient of one Gaussian (parameters g) for one view into out[j * ostride]: zeroed and
-// written, or with ACC added to what out holds (each plane receives at most one term).
-template <bool ACC = false>
-__device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev& cam,
+// Gradient of one Gaussian (parameters g) for one view: put(j, x) receives the gradient x
+// of each plane j that has a nonzero term, at most once per plane; the other planes' are
+// zero.
+template <class Put>
+__device__ __forceinline__ void project_bwd_put(const float* g, const CameraDev& cam,
                                                 float near_plane, float eps2d, int antialias,
                                                 int degree,
                                                 const splat_proj_grad* __restrict__ dproj_row,
-                                                float* out, int ostride = 1) {
-  const auto put = [&](int j, float x) {
-    if (ACC)
-      out[j * ostride] += x;
-    else
-      out[j * ostride] = x;
-  };
-  if (!ACC) {
-#pragma unroll
-    for (int j = 0; j < kParams; j++) out[j * ostride] = 0.0f;
-  }
+                                                Put&& put) {
   Core k;
   if (core(g, cam, near_plane, eps2d, antialias != 0, degree, k)) {
     const splat_proj_grad dp = *dproj_row;
@@ -595,6 +586,27 @@ __device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev&
       for (int t = 0; t < 4; t++) put(6 + t, (dq[t] - u[t] * dot) / nn);
     }
   }
+}
+
+// The same gradient into out[j * ostride]: zeroed and written, or with ACC added to what
+// out holds.
+template <bool ACC = false>
+__device__ __forceinline__ void project_bwd_row(const float* g, const CameraDev& cam,
+                                                float near_plane, float eps2d, int antialias,
+                                                int degree,
+                                                const splat_proj_grad* __restrict__ dproj_row,
+                                                float* out, int ostride = 1) {
+  if (!ACC) {
+#pragma unroll
+    for (int j = 0; j < kParams; j++) out[j * ostride] = 0.0f;
+  }
+  project_bwd_put(g, cam, near_plane, eps2d, antialias, degree, dproj_row,
+                  [&](int j, float x) {
+                    if (ACC)
+                      out[j * ostride] += x;
+                    else
+                      out[j * ostride] = x;
+                  });
 }
 
 __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restrict__ params,
