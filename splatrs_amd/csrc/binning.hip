@@ -45,8 +45,9 @@ __device__ __forceinline__ uint32_t search_offsets(Off off, uint32_t lo, uint32_
 
 constexpr uint32_t kKeygenSpan = 2048;  // Gaussian offsets staged per keygen workgroup
 
-// marks[b] = the Gaussian that holds entry b * kKeygenEntries (the first entry of keygen
-// workgroup b), written by that Gaussian.
+// marks[b] = the Gaussian that holds entry b * EPB (the first entry of keygen workgroup b,
+// EPB entries per workgroup), written by that Gaussian.
+template <uint32_t EPB = kKeygenEntries>
 __global__ void __launch_bounds__(256) block_marks_kernel(const uint32_t* __restrict__ entry_offset,
                                                           uint32_t n, uint32_t* __restrict__ marks,
                                                           const StepStatus* __restrict__ status) {
@@ -54,8 +55,7 @@ __global__ void __launch_bounds__(256) block_marks_kernel(const uint32_t* __rest
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const uint32_t lo = entry_offset[g], hi = entry_offset[g + 1];
-  for (uint32_t b = (lo + kKeygenEntries - 1) / kKeygenEntries; b * kKeygenEntries < hi; b++)
-    marks[b] = g;
+  for (uint32_t b = (lo + EPB - 1) / EPB; b * EPB < hi; b++) marks[b] = g;
 }
 
 // The Gaussians of one keygen workgroup's entries [b * kKeygenEntries, ...): they lie in
@@ -67,12 +67,13 @@ struct KeygenSpan {
   bool staged;
 };
 
+template <uint32_t EPB = kKeygenEntries>
 __device__ __forceinline__ KeygenSpan keygen_span(uint32_t e_total,
                                                   const uint32_t* __restrict__ entry_offset,
                                                   uint32_t n, const uint32_t* __restrict__ marks,
                                                   uint32_t* s_off) {
   const uint32_t b = blockIdx.x;
-  const uint32_t nb = (e_total + kKeygenEntries - 1) / kKeygenEntries;
+  const uint32_t nb = (e_total + EPB - 1) / EPB;
   KeygenSpan sp;
   sp.g0 = marks[b];
   sp.ghi = b + 1 < nb ? marks[b + 1] + 1 : n;
@@ -328,31 +329,18 @@ __global__ void __launch_bounds__(1024) sort_big_kernel(const uint32_t* __restri
 
 // ---- radix binning ---------------------------------------------------------------------
 
-// Depth keys of the rows (rows without entries sort last) and their indices.
-__global__ void __launch_bounds__(256) depth_keys_kernel(const float* __restrict__ depth,
-                                                         const uint32_t* __restrict__ ntiles,
-                                                         uint32_t n, uint32_t* __restrict__ keys,
-                                                         uint32_t* __restrict__ vals,
-                                                         const StepStatus* __restrict__ status) {
-  if (status && status->abort) return;
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  keys[g] = ntiles[g] ? f2bits(depth[g]) : 0xffffffffu;  // visible depths are finite > 0
-  vals[g] = g;
-}
-
 constexpr uint32_t kStagedPairs = 1024;  // views whose work-row table is staged in LDS
 
-// Entry counts of the rows in depth order, and each row's record for keygen_ordered:
-// {first tile (row-major over the row's rectangle, src/rasterize.rs:86-99), rectangle
-// width | tile-grid stride << 16, first entry id, Gaussian}.  Rows without entries get no
-// record (no entry position resolves to them).
+// Depth keys of the rows (rows without entries sort last), their indices, and each row's
+// record for keygen_ordered, in row order (coalesced): {first tile (row-major over the
+// row's rectangle, src/rasterize.rs:86-99), rectangle width | tile-grid stride << 16,
+// first entry id, row}.  Rows without entries get no record.
 template <bool BATCH>
-__global__ void __launch_bounds__(256) ordered_rows_kernel(
-    const uint32_t* __restrict__ order, const uint32_t* __restrict__ ntiles,
+__global__ void __launch_bounds__(256) depth_keys_kernel(
+    const float* __restrict__ depth, const uint32_t* __restrict__ ntiles,
     const uint2* __restrict__ rect, const uint32_t* __restrict__ entry_offset, uint32_t n,
-    uint32_t tiles_x, BatchDev bd, uint32_t* __restrict__ cnt, uint4* __restrict__ rows,
-    const StepStatus* __restrict__ status) {
+    uint32_t tiles_x, BatchDev bd, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+    uint4* __restrict__ recs, const StepStatus* __restrict__ status) {
   __shared__ uint32_t s_wb[BATCH ? kStagedPairs + 1 : 1];  // the views' first work rows
   if (status && status->abort) return;  // uniform
   const bool staged = BATCH && bd.npairs <= kStagedPairs;
@@ -360,11 +348,11 @@ __global__ void __launch_bounds__(256) ordered_rows_kernel(
     for (uint32_t q = threadIdx.x; q <= bd.npairs; q += blockDim.x) s_wb[q] = bd.work_base[q];
     __syncthreads();
   }
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t g = order[i];
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
   const uint32_t c = ntiles[g];
-  cnt[i] = c;
+  keys[g] = c ? f2bits(depth[g]) : 0xffffffffu;  // visible depths are finite > 0
+  vals[g] = g;
   if (c == 0) return;
   const uint2 rc = rect[g];
   const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
@@ -377,7 +365,21 @@ __global__ void __launch_bounds__(256) ordered_rows_kernel(
     stride = tiles_x;
     first = y0 * stride + x0;
   }
-  rows[i] = make_uint4(first, (x1 - x0) | (stride << 16), entry_offset[g], g);
+  recs[g] = make_uint4(first, (x1 - x0) | (stride << 16), entry_offset[g], g);
+}
+
+// Entry counts and records of the rows in depth order.
+__global__ void __launch_bounds__(256) ordered_rows_kernel(
+    const uint32_t* __restrict__ order, const uint32_t* __restrict__ ntiles,
+    const uint4* __restrict__ recs, uint32_t n, uint32_t* __restrict__ cnt,
+    uint4* __restrict__ rows, const StepStatus* __restrict__ status) {
+  if (status && status->abort) return;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t g = order[i];
+  const uint32_t c = ntiles[g];
+  cnt[i] = c;
+  if (c) rows[i] = recs[g];
 }
 
 // (x / w, x % w) for 0 < w < 2^16 and x < 2^22 (a rectangle of a view's tile grid, which
@@ -396,6 +398,9 @@ __device__ __forceinline__ uint2 divmod_small(uint32_t x, uint32_t w) {
   return make_uint2(q, (uint32_t)r);
 }
 
+constexpr uint32_t kOrderedPerLane = 4;
+constexpr uint32_t kOrderedEntries = kKeygenThreads * kOrderedPerLane;  // per workgroup
+
 // Position j of the depth-ordered entry list: the row (soff, marks: its offsets in depth
 // order), whose record gives the entry's tile (the local index row-major over the
 // rectangle) and the reference's entry id (the row's Gaussian-order offset + the local
@@ -410,20 +415,23 @@ __global__ void __launch_bounds__(256) keygen_ordered_kernel(
     if (status->abort) return;
     e_total = status->entries;
   }
-  const uint32_t eb = blockIdx.x * kKeygenEntries;
+  const uint32_t eb = blockIdx.x * kOrderedEntries;
   if (eb >= e_total) return;  // uniform: no entry of this workgroup exists
-  const KeygenSpan sp = keygen_span(e_total, soff, n, marks, s_off);
-  const uint32_t j = eb + threadIdx.x;
-  if (j >= e_total) return;
-  const uint32_t i = keygen_find(sp, j, soff, s_off);
-  const uint4 row = rows[i];
-  const uint32_t local = j - (sp.staged ? s_off[i - sp.g0] : soff[i]);
-  const uint2 qr = divmod_small(local, row.y & 0xffffu);
-  const uint32_t e = row.z + local;
-  tkeys[j] = row.x + qr.x * (row.y >> 16) + qr.y;
-  tvals[j] = e;
-  tgauss[j] = row.w;
-  if (entry_gaussian) entry_gaussian[e] = row.w;
+  const KeygenSpan sp = keygen_span<kOrderedEntries>(e_total, soff, n, marks, s_off);
+#pragma unroll
+  for (uint32_t u = 0; u < kOrderedPerLane; u++) {
+    const uint32_t j = eb + u * kKeygenThreads + threadIdx.x;
+    if (j >= e_total) break;
+    const uint32_t i = keygen_find(sp, j, soff, s_off);
+    const uint4 row = rows[i];
+    const uint32_t local = j - (sp.staged ? s_off[i - sp.g0] : soff[i]);
+    const uint2 qr = divmod_small(local, row.y & 0xffffu);
+    const uint32_t e = row.z + local;
+    tkeys[j] = row.x + qr.x * (row.y >> 16) + qr.y;
+    tvals[j] = e;
+    tgauss[j] = row.w;
+    if (entry_gaussian) entry_gaussian[e] = row.w;
+  }
 }
 
 // The tile-sorted entries: tile ranges, (optional) keys and, when tvals is given, the
@@ -520,7 +528,7 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   // 2. keygen + per-tile counts
   fill_u32(ctx, bb.tile_count, kTileSub * T, 0);
   fill_u32(ctx, small + 1, 1, 0);
-  launch(ctx, "bin_marks", block_marks_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch(ctx, "bin_marks", block_marks_kernel<>, dim3(blocks_for(n, 256)), dim3(256), 0,
          (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
          (const StepStatus*)nullptr);
   launch(ctx, "bin_keygen", keygen_kernel, dim3(blocks_for(E, kKeygenEntries)),
@@ -564,7 +572,7 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
   fill_u32(ctx, bb.tile_count, kTileSub * T, 0);
   fill_u32(ctx, small + 1, 1, 0);
   // The entry kernels cover the capacity and read the step's count from `status`.
-  launch(ctx, "bin_marks", block_marks_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+  launch(ctx, "bin_marks", block_marks_kernel<>, dim3(blocks_for(n, 256)), dim3(256), 0,
          (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
          (const StepStatus*)status);
   if (bd)
@@ -607,6 +615,7 @@ RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_
   }
   rs.soff = (uint32_t*)store["radix_soff"].reserve((n + 1) * 4);
   rs.rows = (uint4*)store["radix_rows"].reserve((n + 1) * 16);
+  rs.recs = (uint4*)store["radix_recs"].reserve((n + 1) * 16);
   rs.counts = (uint32_t*)store["radix_counts"].reserve(radix_count_words(std::max(n, capacity)) * 4);
   return rs;
 }
@@ -624,30 +633,31 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
     if (status) bin_guard(ctx, total64, capacity, status);
   }
   const uint32_t* n_entries = status ? &status->entries : nullptr;
-  // 1. rows by depth
-  launch(ctx, "bin_depth_keys", depth_keys_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         (const float*)pb.depth, (const uint32_t*)pb.ntiles, (uint32_t)n, rs.rk[0], rs.rv[0],
-         (const StepStatus*)status);
+  // 1. rows by depth (and the rows' records, in row order)
+  if (bd)
+    launch(ctx, "bin_depth_keys", depth_keys_kernel<true>, dim3(blocks_for(n, 256)), dim3(256),
+           0, (const float*)pb.depth, (const uint32_t*)pb.ntiles, (const uint2*)pb.rect,
+           (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.tiles_x, *bd, rs.rk[0], rs.rv[0],
+           rs.recs, (const StepStatus*)status);
+  else
+    launch(ctx, "bin_depth_keys", depth_keys_kernel<false>, dim3(blocks_for(n, 256)), dim3(256),
+           0, (const float*)pb.depth, (const uint32_t*)pb.ntiles, (const uint2*)pb.rect,
+           (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.tiles_x, BatchDev{}, rs.rk[0],
+           rs.rv[0], rs.recs, (const StepStatus*)status);
   const int c = radix_sort_pairs(ctx, rs.rk[0], rs.rv[0], rs.rk[1], rs.rv[1], n, (uint32_t)n,
                                  nullptr, 32, rs.counts, scan_sums, status);
   const uint32_t* order = rs.rv[c];
   // 2. the rows' entry offsets and records in depth order
-  if (bd)
-    launch(ctx, "bin_ordered_rows", ordered_rows_kernel<true>, dim3(blocks_for(n, 256)),
-           dim3(256), 0, order, (const uint32_t*)pb.ntiles, (const uint2*)pb.rect,
-           (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.tiles_x, *bd, rs.rk[1 - c], rs.rows,
-           (const StepStatus*)status);
-  else
-    launch(ctx, "bin_ordered_rows", ordered_rows_kernel<false>, dim3(blocks_for(n, 256)),
-           dim3(256), 0, order, (const uint32_t*)pb.ntiles, (const uint2*)pb.rect,
-           (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.tiles_x, BatchDev{}, rs.rk[1 - c],
-           rs.rows, (const StepStatus*)status);
+  launch(ctx, "bin_ordered_rows", ordered_rows_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+         order, (const uint32_t*)pb.ntiles, (const uint4*)rs.recs, (uint32_t)n, rs.rk[1 - c],
+         rs.rows, (const StepStatus*)status);
   exclusive_scan_u32(ctx, rs.rk[1 - c], rs.soff, n, small + 4, scan_sums);
-  launch(ctx, "bin_marks", block_marks_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         (const uint32_t*)rs.soff, (uint32_t)n, bb.block_marks, (const StepStatus*)status);
+  launch(ctx, "bin_marks", block_marks_kernel<kOrderedEntries>, dim3(blocks_for(n, 256)),
+         dim3(256), 0, (const uint32_t*)rs.soff, (uint32_t)n, bb.block_marks,
+         (const StepStatus*)status);
   // 3. entries in depth order, keyed by tile
   launch(ctx, "bin_keygen", keygen_ordered_kernel,
-         dim3(blocks_for(status ? capacity : e_host, kKeygenEntries)), dim3(kKeygenThreads), 0,
+         dim3(blocks_for(status ? capacity : e_host, kOrderedEntries)), dim3(kKeygenThreads), 0,
          (const uint32_t*)rs.soff, (const uint4*)rs.rows, (uint32_t)n, e_host,
          (const uint32_t*)bb.block_marks, rs.ek[0], rs.ev[0], rs.eg[0], bb.entry_gaussian,
          (const StepStatus*)status);

@@ -431,7 +431,8 @@ struct RadixScratch {
   uint32_t *rk[2], *rv[2];  // rows (n)
   uint32_t *ek[2], *ev[2], *eg[2];  // entries (capacity): tile, entry id, Gaussian
   uint32_t* soff;           // n + 1
-  uint4* rows;              // n + 1 row records of the entry generator
+  uint4* rows;              // n + 1 row records of the entry generator, depth order
+  uint4* recs;              // n + 1 the same records in row order
   uint32_t* counts;         // radix_count_words(max(n, capacity))
 };
 RadixScratch radix_scratch(std::map<std::string, DevBuf>& store, size_t n, size_t capacity);
