@@ -295,6 +295,26 @@ int splat_ply_write(const char* path, const float* gaussians, uint64_t n, int32_
 int splat_ply_read(const char* path, float* gaussians, uint64_t capacity, uint64_t* n,
                    int32_t* sh_degree, char* err, size_t err_len);
 
+/* ---- checkpoints ---------------------------------------------------------------------
+ * No counterpart in the reference, whose train loop saves only the model (.ply,
+ * src/ply.rs:19-69, called from src/train.rs:275-279): a checkpoint adds the Adam moments
+ * and step count and the densification statistics, so that a run resumed with
+ * splat_trainer_restore continues exactly as the uninterrupted run would.  Layout (little
+ * endian): "SPLATCKP", u32 version 1, i32 max_sh_degree, u64 n, u64 adam_step, then params,
+ * m, v (n x 59 f32 each, the canonical row order), grad_accum (n f32), denom, max_radii
+ * (n u32 each).  Null arrays are written as zeros (write) or skipped (read).  read: *n,
+ * *adam_step and *max_sh_degree are set from the header; the arrays are filled only if
+ * capacity >= *n (otherwise SPLAT_ERR_CAPACITY), so a call with null arrays and capacity 0
+ * sizes the output.  A file whose length does not match its header is SPLAT_ERR_IO. */
+int splat_checkpoint_write(const char* path, const float* gaussians, const float* m,
+                           const float* v, const float* grad_accum, const uint32_t* denom,
+                           const uint32_t* max_radii, uint64_t n, uint64_t adam_step,
+                           int32_t max_sh_degree, char* err, size_t err_len);
+int splat_checkpoint_read(const char* path, float* gaussians, float* m, float* v,
+                          float* grad_accum, uint32_t* denom, uint32_t* max_radii,
+                          uint64_t capacity, uint64_t* n, uint64_t* adam_step,
+                          int32_t* max_sh_degree, char* err, size_t err_len);
+
 /* ---- trainer: device-resident training (src/train.rs) ---------------------------- */
 
 typedef struct splat_trainer splat_trainer;
@@ -418,6 +438,15 @@ int splat_trainer_info(splat_trainer* t, uint64_t* n, uint64_t* adam_step);
 int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* v,
                            float* grad_accum, uint32_t* denom, uint32_t* max_radii,
                            uint64_t capacity);
+/* Replace the trainer's model and training state with a saved one (the inverse of
+ * splat_trainer_download; no counterpart in the reference): n rows of parameters, Adam
+ * moments m, v (null: zeros), densification statistics (null: zeros) and the Adam step
+ * count.  Capacity grows to n as needed; views and launch shapes are kept.  With learned
+ * densification attached, its extra statistics restart from zero and every row counts as
+ * an original (no lineage). */
+int splat_trainer_restore(splat_trainer* t, const float* gaussians, const float* m,
+                          const float* v, const float* grad_accum, const uint32_t* denom,
+                          const uint32_t* max_radii, uint64_t n, uint64_t adam_step);
 
 /* ---- batched training ----------------------------------------------------------------
  * No counterpart in the reference (one view of one scene per iteration, train.rs:211-247).

@@ -161,6 +161,11 @@ _SIGS = {
     "splat_train_view_at": (u32, [u64, u32, u32]),
     "splat_ply_write": (i32, [C.c_char_p, P, u64, i32, C.c_char_p, C.c_size_t]),
     "splat_ply_read": (i32, [C.c_char_p, P, u64, P, P, C.c_char_p, C.c_size_t]),
+    "splat_checkpoint_write": (i32, [C.c_char_p, P, P, P, P, P, P, u64, u64, i32, C.c_char_p,
+                                     C.c_size_t]),
+    "splat_checkpoint_read": (i32, [C.c_char_p, P, P, P, P, P, P, u64, P, P, P, C.c_char_p,
+                                    C.c_size_t]),
+    "splat_trainer_restore": (i32, [P, P, P, P, P, P, P, u64, u64]),
     "splat_trainer_create": (i32, [P, P, u64, u64, P]),
     "splat_trainer_destroy": (i32, [P]),
     "splat_trainer_add_view": (i32, [P, P, P, P]),
@@ -474,6 +479,53 @@ def ply_read(path):
     return g, deg.value
 
 
+class CheckpointError(SplatError):
+    pass
+
+
+def checkpoint_write(path, state, adam_step, max_sh_degree):
+    """Resumable checkpoint (splat_checkpoint_write): state as Trainer.download(moments=True,
+    stats=True) returns it (missing moments or statistics are written as zeros)."""
+    g = np.ascontiguousarray(state["params"], np.float32)
+    n = len(g)
+
+    def arr(key, dtype, shape):
+        a = state.get(key)
+        return None if a is None else np.ascontiguousarray(a, dtype).reshape(shape)
+    m, v = arr("m", np.float32, g.shape), arr("v", np.float32, g.shape)
+    acc = arr("grad_accum", np.float32, (n,))
+    den, mr = arr("denom", np.uint32, (n,)), arr("max_radii", np.uint32, (n,))
+    err = C.create_string_buffer(512)
+    st = lib().splat_checkpoint_write(os.fsencode(path), _p(g), _p(m), _p(v), _p(acc), _p(den),
+                                      _p(mr), n, adam_step, max_sh_degree, err, len(err))
+    if st != OK:
+        raise CheckpointError(st, err.value.decode())
+
+
+def checkpoint_read(path):
+    """-> (state dict with params, m, v, grad_accum, denom, max_radii; adam_step;
+    max_sh_degree)."""
+    err = C.create_string_buffer(512)
+    n, step, deg = C.c_uint64(), C.c_uint64(), C.c_int32()
+    st = lib().splat_checkpoint_read(os.fsencode(path), None, None, None, None, None, None, 0,
+                                     C.byref(n), C.byref(step), C.byref(deg), err, len(err))
+    if st not in (OK, ERR_CAPACITY):
+        raise CheckpointError(st, err.value.decode())
+    k = n.value
+    out = {"params": np.zeros((k, rec.NUM_PARAMS), np.float32),
+           "m": np.zeros((k, rec.NUM_PARAMS), np.float32),
+           "v": np.zeros((k, rec.NUM_PARAMS), np.float32),
+           "grad_accum": np.zeros(k, np.float32), "denom": np.zeros(k, np.uint32),
+           "max_radii": np.zeros(k, np.uint32)}
+    st = lib().splat_checkpoint_read(os.fsencode(path), _p(out["params"]), _p(out["m"]),
+                                     _p(out["v"]), _p(out["grad_accum"]), _p(out["denom"]),
+                                     _p(out["max_radii"]), k, C.byref(n), C.byref(step),
+                                     C.byref(deg), err, len(err))
+    if st != OK:
+        raise CheckpointError(st, err.value.decode())
+    return out, step.value, deg.value
+
+
 def to_rgb8(image):
     """to_rgb8, src/render.rs:26-31."""
     image = np.ascontiguousarray(image, np.float32)
@@ -654,6 +706,22 @@ class Trainer:
         if stats:
             out.update(grad_accum=acc, denom=den, max_radii=mr)
         return out
+
+    def restore(self, state, adam_step):
+        """Replace the model and training state (splat_trainer_restore): state as download
+        returns it; missing moments or statistics restore as zeros."""
+        g = np.ascontiguousarray(state["params"], np.float32)
+        n = len(g)
+
+        def arr(key, dtype, shape):
+            a = state.get(key)
+            return None if a is None else np.ascontiguousarray(a, dtype).reshape(shape)
+        m, v = arr("m", np.float32, g.shape), arr("v", np.float32, g.shape)
+        acc = arr("grad_accum", np.float32, (n,))
+        den, mr = arr("denom", np.uint32, (n,)), arr("max_radii", np.uint32, (n,))
+        self.ctx.check(lib().splat_trainer_restore(self._h, _p(g), _p(m), _p(v), _p(acc),
+                                                   _p(den), _p(mr), n, adam_step),
+                       "trainer_restore")
 
 
 class Scorer:

@@ -873,6 +873,46 @@ int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* 
   });
 }
 
+int splat_trainer_restore(splat_trainer* t, const float* gaussians, const float* m,
+                          const float* v, const float* grad_accum, const uint32_t* denom,
+                          const uint32_t* max_radii, uint64_t n, uint64_t adam_step) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(n == 0 || gaussians, "null argument");
+    drain(t);
+    splat_ctx* ctx = t->ctx;
+    if (n > t->cap) alloc_model(t, n);
+    t->n = n;
+    t->adam_step = adam_step;
+    if (n) {
+      float* staging = (float*)t->store["aos"].reserve(n * kParams * 4);
+      const std::pair<const float*, DevBuf*> arrays[3] = {{gaussians, &t->params}, {m, &t->m},
+                                                          {v, &t->v}};
+      for (auto& a : arrays) {
+        if (a.first) {
+          SPLAT_HIP(hipMemcpyAsync(staging, a.first, n * kParams * 4, hipMemcpyHostToDevice,
+                                   ctx->stream));
+          aos_to_planes(ctx, staging, n, a.second->as<float>(), t->stride);
+          SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+        } else {
+          fill_u32(ctx, a.second->as<uint32_t>(), t->stride * kParams, 0u);
+        }
+      }
+    }
+    zero_stats(t);
+    if (t->track) learn_track_alloc(t);  // sized for the capacity; no lineage
+    if (n && grad_accum)
+      SPLAT_HIP(hipMemcpyAsync(t->grad_accum.ptr, grad_accum, n * 4, hipMemcpyHostToDevice,
+                               ctx->stream));
+    if (n && denom)
+      SPLAT_HIP(hipMemcpyAsync(t->denom.ptr, denom, n * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (n && max_radii)
+      SPLAT_HIP(hipMemcpyAsync(t->max_radii.ptr, max_radii, n * 4, hipMemcpyHostToDevice,
+                               ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    t->store.erase("aos");
+  });
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------- train loop -------

@@ -153,6 +153,52 @@ def test_trainer_densify_reset_and_render(ctx):
     tr.close()
 
 
+def test_checkpoint_resume_equals_uninterrupted_training(ctx, tmp_path):
+    """Train 12 iterations (statistics accumulating, a densification after 6); a second
+    run stops after 8, writes a checkpoint, and a new trainer restored from it runs the
+    last 4: model, moments, Adam step and statistics equal the uninterrupted run's
+    bitwise."""
+    g, cams, views = small_training_problem(ctx, 4000)
+    extent = scenes.scene_extent(cams)
+    cfg = rec.densify_config(grad_threshold=2e-5)
+
+    def run(tr, iters):
+        for it in iters:
+            lr = rec.learning_rates(rec.position_lr(it, extent))
+            tr.step(it % 8, rec.step_params(1, lr, accumulate_stats=True))
+            if it == 6:
+                tr.densify(cfg, extent, 99)
+
+    def trainer(model):
+        tr = abi.Trainer(ctx, model)
+        for c, v in zip(cams, views):
+            tr.add_view(c, v)
+        return tr
+    say("uninterrupted run")
+    a = trainer(g)
+    run(a, range(1, 13))
+    want = a.download(moments=True, stats=True)
+    want_step = a.info()[1]
+    a.close()
+    say("run to 8, checkpoint, resume in a new trainer")
+    b = trainer(g)
+    run(b, range(1, 9))
+    path = tmp_path / "run.ckpt"
+    abi.checkpoint_write(path, b.download(moments=True, stats=True), b.info()[1], 1)
+    b.close()
+    saved, step, deg = abi.checkpoint_read(path)
+    assert step == 8 and deg == 1
+    c = trainer(scenes.small_scene(7, 100))  # any model: restore replaces it
+    c.restore(saved, step)
+    run(c, range(9, 13))
+    got = c.download(moments=True, stats=True)
+    assert c.info()[1] == want_step
+    c.close()
+    for k in want:
+        assert got[k].shape == want[k].shape, k
+        assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), k
+
+
 def test_graph_steps_equal_eager_steps(ctx):
     """Graph-captured steps (with entry arrays far too small at first, so steps abort, the
     arrays grow and the aborted steps are replayed) give the same model as steps launched
