@@ -438,6 +438,14 @@ int splat_trainer_info(splat_trainer* t, uint64_t* n, uint64_t* adam_step);
 int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* v,
                            float* grad_accum, uint32_t* denom, uint32_t* max_radii,
                            uint64_t capacity);
+/* evaluate, src/train.rs:309-321, on the trainer's views: each listed view rendered with
+ * the given SH degree, antialias flag and background, its PSNR against the view's target
+ * (psnr, src/loss.rs:175-190: binary64 mean squared error of the clamped values, binary32
+ * result; +inf for a perfect match) into psnr[q] (optional) and their binary32 mean, summed
+ * in list order, into *mean (NaN for an empty list). */
+int splat_trainer_evaluate(splat_trainer* t, const uint32_t* views, uint32_t count,
+                           int32_t sh_degree, int32_t antialias, const float* background,
+                           float* psnr, float* mean);
 /* Replace the trainer's model and training state with a saved one (the inverse of
  * splat_trainer_download; no counterpart in the reference): n rows of parameters, Adam
  * moments m, v (null: zeros), densification statistics (null: zeros) and the Adam step

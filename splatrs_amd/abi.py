@@ -166,6 +166,7 @@ _SIGS = {
     "splat_checkpoint_read": (i32, [C.c_char_p, P, P, P, P, P, P, u64, P, P, P, C.c_char_p,
                                     C.c_size_t]),
     "splat_trainer_restore": (i32, [P, P, P, P, P, P, P, u64, u64]),
+    "splat_trainer_evaluate": (i32, [P, P, u32, i32, i32, P, P, P]),
     "splat_trainer_create": (i32, [P, P, u64, u64, P]),
     "splat_trainer_destroy": (i32, [P]),
     "splat_trainer_add_view": (i32, [P, P, P, P]),
@@ -706,6 +707,17 @@ class Trainer:
         if stats:
             out.update(grad_accum=acc, denom=den, max_radii=mr)
         return out
+
+    def evaluate(self, views, sh_degree, antialias=False, background=(0.0, 0.0, 0.0)):
+        """evaluate, src/train.rs:309-321 -> (mean PSNR, per-view PSNR array)."""
+        vs = np.ascontiguousarray(views, np.uint32)
+        bg = np.asarray(background, np.float32)
+        out = np.zeros(len(vs), np.float32)
+        mean = C.c_float()
+        self.ctx.check(lib().splat_trainer_evaluate(self._h, _p(vs), len(vs), sh_degree,
+                                                    int(bool(antialias)), _p(bg), _p(out),
+                                                    C.byref(mean)), "trainer_evaluate")
+        return mean.value, out
 
     def restore(self, state, adam_step):
         """Replace the model and training state (splat_trainer_restore): state as download

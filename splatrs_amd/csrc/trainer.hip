@@ -428,18 +428,19 @@ namespace splat {
 
 void trainer_drain(splat_trainer* t) { drain(t); }
 
-float trainer_eval_psnr(splat_trainer* t, uint32_t views, int32_t sh_degree, const float bg[3]) {
+// Per-view sums of squared clamped differences (src/loss.rs:175-190, in binary64) of the
+// rendered views against their RGB8 targets (View::to_real, src/dataset.rs:23-25).
+std::vector<double> trainer_view_sse(splat_trainer* t, const uint32_t* views, uint32_t count,
+                                     int32_t sh_degree, int32_t antialias, const float bg[3]) {
   drain(t);
   splat_ctx* ctx = t->ctx;
-  views = std::min<uint32_t>(views, (uint32_t)t->views.size());
-  if (views == 0) return 0.0f;
   constexpr unsigned kBlocks = 512;
-  double* partial = (double*)t->store["psnr_partial"].reserve(kBlocks * 8 * (size_t)views);
-  for (uint32_t v = 0; v < views; v++) {
-    const TrainerView& tv = *t->views[v];
+  double* partial = (double*)t->store["psnr_partial"].reserve(kBlocks * 8 * (size_t)count + 8);
+  for (uint32_t q = 0; q < count; q++) {
+    const TrainerView& tv = *t->views[views[q]];
     const uint32_t W = tv.cam.width, H = tv.cam.height;
     const size_t npx = (size_t)W * H;
-    splat_proj_opts opts{0.2f, 0.3f, 0, sh_degree};
+    splat_proj_opts opts{0.2f, 0.3f, antialias, sh_degree};
     ProjBuffers pb = proj_buffers(t);
     project_forward(ctx, t->params.as<float>(), t->n, t->stride, tv.cam, opts, pb);
     BinBuffers bb{};
@@ -450,17 +451,29 @@ float trainer_eval_psnr(splat_trainer* t, uint32_t views, int32_t sh_degree, con
     uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
     raster_forward(ctx, pb, bb, W, H, bg, img, final_t, n_contrib);
     launch(ctx, "sq_err", sq_err_kernel, dim3(kBlocks), dim3(256), 0, (const float*)img,
-           tv.rgb.as<uint8_t>(), npx * 3, partial + (size_t)v * kBlocks);
+           tv.rgb.as<uint8_t>(), npx * 3, partial + (size_t)q * kBlocks);
   }
-  std::vector<double> h((size_t)kBlocks * views);
-  SPLAT_HIP(hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-  SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<double> h((size_t)kBlocks * count), sse(count, 0.0);
+  if (count) {
+    SPLAT_HIP(hipMemcpyAsync(h.data(), partial, h.size() * 8, hipMemcpyDeviceToHost,
+                             ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  for (uint32_t q = 0; q < count; q++)
+    for (unsigned b = 0; b < kBlocks; b++) sse[q] += h[(size_t)q * kBlocks + b];
+  return sse;
+}
+
+float trainer_eval_psnr(splat_trainer* t, uint32_t views, int32_t sh_degree, const float bg[3]) {
+  views = std::min<uint32_t>(views, (uint32_t)t->views.size());
+  if (views == 0) return 0.0f;
+  std::vector<uint32_t> list(views);
+  for (uint32_t v = 0; v < views; v++) list[v] = v;
+  const std::vector<double> sse = trainer_view_sse(t, list.data(), views, sh_degree, 0, bg);
   double tot = 0;
   for (uint32_t v = 0; v < views; v++) {
-    double sse = 0;
-    for (unsigned b = 0; b < kBlocks; b++) sse += h[(size_t)v * kBlocks + b];
     const TrainerView& tv = *t->views[v];
-    const double mse = sse / (double)((size_t)tv.cam.width * tv.cam.height * 3);
+    const double mse = sse[v] / (double)((size_t)tv.cam.width * tv.cam.height * 3);
     tot += mse <= 0 ? 100.0 : -10.0 * std::log10(mse);
   }
   return (float)(tot / views);
@@ -870,6 +883,30 @@ int splat_trainer_download(splat_trainer* t, float* gaussians, float* m, float* 
                                ctx->stream));
     SPLAT_HIP(hipStreamSynchronize(ctx->stream));
     t->store.erase("aos_out");
+  });
+}
+
+int splat_trainer_evaluate(splat_trainer* t, const uint32_t* views, uint32_t count,
+                           int32_t sh_degree, int32_t antialias, const float* background,
+                           float* psnr, float* mean) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(count == 0 || views, "null argument");
+    SPLAT_REQUIRE(sh_degree >= 0 && sh_degree <= 3, "sh_degree must be 0..3");
+    for (uint32_t q = 0; q < count; q++)
+      SPLAT_REQUIRE(views[q] < t->views.size(), "view index out of range");
+    const float zero[3] = {0.0f, 0.0f, 0.0f};
+    const std::vector<double> sse =
+        trainer_view_sse(t, views, count, sh_degree, antialias, background ? background : zero);
+    // evaluate, src/train.rs:309-321: binary32 PSNR per view, summed in order
+    float total = 0.0f;
+    for (uint32_t q = 0; q < count; q++) {
+      const TrainerView& tv = *t->views[views[q]];
+      const double mse = sse[q] / (double)std::max<size_t>((size_t)tv.cam.width * tv.cam.height * 3, 1);
+      const float p = mse <= 0.0 ? INFINITY : (float)(-10.0 * std::log10(mse));
+      if (psnr) psnr[q] = p;
+      total += p;
+    }
+    if (mean) *mean = count ? total / (float)count : NAN;
   });
 }
 

@@ -199,6 +199,41 @@ def test_checkpoint_resume_equals_uninterrupted_training(ctx, tmp_path):
         assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), k
 
 
+def test_evaluate_matches_oracle_psnr(ctx):
+    """evaluate (src/train.rs:309-321) on a subset of the trainer's views after a few
+    steps: per-view PSNR of the oracle's render against the RGB8 targets, binary32 mean in
+    list order; an empty list gives NaN and a bad view index an error."""
+    g, cams, views = small_training_problem(ctx, 3000)
+    tr = abi.Trainer(ctx, g)
+    for c, v in zip(cams, views):
+        tr.add_view(c, v)
+    extent = scenes.scene_extent(cams)
+    for it in range(1, 5):
+        tr.step(it % 8, rec.step_params(1, rec.learning_rates(rec.position_lr(it, extent))))
+    p = tr.download()["params"]
+    pick, bg = [5, 0, 3], (0.1, 0.2, 0.3)
+    mean, per = tr.evaluate(pick, 1, background=bg)
+    want = []
+    for k in pick:
+        po = O.project_all(p, cams[k], rec.proj_opts(1))
+        b = O.bin(po, 256, 256)
+        img, _, _ = O.rasterize_forward(po, b, rec.raster_settings(256, 256, bg))
+        tgt = views[k].astype(np.float32) / np.float32(255.0)
+        d = np.clip(img, 0, 1).astype(np.float64) - np.clip(tgt, 0, 1).astype(np.float64)
+        want.append(np.float32(-10.0 * np.log10(np.mean(d * d))))
+    say(f"evaluate: {per} vs oracle {want}")
+    assert np.allclose(per, want, rtol=1e-5, atol=0)
+    total = np.float32(0)
+    for x in per:
+        total = np.float32(total + x)
+    assert mean == np.float32(total / np.float32(len(pick)))
+    m0, _ = tr.evaluate([], 1)
+    assert np.isnan(m0)
+    with pytest.raises(abi.SplatError):
+        tr.evaluate([8], 1)
+    tr.close()
+
+
 def test_graph_steps_equal_eager_steps(ctx):
     """Graph-captured steps (with entry arrays far too small at first, so steps abort, the
     arrays grow and the aborted steps are replayed) give the same model as steps launched
