@@ -438,12 +438,15 @@ __global__ void proj_unpack_kernel(const splat_proj* __restrict__ in, uint32_t n
 // Gradient of one Gaussian (parameters g) for one view: put(j, x) receives the gradient x
 // of each plane j that has a nonzero term, at most once per plane; the other planes' are
 // zero.
-template <class Put>
-__device__ __forceinline__ void project_bwd_put(const float* g, const CameraDev& cam,
-                                                float near_plane, float eps2d, int antialias,
-                                                int degree,
-                                                const splat_proj_grad* __restrict__ dproj_row,
-                                                Put&& put) {
+// sh_put(ch, dc, basis, nco) receives, for each colour channel with a nonzero term, the
+// factors of that channel's SH gradients: plane 11 + 3 kk + ch gets dc * basis[kk] for
+// kk < nco (src/project.rs:346-360).
+template <class Put, class ShPut>
+__device__ __forceinline__ void project_bwd_put_sh(const float* g, const CameraDev& cam,
+                                                   float near_plane, float eps2d, int antialias,
+                                                   int degree,
+                                                   const splat_proj_grad* __restrict__ dproj_row,
+                                                   Put&& put, ShPut&& sh_put) {
   Core k;
   if (core(g, cam, near_plane, eps2d, antialias != 0, degree, k)) {
     const splat_proj_grad dp = *dproj_row;
@@ -455,10 +458,10 @@ __device__ __forceinline__ void project_bwd_put(const float* g, const CameraDev&
     for (int ch = 0; ch < 3; ch++) {
       if (k.color_raw[ch] > 0.0f) {
         const float dc = dp.color[ch];
+        sh_put(ch, dc, k.basis, nco);
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
           if (kk < nco) {
-            put(11 + 3 * kk + ch, dc * k.basis[kk]);
             const float w = dc * g[11 + 3 * kk + ch];
             ddir.x += bgrad[kk].x * w;
             ddir.y += bgrad[kk].y * w;
@@ -588,6 +591,20 @@ __device__ __forceinline__ void project_bwd_put(const float* g, const CameraDev&
   }
 }
 
+template <class Put>
+__device__ __forceinline__ void project_bwd_put(const float* g, const CameraDev& cam,
+                                                float near_plane, float eps2d, int antialias,
+                                                int degree,
+                                                const splat_proj_grad* __restrict__ dproj_row,
+                                                Put&& put) {
+  project_bwd_put_sh(g, cam, near_plane, eps2d, antialias, degree, dproj_row, put,
+                     [&](int ch, float dc, const float* basis, int nco) {
+#pragma unroll
+                       for (int kk = 0; kk < 16; kk++)
+                         if (kk < nco) put(11 + 3 * kk + ch, dc * basis[kk]);
+                     });
+}
+
 // The same gradient into out[j * ostride]: zeroed and written, or with ACC added to what
 // out holds.
 template <bool ACC = false>
@@ -633,14 +650,17 @@ __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restric
 // (src/rasterize.rs:480-503; the workgroup's rows own one contiguous span of entries,
 // staged through LDS with coalesced loads), the densification statistics
 // (src/densify.rs:36-60, when accumulating) and project_backward_one (src/project.rs:
-// 335-474), whose 59 gradients go to LDS (the staging buffer, reused, [plane][rows])
-// instead of gradient planes in HBM.  After a barrier the workgroup runs Adam
+// 335-474).  Its gradients go to LDS (the staging buffer, reused, [plane][rows]): the 11
+// planes before the SH coefficients as values, the 48 SH planes as their factors (the 16
+// basis values and the 3 colour gradients of the row, zero where a plane has no term), so
+// 30 words per row instead of 59.  After a barrier the workgroup runs Adam
 // (src/adam.rs:44-61) over its 59 x kGradAdamRows parameter and moment values as float4
-// groups of 4 consecutive rows (plane-major), the gradients read back from LDS.  The
-// arithmetic is that of the separate host-tier kernels.  128 rows per workgroup (30 KB of
-// LDS) let 2.5 waves per SIMD run at the kernel's 165 VGPRs; 256 rows (60 KB) allowed 2
-// and ran 6% slower.
+// groups of 4 consecutive rows (plane-major), an SH plane's gradient formed as
+// dc * basis, the product project_backward_one takes.  The arithmetic is that of the
+// separate host-tier kernels (a plane without a term gets a zero gradient of either sign,
+// which Adam treats alike).
 constexpr uint32_t kGradAdamRows = 128;  // Gaussian rows per grad_adam workgroup
+constexpr uint32_t kGradWords = 11 + 16 + 3;  // LDS words per row: planes 0-10, basis, dc
 
 __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
     float* __restrict__ params, float* __restrict__ m, float* __restrict__ v, uint32_t n,
@@ -649,8 +669,9 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
     float c1, float c2, float b1, float b2, float eps, splat_learning_rates lr,
     const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status) {
   constexpr uint32_t R = kGradAdamRows;
-  constexpr uint32_t kChunk = 4 * R;  // entries staged per pass
-  constexpr uint32_t kShared = kChunk * kSlots > kParams * R ? kChunk * kSlots : kParams * R;
+  constexpr uint32_t kChunk = 2 * R;  // entries staged per pass
+  constexpr uint32_t kShared =
+      kChunk * kSlots > kGradWords * R ? kChunk * kSlots : kGradWords * R;
   __shared__ __align__(16) float sh[kShared];
   if (status && status->abort) return;  // uniform: every lane reads the same flag
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -679,6 +700,9 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
     }
   }
   __syncthreads();  // the staging buffer now holds gradients
+  float* const gw = sh + threadIdx.x;  // this row's column: word w at gw[w * R]
+#pragma unroll
+  for (uint32_t w = 0; w < kGradWords; w++) gw[w * R] = 0.0f;
   if (i < n) {
   splat_proj_grad pg;
   pg.mean2d[0] = sum[0];
@@ -710,9 +734,16 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
   float g[kParams];
 #pragma unroll
   for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
-  project_bwd_row(g, cam, near_plane, eps2d, antialias, degree, &pg, sh + threadIdx.x, R);
+  project_bwd_put_sh(g, cam, near_plane, eps2d, antialias, degree, &pg,
+                     [&](int j, float x) { gw[j * R] = x; },  // planes 0-10 only
+                     [&](int ch, float dc, const float* basis, int nco) {
+#pragma unroll
+                       for (int kk = 0; kk < 16; kk++)
+                         if (kk < nco) gw[(11 + kk) * R] = basis[kk];
+                       gw[(27 + ch) * R] = dc;
+                     });
   }
-  __syncthreads();  // every row's gradients are in LDS
+  __syncthreads();  // every row's gradient words are in LDS
   const uint32_t rows = ie - ib;
   constexpr uint32_t kPerPlane = R / 4;  // float4 groups per plane
   constexpr uint32_t kGroups = kParams * kPerPlane;  // float4 groups of the workgroup
@@ -722,12 +753,19 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
     if (r >= rows) continue;
     const float rate = plane_lr((int)j, lr);
     const size_t at = (size_t)j * stride + ib + r;
-    const float* gq = sh + j * R + r;
+    float4 G;
+    if (j < 11) {
+      G = *reinterpret_cast<const float4*>(sh + j * R + r);
+    } else {
+      const uint32_t kk = (j - 11) / 3, ch = (j - 11) % 3;
+      const float4 dc = *reinterpret_cast<const float4*>(sh + (27 + ch) * R + r);
+      const float4 bs = *reinterpret_cast<const float4*>(sh + (11 + kk) * R + r);
+      G = make_float4(dc.x * bs.x, dc.y * bs.y, dc.z * bs.z, dc.w * bs.w);
+    }
     if (r + 4 <= rows) {
       float4 P = load_stream(params + at);
       float4 M = load_stream(m + at);
       float4 V = load_stream(v + at);
-      const float4 G = *reinterpret_cast<const float4*>(gq);
       adam_update(P.x, G.x, M.x, V.x, rate, b1, b2, c1, c2, eps);
       adam_update(P.y, G.y, M.y, V.y, rate, b1, b2, c1, c2, eps);
       adam_update(P.z, G.z, M.z, V.z, rate, b1, b2, c1, c2, eps);
@@ -736,6 +774,7 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
       store_stream(m + at, M);
       store_stream(v + at, V);
     } else {
+      const float gq[4] = {G.x, G.y, G.z, G.w};
       for (uint32_t k = 0; r + k < rows; k++)
         adam_update(params[at + k], gq[k], m[at + k], v[at + k], rate, b1, b2, c1, c2, eps);
     }
