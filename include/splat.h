@@ -295,6 +295,63 @@ int splat_ply_write(const char* path, const float* gaussians, uint64_t n, int32_
 int splat_ply_read(const char* path, float* gaussians, uint64_t capacity, uint64_t* n,
                    int32_t* sh_degree, char* err, size_t err_len);
 
+/* ---- COLMAP sparse models (src/colmap/*.rs, src/dataset.rs:83-129,213-237), host ------
+ * Binary (.bin, preferred) or text (.txt) cameras / images / points3D in one directory;
+ * 2D observations and point tracks are parsed past and not kept. */
+typedef struct {
+  uint32_t id;
+  int32_t model;        /* COLMAP model id: 0 SIMPLE_PINHOLE, 1 PINHOLE, ... 11
+                           RAD_TAN_THIN_PRISM_FISHEYE (src/colmap/mod.rs:17-63) */
+  uint64_t width, height;
+  uint32_t num_params;  /* the model's parameter count */
+  uint32_t pad;
+  double params[16];
+} splat_colmap_camera;
+
+typedef struct {
+  uint32_t id;
+  uint32_t camera_id;
+  double qvec[4];       /* world-to-camera rotation (w, x, y, z) */
+  double tvec[3];       /* world-to-camera translation */
+  char name[256];       /* file name relative to the images directory, NUL-terminated */
+} splat_colmap_image;
+
+typedef struct {
+  uint64_t id;
+  double xyz[3];
+  uint8_t rgb[3];
+  uint8_t pad[5];
+  double error;
+} splat_colmap_point;
+
+/* read_model, src/colmap/mod.rs:198-238: counts are always set; records are written only if
+ * every capacity suffices (else SPLAT_ERR_CAPACITY).  Cameras and images in file order.
+ * Missing files: SPLAT_ERR_INVALID; unreadable: SPLAT_ERR_IO; malformed (unknown model,
+ * wrong parameter count, truncated binary, implausible counts, bad tokens, names of 256
+ * bytes or more): SPLAT_ERR_FORMAT. */
+int splat_colmap_read(const char* dir, splat_colmap_camera* cameras, uint64_t camera_capacity,
+                      uint64_t* num_cameras, splat_colmap_image* images, uint64_t image_capacity,
+                      uint64_t* num_images, splat_colmap_point* points, uint64_t point_capacity,
+                      uint64_t* num_points, char* err, size_t err_len);
+/* write_model_bin / write_model_txt (src/colmap/binary.rs:165-224, text.rs:132-190):
+ * cameras sorted by id, images without observations, points without tracks; text numbers
+ * in the shortest form that reads back exactly, without exponent. */
+int splat_colmap_write(const char* dir, int32_t binary, const splat_colmap_camera* cameras,
+                       uint64_t num_cameras, const splat_colmap_image* images,
+                       uint64_t num_images, const splat_colmap_point* points,
+                       uint64_t num_points, char* err, size_t err_len);
+/* camera_from_colmap, src/dataset.rs:83-129: PINHOLE / SIMPLE_PINHOLE intrinsics and the
+ * normalised quaternion's rotation, in binary32; other models or an unknown camera id are
+ * SPLAT_ERR_INVALID. */
+int splat_camera_from_colmap(const splat_colmap_camera* cameras, uint64_t num_cameras,
+                             const splat_colmap_image* image, splat_camera* out, char* err,
+                             size_t err_len);
+/* match_image_size, src/dataset.rs:213-237: the camera unchanged for its own size, scaled
+ * (Camera::resized, src/camera.rs:48-62) for a uniformly scaled image (both axis factors
+ * within one pixel), SPLAT_ERR_INVALID otherwise. */
+int splat_camera_match_image_size(const splat_camera* camera, uint32_t width, uint32_t height,
+                                  splat_camera* out);
+
 /* ---- checkpoints ---------------------------------------------------------------------
  * No counterpart in the reference, whose train loop saves only the model (.ply,
  * src/ply.rs:19-69, called from src/train.rs:275-279): a checkpoint adds the Adam moments
@@ -305,7 +362,8 @@ int splat_ply_read(const char* path, float* gaussians, uint64_t capacity, uint64
  * (n u32 each).  Null arrays are written as zeros (write) or skipped (read).  read: *n,
  * *adam_step and *max_sh_degree are set from the header; the arrays are filled only if
  * capacity >= *n (otherwise SPLAT_ERR_CAPACITY), so a call with null arrays and capacity 0
- * sizes the output.  A file whose length does not match its header is SPLAT_ERR_IO. */
+ * sizes the output.  A foreign file, an unknown version or a length that does not match
+ * the header is SPLAT_ERR_FORMAT. */
 int splat_checkpoint_write(const char* path, const float* gaussians, const float* m,
                            const float* v, const float* grad_accum, const uint32_t* denom,
                            const uint32_t* max_radii, uint64_t n, uint64_t adam_step,

@@ -166,6 +166,12 @@ _SIGS = {
     "splat_checkpoint_read": (i32, [C.c_char_p, P, P, P, P, P, P, u64, P, P, P, C.c_char_p,
                                     C.c_size_t]),
     "splat_trainer_restore": (i32, [P, P, P, P, P, P, P, u64, u64]),
+    "splat_colmap_read": (i32, [C.c_char_p, P, u64, P, P, u64, P, P, u64, P, C.c_char_p,
+                                C.c_size_t]),
+    "splat_colmap_write": (i32, [C.c_char_p, i32, P, u64, P, u64, P, u64, C.c_char_p,
+                                 C.c_size_t]),
+    "splat_camera_from_colmap": (i32, [P, u64, P, P, C.c_char_p, C.c_size_t]),
+    "splat_camera_match_image_size": (i32, [P, u32, u32, P]),
     "splat_trainer_evaluate": (i32, [P, P, u32, i32, i32, P, P, P]),
     "splat_trainer_create": (i32, [P, P, u64, u64, P]),
     "splat_trainer_destroy": (i32, [P]),
@@ -525,6 +531,65 @@ def checkpoint_read(path):
     if st != OK:
         raise CheckpointError(st, err.value.decode())
     return out, step.value, deg.value
+
+
+class ColmapError(SplatError):
+    pass
+
+
+def colmap_read(directory):
+    """read_model (src/colmap/mod.rs:198-238) -> (cameras, images, points) as structured
+    arrays of records.COLMAP_CAMERA / COLMAP_IMAGE / COLMAP_POINT."""
+    err = C.create_string_buffer(512)
+    nc, ni, npt = C.c_uint64(), C.c_uint64(), C.c_uint64()
+    d = os.fsencode(directory)
+    st = lib().splat_colmap_read(d, None, 0, C.byref(nc), None, 0, C.byref(ni), None, 0,
+                                 C.byref(npt), err, len(err))
+    if st not in (OK, ERR_CAPACITY):
+        raise ColmapError(st, err.value.decode())
+    cams = np.zeros(nc.value, rec.COLMAP_CAMERA)
+    imgs = np.zeros(ni.value, rec.COLMAP_IMAGE)
+    pts = np.zeros(npt.value, rec.COLMAP_POINT)
+    st = lib().splat_colmap_read(d, _p(cams), len(cams), C.byref(nc), _p(imgs), len(imgs),
+                                 C.byref(ni), _p(pts), len(pts), C.byref(npt), err, len(err))
+    if st != OK:
+        raise ColmapError(st, err.value.decode())
+    return cams, imgs, pts
+
+
+def colmap_write(directory, cameras, images, points, binary=True):
+    """write_model_bin / write_model_txt (src/colmap/binary.rs:165-224, text.rs:132-190)."""
+    c = np.ascontiguousarray(cameras, rec.COLMAP_CAMERA)
+    i = np.ascontiguousarray(images, rec.COLMAP_IMAGE)
+    p = np.ascontiguousarray(points, rec.COLMAP_POINT)
+    err = C.create_string_buffer(512)
+    st = lib().splat_colmap_write(os.fsencode(directory), int(bool(binary)), _p(c), len(c),
+                                  _p(i), len(i), _p(p), len(p), err, len(err))
+    if st != OK:
+        raise ColmapError(st, err.value.decode())
+
+
+def camera_from_colmap(cameras, image):
+    """camera_from_colmap (src/dataset.rs:83-129) -> records.CAMERA."""
+    c = np.ascontiguousarray(cameras, rec.COLMAP_CAMERA)
+    im = np.ascontiguousarray(image, rec.COLMAP_IMAGE).reshape(())
+    out = np.zeros((), rec.CAMERA)
+    err = C.create_string_buffer(512)
+    st = lib().splat_camera_from_colmap(_p(c), len(c), _p(im), _p(out), err, len(err))
+    if st != OK:
+        raise ColmapError(st, err.value.decode())
+    return out
+
+
+def match_image_size(camera, width, height):
+    """match_image_size (src/dataset.rs:213-237) -> records.CAMERA."""
+    cam = np.ascontiguousarray(camera, rec.CAMERA).reshape(())
+    out = np.zeros((), rec.CAMERA)
+    st = lib().splat_camera_match_image_size(_p(cam), width, height, _p(out))
+    if st != OK:
+        raise ColmapError(st, f"image size {width}x{height} does not match the camera's "
+                              f"{int(cam['width'])}x{int(cam['height'])}")
+    return out
 
 
 def to_rgb8(image):

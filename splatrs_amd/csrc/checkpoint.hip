@@ -90,12 +90,12 @@ int splat_checkpoint_read(const char* path, float* gaussians, float* m, float* v
   int32_t degree = 0;
   uint64_t count = 0, step = 0;
   if (std::fread(magic, 1, 8, f) != 8 || std::memcmp(magic, kMagic, 8) != 0)
-    return set_err(err, err_len, SPLAT_ERR_IO, "not a checkpoint file");
+    return set_err(err, err_len, SPLAT_ERR_FORMAT, "not a checkpoint file");
   if (!get(f, &version, 4, 1) || version != kVersion)
-    return set_err(err, err_len, SPLAT_ERR_IO, "unsupported checkpoint version");
+    return set_err(err, err_len, SPLAT_ERR_FORMAT, "unsupported checkpoint version");
   if (!get(f, &degree, 4, 1) || !get(f, &count, 8, 1) || !get(f, &step, 8, 1) || degree < 0 ||
       degree > 3)
-    return set_err(err, err_len, SPLAT_ERR_IO, "malformed checkpoint header");
+    return set_err(err, err_len, SPLAT_ERR_FORMAT, "malformed checkpoint header");
   // the file must hold exactly the arrays the header announces
   const uint64_t body = count * (3ull * kParams * 4 + 12);
   const long here = std::ftell(f);
@@ -103,7 +103,7 @@ int splat_checkpoint_read(const char* path, float* gaussians, float* m, float* v
     return set_err(err, err_len, SPLAT_ERR_IO, "cannot size checkpoint");
   const long end = std::ftell(f);
   if (end < here || (uint64_t)(end - here) != body)
-    return set_err(err, err_len, SPLAT_ERR_IO, "truncated or oversized checkpoint");
+    return set_err(err, err_len, SPLAT_ERR_FORMAT, "truncated or oversized checkpoint");
   std::fseek(f, here, SEEK_SET);
   *n = count;
   if (adam_step) *adam_step = step;

@@ -65,6 +65,19 @@ STEP_PARAMS = np.dtype([
 STEP_RESULT = np.dtype([("loss", "<f4"), ("pad_", "<u4"), ("num_entries", "<u8"),
                         ("num_visible", "<u8")])
 
+COLMAP_CAMERA = np.dtype([("id", "<u4"), ("model", "<i4"), ("width", "<u8"), ("height", "<u8"),
+                          ("num_params", "<u4"), ("pad", "<u4"), ("params", "<f8", (16,))])
+COLMAP_IMAGE = np.dtype([("id", "<u4"), ("camera_id", "<u4"), ("qvec", "<f8", (4,)),
+                         ("tvec", "<f8", (3,)), ("name", "S256")])
+COLMAP_POINT = np.dtype([("id", "<u8"), ("xyz", "<f8", (3,)), ("rgb", "u1", (3,)),
+                         ("pad", "u1", (5,)), ("error", "<f8")])
+# COLMAP camera models in id order with their parameter counts (src/colmap/mod.rs:17-135)
+COLMAP_MODELS = [("SIMPLE_PINHOLE", 3), ("PINHOLE", 4), ("SIMPLE_RADIAL", 4), ("RADIAL", 5),
+                 ("OPENCV", 8), ("OPENCV_FISHEYE", 8), ("FULL_OPENCV", 12), ("FOV", 5),
+                 ("SIMPLE_RADIAL_FISHEYE", 4), ("RADIAL_FISHEYE", 5),
+                 ("THIN_PRISM_FISHEYE", 12), ("RAD_TAN_THIN_PRISM_FISHEYE", 16)]
+assert COLMAP_CAMERA.itemsize == 160 and COLMAP_IMAGE.itemsize == 320
+assert COLMAP_POINT.itemsize == 48
 assert CAMERA.itemsize == 72 and PROJ.itemsize == 60 and PROJ_GRAD.itemsize == 36
 assert STEP_PARAMS.itemsize == 68 and STEP_RESULT.itemsize == 24
 assert ADAM_STATE.itemsize == 24 and DENSIFY_CONFIG.itemsize == 32

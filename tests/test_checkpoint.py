@@ -88,7 +88,7 @@ def test_malformed_files_are_errors(tmp_path, damage):
     p.write_bytes(bytes(raw))
     with pytest.raises(abi.CheckpointError) as e:
         abi.checkpoint_read(p)
-    assert e.value.status == abi.ERR_IO
+    assert e.value.status == abi.ERR_FORMAT
 
 
 def test_invalid_arguments(tmp_path):
