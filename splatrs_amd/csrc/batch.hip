@@ -252,7 +252,7 @@ void enqueue(splat_batch* b, const BatchBufs& x, const splat_batch_step_params& 
              const RasterVariant& fwd, const RasterVariant& bwd) {
   splat_ctx* ctx = b->ctx;
   const BatchDev& bd = b->bd;
-  project_forward_batch(ctx, b->params.as<float>(), b->stride, bd, b->work, p.near_plane,
+  project_forward_batch(ctx, b->params.as<float>(), b->stride, bd, b->rows, p.near_plane,
                         p.antialias, p.sh_degree, x.pb);
   bin_build_radix_device(ctx, x.pb, b->work, x.bb, b->entry_cap, x.rs, x.small, x.total64,
                          x.scan_sums, x.status, &bd);
@@ -489,7 +489,7 @@ int splat_batch_tune(splat_batch* b, const uint32_t* views, uint32_t nviews,
     for (;;) {
       x = reserve_bufs(b);
       clear_status(b, x.status);
-      project_forward_batch(ctx, b->params.as<float>(), b->stride, b->bd, b->work, p->near_plane,
+      project_forward_batch(ctx, b->params.as<float>(), b->stride, b->bd, b->rows, p->near_plane,
                             p->antialias, p->sh_degree, x.pb);
       bin_build_radix_device(ctx, x.pb, b->work, x.bb, b->entry_cap, x.rs, x.small, x.total64,
                              x.scan_sums, x.status, &b->bd);

@@ -381,13 +381,14 @@ void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size
                const float* slots, const StatsArgs& sa, float c1, float c2,
                const splat_adam_state& st, const splat_learning_rates& lr,
                const StepDyn* dyn = nullptr, const StepStatus* status = nullptr);
-// Batched: projections of every (scene, view) pair (total `work` rows), and the backward
+// Batched: projections of every (scene, view) pair into the work rows (one lane per scene
+// row, `rows` plane rows in total, each read once for all of its scene's views), and the backward
 // tail + Adam of every scene Gaussian (total `rows` plane rows, a multiple of 64): the
 // projection backward of each pair's summed slots (dproj, one per work row) summed over
 // the scene's views in step order, then Adam with the scene's position learning rate;
 // c1, c2, lr come from dyn.
 void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,
-                           const BatchDev& bd, size_t work, float near_plane, int antialias,
+                           const BatchDev& bd, size_t rows, float near_plane, int antialias,
                            int degree, ProjBuffers pb);
 void grad_adam_batch(splat_ctx* ctx, float* params, float* m, float* v, size_t stride,
                      const BatchDev& bd, size_t rows, float near_plane, int antialias, int degree,

@@ -288,11 +288,8 @@ __device__ __forceinline__ void raster_record(float mx, float my, float A, float
 }
 
 // project_one (src/project.rs:277-330) of parameter row pi into projection row i.
-__device__ __forceinline__ void project_row(const float* __restrict__ params, size_t stride,
-                                            size_t pi, const CameraDev& cam, float near_plane,
-                                            float eps2d, int antialias, int degree,
-                                            const ProjBuffers& pb, size_t i) {
-  float g[kParams];
+__device__ __forceinline__ void load_row(const float* __restrict__ params, size_t stride,
+                                         size_t pi, int degree, float g[kParams]) {
   const int ncoef = (degree + 1) * (degree + 1);
 #pragma unroll
   for (int j = 0; j < kParams; j++) {
@@ -301,6 +298,12 @@ __device__ __forceinline__ void project_row(const float* __restrict__ params, si
     g[j] = (j < 11 + 3 * ncoef) ? __builtin_nontemporal_load(params + (size_t)j * stride + pi)
                                 : 0.0f;
   }
+}
+
+// project_one of the parameters g into projection row i.
+__device__ __forceinline__ void project_g(const float g[kParams], const CameraDev& cam,
+                                          float near_plane, float eps2d, int antialias,
+                                          int degree, const ProjBuffers& pb, size_t i) {
   Core k;
   bool ok = core(g, cam, near_plane, eps2d, antialias != 0, degree, k);
   uint32_t tiles_x = (cam.width + kTile - 1) / kTile, tiles_y = (cam.height + kTile - 1) / kTile;
@@ -340,6 +343,15 @@ __device__ __forceinline__ void project_row(const float* __restrict__ params, si
   pb.ntiles[i] = (tx1 - tx0) * (ty1 - ty0);
 }
 
+__device__ __forceinline__ void project_row(const float* __restrict__ params, size_t stride,
+                                            size_t pi, const CameraDev& cam, float near_plane,
+                                            float eps2d, int antialias, int degree,
+                                            const ProjBuffers& pb, size_t i) {
+  float g[kParams];
+  load_row(params, stride, pi, degree, g);
+  project_g(g, cam, near_plane, eps2d, antialias, degree, pb, i);
+}
+
 __global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restrict__ params,
                                                           uint32_t n, uint32_t stride,
                                                           CameraDev cam, float near_plane,
@@ -354,16 +366,26 @@ __global__ void __launch_bounds__(256) project_fwd_kernel(const float* __restric
 
 // All (scene, view) pairs of a batched step: row w of the projection buffers is Gaussian
 // w - work_base[v] of pair v's scene seen through pair v's camera.
+// Batched: one lane per scene row (scene rows start at multiples of 64, so a wave's rows
+// share a scene); the row's parameters are read once and projected for each of its scene's
+// views into that view's work row.
 __global__ void __launch_bounds__(256) project_fwd_batch_kernel(const float* __restrict__ params,
                                                                 uint32_t stride, BatchDev bd,
-                                                                uint32_t work, float near_plane,
+                                                                uint32_t rows, float near_plane,
                                                                 float eps2d, int antialias,
                                                                 int degree, ProjBuffers pb) {
-  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= work) return;
-  const uint32_t v = find_segment(bd.work_base, bd.npairs, w);
-  const uint32_t pi = bd.scene_off[bd.pair_scene[v]] + (w - bd.work_base[v]);
-  project_row(params, stride, pi, bd.cams[v], near_plane, eps2d, antialias, degree, pb, w);
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= rows) return;
+  const uint32_t s = find_segment(bd.scene_off, bd.nscenes, p);
+  const uint32_t i = p - bd.scene_off[s];
+  const uint32_t k0 = bd.scene_pair_begin[s], k1 = bd.scene_pair_begin[s + 1];
+  if (i >= bd.scene_n[s] || k0 == k1) return;  // padding, or a scene without views
+  float g[kParams];
+  load_row(params, stride, p, degree, g);
+  for (uint32_t k = k0; k < k1; k++) {
+    const uint32_t v = bd.scene_pairs[k];
+    project_g(g, bd.cams[v], near_plane, eps2d, antialias, degree, pb, bd.work_base[v] + i);
+  }
 }
 
 // ProjBuffers -> Proj records (src/project.rs:47-64).
@@ -931,10 +953,10 @@ void grad_adam(splat_ctx* ctx, float* params, float* m, float* v, size_t n, size
 }
 
 void project_forward_batch(splat_ctx* ctx, const float* params, size_t stride,
-                           const BatchDev& bd, size_t work, float near_plane, int antialias,
+                           const BatchDev& bd, size_t rows, float near_plane, int antialias,
                            int degree, ProjBuffers pb) {
-  launch(ctx, "project_fwd", project_fwd_batch_kernel, dim3(blocks_for(work, 256)), dim3(256), 0,
-         params, (uint32_t)stride, bd, (uint32_t)work, near_plane, 0.3f, antialias, degree, pb);
+  launch(ctx, "project_fwd", project_fwd_batch_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0,
+         params, (uint32_t)stride, bd, (uint32_t)rows, near_plane, 0.3f, antialias, degree, pb);
 }
 
 void grad_adam_batch(splat_ctx* ctx, float* params, float* m, float* v, size_t stride,
