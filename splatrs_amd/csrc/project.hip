@@ -709,16 +709,24 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
   const uint32_t ib = blockIdx.x * blockDim.x, ie = min(ib + blockDim.x, n);
   const uint32_t span0 = entry_offset[ib], span1 = entry_offset[ie];
   const uint32_t e0 = i < n ? entry_offset[i] : 0u, e1 = i < n ? entry_offset[i + 1] : 0u;
+  // words per entry: the backward writes the absolute sums (slots 9, 10) exactly when the
+  // step accumulates AbsGS statistics
+  const bool abs = sa.grad_accum && sa.absgrad;
+  const uint32_t ns = abs ? kSlots : 9;
   for (uint32_t c0 = span0; c0 < span1; c0 += kChunk) {
     const uint32_t c1e = min(c0 + kChunk, span1);
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (c1e - c0) * kSlots; q += blockDim.x)
-      sh[q] = slots[(size_t)c0 * kSlots + q];
+    for (uint32_t q = threadIdx.x; q < (c1e - c0) * ns; q += blockDim.x)
+      sh[q] = slots[(size_t)c0 * ns + q];
     __syncthreads();
     for (uint32_t e = max(e0, c0); e < min(e1, c1e); e++) {
-      const float* q = sh + (e - c0) * kSlots;
+      const float* q = sh + (e - c0) * ns;
 #pragma unroll
-      for (int k = 0; k < kSlots; k++) sum[k] += q[k];
+      for (int k = 0; k < 9; k++) sum[k] += q[k];
+      if (abs) {
+        sum[9] += q[9];
+        sum[10] += q[10];
+      }
     }
   }
   __syncthreads();  // the staging buffer now holds gradients

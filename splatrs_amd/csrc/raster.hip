@@ -418,6 +418,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   __shared__ uint32_t s_tile;
   // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums.
   __shared__ float red[ORDERED ? B * kSlots * 256 : B * kSlots * 16];
+  // words per entry in `slots`: the absolute sums (slots 9, 10) only when asked for
+  constexpr uint32_t SS = ABS ? kSlots : 9;
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
   for (;;) {
   const uint32_t tile = next_tile<PERSIST>(counter, &s_tile);
@@ -456,9 +458,9 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   __syncthreads();
   const uint32_t kmax = min(smax_last, count);
   // Entries no pixel reaches get zero slots.
-  for (uint32_t q = kmax * kSlots + threadIdx.x; q < count * kSlots; q += blockDim.x) {
-    const uint32_t k = q / kSlots;
-    slots[(size_t)sorted_entry[start + k] * kSlots + q % kSlots] = 0.0f;
+  for (uint32_t q = kmax * SS + threadIdx.x; q < count * SS; q += blockDim.x) {
+    const uint32_t k = q / SS;
+    slots[(size_t)sorted_entry[start + k] * SS + q % SS] = 0.0f;
   }
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   // the last contributor of any pixel of this wave
@@ -524,8 +526,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     }
     __syncthreads();
     // Reduce the batch and write its slots.
-    for (uint32_t q = threadIdx.x; q < cnt * kSlots; q += blockDim.x) {
-      const uint32_t kk = q / kSlots, slot = q % kSlots;
+    for (uint32_t q = threadIdx.x; q < cnt * SS; q += blockDim.x) {
+      const uint32_t kk = q / SS, slot = q % SS;
       float acc = 0.0f;
       if (ORDERED) {
         const float* p = &red[(kk * kSlots + slot) * 256];
@@ -548,7 +550,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
           if (slot == 2 || slot == 4) acc *= 0.5f;
         }
       }
-      slots[(size_t)sorted_entry[start + lo + kk] * kSlots + slot] = acc;
+      slots[(size_t)sorted_entry[start + lo + kk] * SS + slot] = acc;
     }
   }
   if (!PERSIST) break;
@@ -559,7 +561,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
 // wave per 64 / NS consecutive rows, one lane per (row, slot): the rows' entries form one
 // span, staged through LDS in windows with coalesced loads, and each lane adds its slot of
 // its row's entries in order, so every sum keeps the reference's sequential order.  NS is
-// 9, or 11 with the absolute sums (slots 9 and 10, into abs_grads).
+// the record width the backward wrote: 9, or 11 with the absolute sums (slots 9 and 10,
+// into abs_grads).
 constexpr uint32_t kReduceWindow = 256;  // entries staged per window
 
 template <int NS>
@@ -580,8 +583,8 @@ __global__ void __launch_bounds__(64) reduce_slots_kernel(const uint32_t* __rest
   const uint32_t e0 = live ? entry_offset[row] : 0u, e1 = live ? entry_offset[row + 1] : 0u;
   float sum = 0.0f;
   for (uint32_t c0 = span0; c0 < span1; c0 += kReduceWindow) {
-    const uint32_t words = (min(c0 + kReduceWindow, span1) - c0) * kSlots;
-    const float* src = slots + (size_t)c0 * kSlots;
+    const uint32_t words = (min(c0 + kReduceWindow, span1) - c0) * NS;
+    const float* src = slots + (size_t)c0 * NS;
     __syncthreads();
     for (uint32_t q0 = 0; q0 < words; q0 += 64 * 8) {
       float t[8];
@@ -598,7 +601,7 @@ __global__ void __launch_bounds__(64) reduce_slots_kernel(const uint32_t* __rest
     }
     __syncthreads();
     const uint32_t hi = min(e1, c0 + kReduceWindow);
-    for (uint32_t e = max(e0, c0); e < hi; e++) sum += sh[(e - c0) * kSlots + k];
+    for (uint32_t e = max(e0, c0); e < hi; e++) sum += sh[(e - c0) * NS + k];
   }
   if (!live) return;
   // splat_proj_grad holds slots 0-8 in slot order
@@ -669,10 +672,15 @@ void bwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
               const float* dimage, bool ordered, float* slots, bool need_abs, const BatchDev& bd,
               uint32_t* counter) {
   const unsigned pc = rv.per_cu;
-  if (ordered)
+  // the record width (9 or 11 words per entry) follows need_abs in both modes
+  if (ordered && need_abs)
     return bwd_go<true, true, 8, BATCH, PERSIST>(ctx, "raster_bwd_ordered", pb, bb, T, width,
                                                  height, bg, final_t, n_contrib, dimage, slots,
                                                  bd, counter, pc);
+  if (ordered)
+    return bwd_go<true, false, 8, BATCH, PERSIST>(ctx, "raster_bwd_ordered", pb, bb, T, width,
+                                                  height, bg, final_t, n_contrib, dimage, slots,
+                                                  bd, counter, pc);
   if (rv.bwd_batch == 16) {
     if (need_abs)
       return bwd_go<false, true, 16, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
