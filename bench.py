@@ -35,9 +35,10 @@ def algorithmic_bytes(name, N, V, E, P):
         "project_bwd": 508 * N,              # 236 params + 36 ProjGrad read; 236 grads written
         "adam": 1652 * N,                    # p, g, m, v read; p, m, v written (59 x 4 B each)
         "raster_fwd": 52 * E + 20 * P,       # 48 B record + 4 B id per entry; image, T, n
-        "raster_bwd": 100 * E + 20 * P,      # record + 2 ids per entry, 44 B slots; dL, T, n
-        "raster_bwd_ordered": 100 * E + 20 * P,
-        "reduce_slots": 44 * E + 44 * N,     # slots read; ProjGrad + abs written
+        # record + 2 ids per entry, 36 B slot record (44 B with the AbsGS sums); dL, T, n
+        "raster_bwd": 92 * E + 20 * P,
+        "raster_bwd_ordered": 92 * E + 20 * P,
+        "reduce_slots": 36 * E + 36 * N,     # slots read; ProjGrad written
         "loss_ssim_stats": 51 * P,           # pred 12 + target 3 read; 3 partial maps written
         "loss_grad": 63 * P,                 # 3 maps + pred + target read; dL/dimage written
         "bin_keygen": 12 * E + 16 * N,
@@ -45,7 +46,7 @@ def algorithmic_bytes(name, N, V, E, P):
         "bin_sort_small": 24 * E,
         "stats": 24 * N,
         # slots + params + both moments read; params and moments written (no gradient planes)
-        "grad_adam": 44 * E + 1416 * N,
+        "grad_adam": 36 * E + 1416 * N,
     }
     return table.get(name)
 
