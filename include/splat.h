@@ -295,7 +295,7 @@ int splat_ply_write(const char* path, const float* gaussians, uint64_t n, int32_
 int splat_ply_read(const char* path, float* gaussians, uint64_t capacity, uint64_t* n,
                    int32_t* sh_degree, char* err, size_t err_len);
 
-/* ---- COLMAP sparse models (src/colmap/*.rs, src/dataset.rs:83-129,213-237), host ------
+/* ---- COLMAP sparse models (src/colmap/{mod,binary,text}.rs, src/dataset.rs:83-129,213-237) --
  * Binary (.bin, preferred) or text (.txt) cameras / images / points3D in one directory;
  * 2D observations and point tracks are parsed past and not kept. */
 typedef struct {
