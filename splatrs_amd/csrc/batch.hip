@@ -224,6 +224,9 @@ BatchBufs reserve_bufs(splat_batch* b) {
   x.bb.bucket = nullptr;
   x.bb.block_marks = (uint32_t*)st["block_marks"].reserve((b->entry_cap / 256 + 2) * 4);
   x.bb.tile_start = (uint32_t*)st["tile_start"].reserve(T * 4 + 8);
+  // Tiles run in index order: neighbouring tiles of one view share Gaussian records in L2,
+  // which the persistent backward loses (+18%) when the largest tiles go first.
+  x.bb.tile_order = nullptr;
   x.bb.entry_gaussian = nullptr;  // not needed: the tile sort carries each entry's Gaussian
   x.bb.sorted_entry = (uint32_t*)st["sorted_entry"].reserve(ec * 4);
   x.bb.sorted_gauss = (uint32_t*)st["sorted_gauss"].reserve(ec * 4);

@@ -473,6 +473,48 @@ void tile_starts(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint32_t* total
   exclusive_scan_u32(ctx, totals, bb.tile_start, T, total_dev, scan_sums);
 }
 
+// Dispatch order of the compositing kernels: tiles by decreasing entry count, in
+// kOrderBuckets buckets relative to the largest tile (order within a bucket unspecified),
+// so the largest tiles start first and the smallest fill the end of the launch.  One
+// workgroup; the result is a permutation of [0, T) whatever the counts.
+constexpr uint32_t kOrderBuckets = 64;
+
+__global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ tile_start,
+                                                          uint32_t T, uint32_t* __restrict__ order) {
+  __shared__ uint32_t smax, hist[kOrderBuckets], cursor[kOrderBuckets];
+  const uint32_t t = threadIdx.x;
+  if (t == 0) smax = 0;
+  if (t < kOrderBuckets) hist[t] = 0;
+  __syncthreads();
+  uint32_t m = 0;
+  for (uint32_t q = t; q < T; q += blockDim.x) m = max(m, tile_start[q + 1] - tile_start[q]);
+  for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
+  if ((t & 63u) == 0) atomicMax(&smax, m);
+  __syncthreads();
+  const uint64_t mx = smax;
+  const auto bucket = [&](uint32_t q) {
+    const uint64_t c = tile_start[q + 1] - tile_start[q];
+    return (uint32_t)(((mx - c) * kOrderBuckets) / (mx + 1));  // 0: the largest tiles
+  };
+  for (uint32_t q = t; q < T; q += blockDim.x) atomicAdd(&hist[bucket(q)], 1u);
+  __syncthreads();
+  if (t == 0) {
+    uint32_t run = 0;
+    for (uint32_t b = 0; b < kOrderBuckets; b++) {
+      cursor[b] = run;
+      run += hist[b];
+    }
+  }
+  __syncthreads();
+  for (uint32_t q = t; q < T; q += blockDim.x) order[atomicAdd(&cursor[bucket(q)], 1u)] = q;
+}
+
+void tile_order(splat_ctx* ctx, const BinBuffers& bb, size_t T) {
+  if (!bb.tile_order || T == 0) return;
+  launch(ctx, "bin_tile_order", tile_order_kernel, dim3(1), dim3(1024), 0,
+         (const uint32_t*)bb.tile_start, (uint32_t)T, bb.tile_order);
+}
+
 }  // namespace
 
 size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uint32_t tiles_y,
@@ -485,6 +527,7 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   bb.tile_count = (uint32_t*)store["tile_count"].reserve(kTileSub * T * 4);
   bb.tile_sub = (uint32_t*)store["tile_sub"].reserve((kTileSub + 1) * T * 4);
   bb.tile_start = (uint32_t*)store["tile_start"].reserve((T + 1) * 4);
+  bb.tile_order = (uint32_t*)store["tile_order"].reserve(T * 4 + 4);
   uint32_t* small = (uint32_t*)store["bin_small"].reserve(64);  // [0] E, [1] big count, [2] T
   unsigned long long* total64 = (unsigned long long*)store["bin_total64"].reserve(8);
   uint32_t* big_list = (uint32_t*)store["bin_big_list"].reserve(T * 4);
@@ -552,6 +595,7 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
          (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
          bb.sorted_entry, bb.sorted_gauss, keys_out, (const uint32_t*)big_list,
          (const uint32_t*)(small + 1));
+  tile_order(ctx, bb, T);
   return E;
 }
 
@@ -600,6 +644,7 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
          (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
          bb.sorted_entry, bb.sorted_gauss, (uint64_t*)nullptr, (const uint32_t*)big_list,
          (const uint32_t*)(small + 1));
+  tile_order(ctx, bb, T);
 }
 
 
@@ -680,6 +725,7 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
          direct ? nullptr : (const uint32_t*)rs.eg[c2],
          e_host, (uint32_t)T, (const float*)pb.depth, bb.sorted_entry,
          bb.sorted_gauss, bb.tile_start, keys_out, (const StepStatus*)status);
+  tile_order(ctx, bb, T);
 }
 
 }  // namespace splat

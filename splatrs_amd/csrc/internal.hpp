@@ -343,6 +343,7 @@ struct BinBuffers {
   uint32_t* sorted_entry;    // E: entry ids in (tile, depth, entry) order
   uint32_t* sorted_gauss;    // E: their Gaussian ids
   uint32_t tiles_x, tiles_y;
+  uint32_t* tile_order = nullptr;  // T: compositing dispatch order (null: tile index order)
 };
 
 // ---- kernel launchers (one per kernel file) -------------------------------------------

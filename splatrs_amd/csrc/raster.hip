@@ -59,13 +59,22 @@ __device__ __forceinline__ TileGeom tile_geom(uint32_t tile, uint32_t tiles_x, u
 
 // The workgroup's next tile: its block index, or (persistent launch) the next one handed
 // out by a shared counter.  Every lane sees the same tile.
+// With a dispatch order (tiles by decreasing entry count) the t-th tile taken is order[t];
+// a tile's results do not depend on when it runs.
 template <bool PERSIST>
-__device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* s_tile) {
-  if (!PERSIST) return blockIdx.x;
-  __syncthreads();  // the previous tile's LDS use is over
-  if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1u);
-  __syncthreads();
-  return *s_tile;
+__device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* s_tile,
+                                              const uint32_t* __restrict__ order,
+                                              uint32_t num_tiles) {
+  uint32_t t;
+  if (!PERSIST) {
+    t = blockIdx.x;
+  } else {
+    __syncthreads();  // the previous tile's LDS use is over
+    if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1u);
+    __syncthreads();
+    t = *s_tile;
+  }
+  return order && t < num_tiles ? order[t] : t;
 }
 
 // Tile-local pixel range of one entry (the clip closure of src/rasterize.rs:158-164).
@@ -129,6 +138,7 @@ __device__ __forceinline__ bool evaluate(float4 r0, float4 r1, float px, float p
 template <int B, bool BATCH, bool PERSIST>
 __global__ void __launch_bounds__(256) raster_fwd_kernel(
     const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
+    const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ sorted_gauss, uint32_t num_tiles, uint32_t tiles_x, uint32_t W,
     uint32_t H, float bg0, float bg1, float bg2, float* __restrict__ image,
     float* __restrict__ final_t, uint32_t* __restrict__ n_contrib, BatchDev bd,
@@ -139,7 +149,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
   __shared__ uint32_t s_tile;
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
   for (;;) {
-  const uint32_t tile = next_tile<PERSIST>(counter, &s_tile);
+  const uint32_t tile = next_tile<PERSIST>(counter, &s_tile, order, num_tiles);
   if (tile >= num_tiles) break;
   const TileGeom tg = tile_geom<BATCH>(tile, tiles_x, W, H, bd);
   const TileInfo ti = tg.ti;
@@ -405,6 +415,7 @@ __device__ __forceinline__ float row_allsum16(float v) {
 template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST>
 __global__ void __launch_bounds__(256) raster_bwd_kernel(
     const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
+    const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ sorted_gauss, const uint32_t* __restrict__ sorted_entry,
     uint32_t num_tiles, uint32_t tiles_x, uint32_t W, uint32_t H, float bg0, float bg1,
     float bg2, const float* __restrict__ final_t, const uint32_t* __restrict__ n_contrib,
@@ -422,7 +433,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   constexpr uint32_t SS = ABS ? kSlots : 9;
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
   for (;;) {
-  const uint32_t tile = next_tile<PERSIST>(counter, &s_tile);
+  const uint32_t tile = next_tile<PERSIST>(counter, &s_tile, order, num_tiles);
   if (tile >= num_tiles) break;
   if (threadIdx.x == 0) smax_last = 0;
   const TileGeom tg = tile_geom<BATCH>(tile, tiles_x, W, H, bd);
@@ -629,9 +640,9 @@ void fwd_go(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, unsigne
   if (PERSIST) fill_u32(ctx, counter, 1, 0u);
   const unsigned grid = PERSIST ? persistent_grid(ctx, per_cu, T) : T;
   launch(ctx, "raster_fwd", raster_fwd_kernel<B, BATCH, PERSIST>, dim3(grid), dim3(256), 0,
-         (const float4*)pb.rec, (const uint32_t*)bb.tile_start, (const uint32_t*)bb.sorted_gauss,
-         T, bb.tiles_x, width, height, bg[0], bg[1], bg[2], image, final_t, n_contrib, bd,
-         counter);
+         (const float4*)pb.rec, (const uint32_t*)bb.tile_start, (const uint32_t*)bb.tile_order,
+         (const uint32_t*)bb.sorted_gauss, T, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
+         image, final_t, n_contrib, bd, counter);
 }
 
 template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST>
@@ -643,7 +654,8 @@ void bwd_go(splat_ctx* ctx, const char* name, const ProjBuffers& pb, const BinBu
   const unsigned grid = PERSIST ? persistent_grid(ctx, per_cu, T) : T;
   launch(ctx, name, raster_bwd_kernel<ORDERED, ABS, BF, BATCH, PERSIST>, dim3(grid), dim3(256),
          0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start,
-         (const uint32_t*)bb.sorted_gauss, (const uint32_t*)bb.sorted_entry, T, bb.tiles_x, width,
+         (const uint32_t*)bb.tile_order, (const uint32_t*)bb.sorted_gauss,
+         (const uint32_t*)bb.sorted_entry, T, bb.tiles_x, width,
          height, bg[0], bg[1], bg[2], final_t, n_contrib, dimage, slots, bd, counter);
 }
 

@@ -232,6 +232,7 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   bb.tile_sub = (uint32_t*)st["tile_sub"].reserve((kTileSub + 1) * T * 4);
   bb.block_marks = (uint32_t*)st["block_marks"].reserve((ec / 256 + 2) * 4);
   bb.tile_start = (uint32_t*)st["tile_start"].reserve((T + 1) * 4);
+  bb.tile_order = (uint32_t*)st["tile_order"].reserve(T * 4 + 4);
   bb.entry_gaussian = (uint32_t*)st["entry_gaussian"].reserve(ec * 4);
   bb.entry_tile = (uint32_t*)st["entry_tile"].reserve(ec * 4);
   bb.entry_slot = (uint32_t*)st["entry_slot"].reserve(ec * 4);
