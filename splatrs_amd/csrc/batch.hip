@@ -236,7 +236,7 @@ BatchBufs reserve_bufs(splat_batch* b) {
   x.rs = radix_scratch(st, w, ec);
   x.scan_sums = (uint32_t*)st["scan_sums"].reserve(
       scan_temp_bytes(std::max({w, T, radix_count_words(std::max(w, ec))})));
-  x.counter = (uint32_t*)st["counter"].reserve(16);
+  x.counter = (uint32_t*)st["counter"].reserve(64);
   x.image = (float*)st["image"].reserve(px * 12 + 4);
   x.final_t = (float*)st["final_t"].reserve(px * 4 + 4);
   x.n_contrib = (uint32_t*)st["n_contrib"].reserve(px * 4 + 4);

@@ -59,6 +59,32 @@ __device__ __forceinline__ TileGeom tile_geom(uint32_t tile, uint32_t tiles_x, u
 
 // The workgroup's next tile: its block index, or (persistent launch) the next one handed
 // out by a shared counter.  Every lane sees the same tile.
+constexpr uint32_t kXcds = 8;  // accelerator complex dies, each with its own L2
+
+// The die the calling wave runs on (hardware register XCC_ID).
+__device__ __forceinline__ uint32_t xcc_id() {
+  uint32_t x;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+  return x % kXcds;
+}
+
+// Persistent launches: die x takes, from its own counter, the tiles of its eighth of
+// [0, T) in index order (neighbouring tiles share Gaussian records in that die's L2), then
+// whatever the other eighths have left.  Counters only grow, so every range drains and the
+// call returns T once all are taken.  Which die a wave reports only moves work between
+// dies; every tile is still taken exactly once.
+__device__ __forceinline__ uint32_t take_tile(uint32_t* counter, uint32_t T) {
+  const uint32_t x = xcc_id();
+  for (uint32_t k = 0; k < kXcds; k++) {
+    const uint32_t r = (x + k) % kXcds;
+    const uint32_t lo = (uint32_t)((uint64_t)T * r / kXcds);
+    const uint32_t hi = (uint32_t)((uint64_t)T * (r + 1) / kXcds);
+    const uint32_t t = atomicAdd(&counter[r], 1u);
+    if (t < hi - lo) return lo + t;
+  }
+  return T;
+}
+
 // With a dispatch order (tiles by decreasing entry count) the t-th tile taken is order[t];
 // a tile's results do not depend on when it runs.
 template <bool PERSIST>
@@ -70,7 +96,7 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t* counter, uint32_t* s_til
     t = blockIdx.x;
   } else {
     __syncthreads();  // the previous tile's LDS use is over
-    if (threadIdx.x == 0) *s_tile = atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) *s_tile = take_tile(counter, num_tiles);
     __syncthreads();
     t = *s_tile;
   }
@@ -637,7 +663,7 @@ template <int B, bool BATCH, bool PERSIST>
 void fwd_go(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, unsigned T,
             uint32_t width, uint32_t height, const float bg[3], float* image, float* final_t,
             uint32_t* n_contrib, const BatchDev& bd, uint32_t* counter, unsigned per_cu) {
-  if (PERSIST) fill_u32(ctx, counter, 1, 0u);
+  if (PERSIST) fill_u32(ctx, counter, kXcds, 0u);
   const unsigned grid = PERSIST ? persistent_grid(ctx, per_cu, T) : T;
   launch(ctx, "raster_fwd", raster_fwd_kernel<B, BATCH, PERSIST>, dim3(grid), dim3(256), 0,
          (const float4*)pb.rec, (const uint32_t*)bb.tile_start, (const uint32_t*)bb.tile_order,
@@ -650,7 +676,7 @@ void bwd_go(splat_ctx* ctx, const char* name, const ProjBuffers& pb, const BinBu
             unsigned T, uint32_t width, uint32_t height, const float bg[3], const float* final_t,
             const uint32_t* n_contrib, const float* dimage, float* slots, const BatchDev& bd,
             uint32_t* counter, unsigned per_cu) {
-  if (PERSIST) fill_u32(ctx, counter, 1, 0u);
+  if (PERSIST) fill_u32(ctx, counter, kXcds, 0u);
   const unsigned grid = PERSIST ? persistent_grid(ctx, per_cu, T) : T;
   launch(ctx, name, raster_bwd_kernel<ORDERED, ABS, BF, BATCH, PERSIST>, dim3(grid), dim3(256),
          0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start,

@@ -222,7 +222,7 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   P.bwd_batch = t->bwd_shape.bwd_batch;
   P.bwd_persistent = t->bwd_shape.persistent;
   P.bwd_per_cu = (int32_t)t->bwd_shape.per_cu;
-  P.counter = (uint32_t*)st["raster_counter"].reserve(16);
+  P.counter = (uint32_t*)st["raster_counter"].reserve(64);
   P.pb = proj_buffers(t);
   BinBuffers& bb = P.bb;
   bb.tiles_x = tx;
@@ -656,7 +656,7 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
     float* image = (float*)t->store["image"].reserve(npx * 12 + 4);
     float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
     uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
-    uint32_t* counter = (uint32_t*)t->store["raster_counter"].reserve(16);
+    uint32_t* counter = (uint32_t*)t->store["raster_counter"].reserve(64);
     raster_forward(ctx, pb, bb, W, H, p->background, image, final_t, n_contrib, t->fwd_shape,
                    nullptr, counter);
 
@@ -794,7 +794,7 @@ int splat_trainer_tune(splat_trainer* t, uint32_t view, const splat_step_params*
     float* dimage = (float*)t->store["dimage"].reserve(npx * 12 + 4);
     float* scratch = (float*)t->store["loss_scratch"].reserve(loss_scratch_floats(W, H) * 4);
     float* slots = (float*)t->store["slots"].reserve((std::max(E, t->entry_cap) + 1) * kSlots * 4);
-    uint32_t* counter = (uint32_t*)t->store["raster_counter"].reserve(16);
+    uint32_t* counter = (uint32_t*)t->store["raster_counter"].reserve(64);
     raster_forward(ctx, pb, bb, W, H, p->background, image, final_t, n_contrib);
     loss_forward_backward(ctx, image, nullptr, tv.rgb.as<uint8_t>(), W, H, p->lambda_dssim, true,
                           dimage, scratch, nullptr);
