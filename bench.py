@@ -365,12 +365,19 @@ def main():
             if "valu_insts" in t:
                 # The compositing kernels are bound by instruction issue, not by HBM: wave64
                 # VALU instructions per launch (rocprofv3 SQ_INSTS_VALU on this workload) per
-                # second of this run, against one instruction per CU per clock (256 CUs at
-                # 2.4 GHz, the non-packed FP32 rate of the 157.3 TFLOP/s peak).
+                # second of this run, against one instruction per SIMD per quad-cycle (256 CUs
+                # x 4 SIMDs at 2.4 GHz, the non-packed FP32 rate of the 157.3 TFLOP/s peak).
+                # gfx950 issues a second VALU instruction in some quad-cycles
+                # (SQ_ACTIVE_INST_VALU2 counts them), so `frac` can pass 1; `busy_frac` is
+                # the share of SIMD quad-cycles that issued at least one VALU instruction.
                 rate = t["valu_insts"] / mean_s / 1e9
                 valu = {"insts_per_launch": int(t["valu_insts"]), "achieved": round(rate, 1),
                         "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
                         "frac": round(rate / VALU_PEAK_GINST, 3)}
+                if "valu_dual_quadcycles" in t:
+                    single = 1.0 - t["valu_dual_quadcycles"] / t["valu_insts"]
+                    valu["dual_issue_quadcycles_per_launch"] = int(t["valu_dual_quadcycles"])
+                    valu["busy_frac"] = round(rate * single / VALU_PEAK_GINST, 3)
     roofline = {"kernel": roof_name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": int(b),
