@@ -453,10 +453,11 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   __shared__ uint32_t srect[B];
   __shared__ uint32_t smax_last;
   __shared__ uint32_t s_tile;
-  // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums.
-  __shared__ float red[ORDERED ? B * kSlots * 256 : B * kSlots * 16];
   // words per entry in `slots`: the absolute sums (slots 9, 10) only when asked for
   constexpr uint32_t SS = ABS ? kSlots : 9;
+  // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums of the SS slots
+  // (20 KB at B = 32 without the absolute sums: eight workgroups per CU).
+  __shared__ float red[ORDERED ? B * kSlots * 256 : B * SS * 16];
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
   for (;;) {
   const uint32_t tile = next_tile<PERSIST>(counter, &s_tile, order, num_tiles);
@@ -517,7 +518,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       srect[threadIdx.x] = entry_rect(a, b, c, ti);
     }
     if (!ORDERED)  // partials of entries a wave skips stay zero
-      for (uint32_t q = threadIdx.x; q < cnt * kSlots * 16; q += blockDim.x) red[q] = 0.0f;
+      for (uint32_t q = threadIdx.x; q < cnt * SS * 16; q += blockDim.x) red[q] = 0.0f;
     __syncthreads();
     if (ORDERED) {
       for (int32_t kk = (int32_t)cnt - 1; kk >= 0; kk--) {
@@ -557,7 +558,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
             r = l16 == 13 ? e9 : r;
             r = l16 == 11 ? e10 : r;
           }
-          if (writer) red[kk * kSlots * 16 + woff] = r;
+          if (writer) red[kk * SS * 16 + woff] = r;
         }
       }
     }
@@ -572,7 +573,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       } else {
         // pair_terms_fast's products: slots 0, 1 combine both position sums with the conic
         const auto part_sum = [&](uint32_t sl) {
-          const float* p = &red[(kk * kSlots + sl) * 16];
+          const float* p = &red[(kk * SS + sl) * 16];
           float a = 0.0f;
 #pragma unroll
           for (int t = 0; t < 16; t++) a += p[t];
