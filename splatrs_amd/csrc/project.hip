@@ -667,6 +667,60 @@ __global__ void __launch_bounds__(256) project_bwd_kernel(const float* __restric
   for (int j = 0; j < kParams; j++) grads[(size_t)j * stride + i] = out[j];
 }
 
+// Adam (src/adam.rs:44-61) over a workgroup's 59 x R parameter and moment values, as
+// float4 groups of 4 consecutive rows (plane-major) starting at row `ib`; grad(j, r) gives
+// the float4 gradient of plane j, rows r..r+3.  kU groups per lane are in flight: their
+// parameter and moment loads are issued together, before any of their updates (one group's
+// 48 bytes per lane between waits leaves HBM latency exposed at a few waves per SIMD).
+// Groups that run past `rows` take the scalar path.  `ib` and `stride` are multiples of 64.
+template <uint32_t R, typename GradFn>
+__device__ __forceinline__ void adam_groups(float* __restrict__ params, float* __restrict__ m,
+                                            float* __restrict__ v, uint32_t stride, uint32_t ib,
+                                            uint32_t rows, const splat_learning_rates& lr,
+                                            float b1, float b2, float c1, float c2, float eps,
+                                            GradFn grad) {
+  constexpr uint32_t kPerPlane = R / 4;  // float4 groups per plane
+  constexpr uint32_t kGroups = kParams * kPerPlane;
+  constexpr uint32_t kU = 4;
+  for (uint32_t q0 = threadIdx.x; q0 < kGroups; q0 += kU * R) {
+    float4 P[kU], M[kU], V[kU];
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      // unconditional, so the loads issue back to back: a group past the rows (used by the
+      // scalar path or not at all) reads rows ib..ib+3 of plane 0 instead, which exist since
+      // ib < n <= stride and both ib and stride are multiples of 64
+      const uint32_t q = q0 + u * R;
+      const uint32_t j = q / kPerPlane, r = (q % kPerPlane) * 4u;
+      const size_t at = q < kGroups && r + 4 <= rows ? (size_t)j * stride + ib + r : (size_t)ib;
+      P[u] = load_stream(params + at);
+      M[u] = load_stream(m + at);
+      V[u] = load_stream(v + at);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kU; u++) {
+      const uint32_t q = q0 + u * R;
+      const uint32_t j = q / kPerPlane, r = (q % kPerPlane) * 4u;
+      if (q >= kGroups || r >= rows) continue;
+      const float rate = plane_lr((int)j, lr);
+      const size_t at = (size_t)j * stride + ib + r;
+      const float4 G = grad(j, r);
+      if (r + 4 <= rows) {
+        adam_update(P[u].x, G.x, M[u].x, V[u].x, rate, b1, b2, c1, c2, eps);
+        adam_update(P[u].y, G.y, M[u].y, V[u].y, rate, b1, b2, c1, c2, eps);
+        adam_update(P[u].z, G.z, M[u].z, V[u].z, rate, b1, b2, c1, c2, eps);
+        adam_update(P[u].w, G.w, M[u].w, V[u].w, rate, b1, b2, c1, c2, eps);
+        store_stream(params + at, P[u]);
+        store_stream(m + at, M[u]);
+        store_stream(v + at, V[u]);
+      } else {
+        const float gq[4] = {G.x, G.y, G.z, G.w};
+        for (uint32_t k = 0; r + k < rows; k++)
+          adam_update(params[at + k], gq[k], m[at + k], v[at + k], rate, b1, b2, c1, c2, eps);
+      }
+    }
+  }
+}
+
 // The per-Gaussian part of a train step's backward and the optimizer in one pass, one
 // lane per Gaussian row: the row's slots summed over its entries in entry order
 // (src/rasterize.rs:480-503; the workgroup's rows own one contiguous span of entries,
@@ -774,41 +828,14 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
                      });
   }
   __syncthreads();  // every row's gradient words are in LDS
-  const uint32_t rows = ie - ib;
-  constexpr uint32_t kPerPlane = R / 4;  // float4 groups per plane
-  constexpr uint32_t kGroups = kParams * kPerPlane;  // float4 groups of the workgroup
-#pragma unroll 2
-  for (uint32_t q = threadIdx.x; q < kGroups; q += blockDim.x) {
-    const uint32_t j = q / kPerPlane, r = (q % kPerPlane) * 4u;
-    if (r >= rows) continue;
-    const float rate = plane_lr((int)j, lr);
-    const size_t at = (size_t)j * stride + ib + r;
-    float4 G;
-    if (j < 11) {
-      G = *reinterpret_cast<const float4*>(sh + j * R + r);
-    } else {
-      const uint32_t kk = (j - 11) / 3, ch = (j - 11) % 3;
-      const float4 dc = *reinterpret_cast<const float4*>(sh + (27 + ch) * R + r);
-      const float4 bs = *reinterpret_cast<const float4*>(sh + (11 + kk) * R + r);
-      G = make_float4(dc.x * bs.x, dc.y * bs.y, dc.z * bs.z, dc.w * bs.w);
-    }
-    if (r + 4 <= rows) {
-      float4 P = load_stream(params + at);
-      float4 M = load_stream(m + at);
-      float4 V = load_stream(v + at);
-      adam_update(P.x, G.x, M.x, V.x, rate, b1, b2, c1, c2, eps);
-      adam_update(P.y, G.y, M.y, V.y, rate, b1, b2, c1, c2, eps);
-      adam_update(P.z, G.z, M.z, V.z, rate, b1, b2, c1, c2, eps);
-      adam_update(P.w, G.w, M.w, V.w, rate, b1, b2, c1, c2, eps);
-      store_stream(params + at, P);
-      store_stream(m + at, M);
-      store_stream(v + at, V);
-    } else {
-      const float gq[4] = {G.x, G.y, G.z, G.w};
-      for (uint32_t k = 0; r + k < rows; k++)
-        adam_update(params[at + k], gq[k], m[at + k], v[at + k], rate, b1, b2, c1, c2, eps);
-    }
-  }
+  adam_groups<R>(params, m, v, stride, ib, ie - ib, lr, b1, b2, c1, c2, eps,
+                 [&](uint32_t j, uint32_t r) {
+                   if (j < 11) return *reinterpret_cast<const float4*>(sh + j * R + r);
+                   const uint32_t kk = (j - 11) / 3, ch = (j - 11) % 3;
+                   const float4 dc = *reinterpret_cast<const float4*>(sh + (27 + ch) * R + r);
+                   const float4 bs = *reinterpret_cast<const float4*>(sh + (11 + kk) * R + r);
+                   return make_float4(dc.x * bs.x, dc.y * bs.y, dc.z * bs.z, dc.w * bs.w);
+                 });
 }
 
 // Batched backward tail and optimizer, one wave of 64 scene rows per workgroup (scene rows
@@ -859,32 +886,10 @@ __global__ void __launch_bounds__(kBatchRows) grad_adam_batch_kernel(
   }
   __syncthreads();
   lr.mean = bd.lr_mean[s];
-  constexpr uint32_t kPerPlane = R / 4;  // float4 groups per plane
-  constexpr uint32_t kGroups = kParams * kPerPlane;
-#pragma unroll 2
-  for (uint32_t q = t; q < kGroups; q += R) {
-    const uint32_t j = q / kPerPlane, r = (q % kPerPlane) * 4u;
-    if (r >= rows) continue;
-    const float rate = plane_lr((int)j, lr);
-    const size_t at = (size_t)j * stride + ib + r;
-    const float* gq = acc + j * R + r;
-    if (r + 4 <= rows) {
-      float4 P = load_stream(params + at);
-      float4 M = load_stream(m + at);
-      float4 V = load_stream(v + at);
-      const float4 G = *reinterpret_cast<const float4*>(gq);
-      adam_update(P.x, G.x, M.x, V.x, rate, b1, b2, c1, c2, eps);
-      adam_update(P.y, G.y, M.y, V.y, rate, b1, b2, c1, c2, eps);
-      adam_update(P.z, G.z, M.z, V.z, rate, b1, b2, c1, c2, eps);
-      adam_update(P.w, G.w, M.w, V.w, rate, b1, b2, c1, c2, eps);
-      store_stream(params + at, P);
-      store_stream(m + at, M);
-      store_stream(v + at, V);
-    } else {
-      for (uint32_t kk = 0; r + kk < rows; kk++)
-        adam_update(params[at + kk], gq[kk], m[at + kk], v[at + kk], rate, b1, b2, c1, c2, eps);
-    }
-  }
+  adam_groups<R>(params, m, v, stride, ib, rows, lr, b1, b2, c1, c2, eps,
+                 [&](uint32_t j, uint32_t r) {
+                   return *reinterpret_cast<const float4*>(acc + j * R + r);
+                 });
 }
 
 // AoS (n x 59) <-> planes, staged through LDS so both sides are read and written with
