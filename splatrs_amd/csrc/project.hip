@@ -770,8 +770,23 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
   for (uint32_t c0 = span0; c0 < span1; c0 += kChunk) {
     const uint32_t c1e = min(c0 + kChunk, span1);
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < (c1e - c0) * ns; q += blockDim.x)
-      sh[q] = slots[(size_t)c0 * ns + q];
+    {  // 8 coalesced loads per lane in flight before their LDS stores
+      const uint32_t words = (c1e - c0) * ns;
+      const float* src = slots + (size_t)c0 * ns;
+      for (uint32_t q0 = threadIdx.x; q0 < words; q0 += 8 * R) {
+        float x[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+          const uint32_t q = q0 + u * R;
+          x[u] = src[min(q, words - 1)];  // in the span: no branch between the loads
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+          const uint32_t q = q0 + u * R;
+          if (q < words) sh[q] = x[u];
+        }
+      }
+    }
     __syncthreads();
     for (uint32_t e = max(e0, c0); e < min(e1, c1e); e++) {
       const float* q = sh + (e - c0) * ns;
