@@ -85,6 +85,29 @@ __device__ __forceinline__ KeygenSpan keygen_span(uint32_t e_total,
   return sp;
 }
 
+// (x / w, x % w) for 0 < w < 2^16 and x < 2^22 (a rectangle of a view's tile grid, which
+// has fewer than 2^22 tiles): the estimate from the reciprocal (relative error below
+// 2^-22) is the quotient or one less, and one correction step makes it exact.
+__device__ __forceinline__ uint2 divmod_small(uint32_t x, uint32_t w) {
+  uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)w));
+  int32_t r = (int32_t)(x - q * w);
+  if (r < 0) {
+    q--;
+    r += (int32_t)w;
+  } else if (r >= (int32_t)w) {
+    q++;
+    r -= (int32_t)w;
+  }
+  return make_uint2(q, (uint32_t)r);
+}
+
+// Tile of an entry within its rectangle of width w: (local / w, local % w), exact for any
+// local (the reciprocal form covers every rectangle of fewer than 2^22 tiles).
+__device__ __forceinline__ uint2 rect_divmod(uint32_t local, uint32_t w) {
+  if (local < (1u << 22)) return divmod_small(local, w);
+  return make_uint2(local / w, local % w);
+}
+
 __device__ __forceinline__ uint32_t keygen_find(const KeygenSpan& sp, uint32_t e,
                                                 const uint32_t* __restrict__ entry_offset,
                                                 const uint32_t* s_off) {
@@ -121,7 +144,8 @@ __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict_
     const uint2 rc = rect[g];
     const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
     const uint32_t w = x1 - x0;
-    const uint32_t ty = y0 + local / w, tx = x0 + local % w;
+    const uint2 qr = rect_divmod(local, w);
+    const uint32_t ty = y0 + qr.x, tx = x0 + qr.y;
     const uint32_t tile = ty * tiles_x + tx;
     entry_gaussian[e] = g;
     entry_tile[e] = tile;
@@ -157,7 +181,8 @@ __global__ void __launch_bounds__(256) keygen_batch_kernel(const uint32_t* __res
     const uint2 rc = rect[g];
     const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
     const uint32_t w = x1 - x0;
-    const uint32_t ty = y0 + local / w, tx = x0 + local % w;
+    const uint2 qr = rect_divmod(local, w);
+    const uint32_t ty = y0 + qr.x, tx = x0 + qr.y;
     const uint32_t tile = bd.tile_base[v] + ty * bd.tiles_x[v] + tx;
     entry_gaussian[e] = g;
     entry_tile[e] = tile;
@@ -214,18 +239,20 @@ __global__ void guard_kernel(const unsigned long long* __restrict__ total64,
 }
 
 // One bitonic step over `n` keys (virtual +inf padding beyond n).  Pairs p in [0, half).
+// `size` (>= 2) and `stride` are powers of two: shifts and masks index the pairs.
 template <class Load, class Store>
 __device__ __forceinline__ void bitonic_pairs(uint32_t n, uint32_t half, uint32_t size,
                                               uint32_t stride, bool flip, Load ld, Store st) {
+  const uint32_t lh = (uint32_t)__builtin_ctz(size) - 1u;  // log2(size / 2)
+  const uint32_t ls = stride ? (uint32_t)__builtin_ctz(stride) : 0u;
   for (uint32_t p = threadIdx.x; p < half; p += blockDim.x) {
     uint32_t i, j;
     if (flip) {
-      const uint32_t h = size >> 1;
-      const uint32_t blk = p / h, off = p % h;
-      i = blk * size + off;
-      j = blk * size + size - 1 - off;
+      const uint32_t blk = p >> lh, off = p & ((1u << lh) - 1u);
+      i = (blk << (lh + 1u)) + off;
+      j = (blk << (lh + 1u)) + size - 1 - off;
     } else {
-      i = (p / stride) * 2 * stride + p % stride;
+      i = ((p >> ls) << (ls + 1u)) + (p & (stride - 1u));
       j = i + stride;
     }
     if (j < n) {
@@ -380,22 +407,6 @@ __global__ void __launch_bounds__(256) ordered_rows_kernel(
   const uint32_t c = ntiles[g];
   cnt[i] = c;
   if (c) rows[i] = recs[g];
-}
-
-// (x / w, x % w) for 0 < w < 2^16 and x < 2^22 (a rectangle of a view's tile grid, which
-// has fewer than 2^22 tiles): the estimate from the reciprocal (relative error below
-// 2^-22) is the quotient or one less, and one correction step makes it exact.
-__device__ __forceinline__ uint2 divmod_small(uint32_t x, uint32_t w) {
-  uint32_t q = (uint32_t)((float)x * __builtin_amdgcn_rcpf((float)w));
-  int32_t r = (int32_t)(x - q * w);
-  if (r < 0) {
-    q--;
-    r += (int32_t)w;
-  } else if (r >= (int32_t)w) {
-    q++;
-    r -= (int32_t)w;
-  }
-  return make_uint2(q, (uint32_t)r);
 }
 
 constexpr uint32_t kOrderedPerLane = 4;
