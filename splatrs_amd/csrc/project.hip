@@ -770,18 +770,19 @@ __global__ void __launch_bounds__(kGradAdamRows) grad_adam_kernel(
   for (uint32_t c0 = span0; c0 < span1; c0 += kChunk) {
     const uint32_t c1e = min(c0 + kChunk, span1);
     __syncthreads();
-    {  // 8 coalesced loads per lane in flight before their LDS stores
+    {  // 16 coalesced loads per lane in flight before their LDS stores
+      constexpr uint32_t kLoads = 16;
       const uint32_t words = (c1e - c0) * ns;
       const float* src = slots + (size_t)c0 * ns;
-      for (uint32_t q0 = threadIdx.x; q0 < words; q0 += 8 * R) {
-        float x[8];
+      for (uint32_t q0 = threadIdx.x; q0 < words; q0 += kLoads * R) {
+        float x[kLoads];
 #pragma unroll
-        for (uint32_t u = 0; u < 8; u++) {
+        for (uint32_t u = 0; u < kLoads; u++) {
           const uint32_t q = q0 + u * R;
           x[u] = src[min(q, words - 1)];  // in the span: no branch between the loads
         }
 #pragma unroll
-        for (uint32_t u = 0; u < 8; u++) {
+        for (uint32_t u = 0; u < kLoads; u++) {
           const uint32_t q = q0 + u * R;
           if (q < words) sh[q] = x[u];
         }

@@ -611,7 +611,8 @@ __global__ void __launch_bounds__(64) reduce_slots_kernel(const uint32_t* __rest
                                                           float* __restrict__ abs_grads,
                                                           const StepStatus* __restrict__ status) {
   constexpr uint32_t kRows = 64 / NS;
-  __shared__ float sh[kReduceWindow * kSlots];
+  constexpr uint32_t kLoads = 16;  // loads per lane in flight before their LDS stores
+  __shared__ float sh[kReduceWindow * NS];
   if (status && status->abort) return;  // uniform
   const uint32_t lane = threadIdx.x;
   const uint32_t r0 = blockIdx.x * kRows, rl = lane / NS, k = lane % NS;
@@ -624,15 +625,15 @@ __global__ void __launch_bounds__(64) reduce_slots_kernel(const uint32_t* __rest
     const uint32_t words = (min(c0 + kReduceWindow, span1) - c0) * NS;
     const float* src = slots + (size_t)c0 * NS;
     __syncthreads();
-    for (uint32_t q0 = 0; q0 < words; q0 += 64 * 8) {
-      float t[8];
+    for (uint32_t q0 = 0; q0 < words; q0 += 64 * kLoads) {
+      float t[kLoads];
 #pragma unroll
-      for (uint32_t u = 0; u < 8; u++) {
+      for (uint32_t u = 0; u < kLoads; u++) {
         const uint32_t q = q0 + u * 64 + lane;
-        t[u] = q < words ? src[q] : 0.0f;
+        t[u] = src[min(q, words - 1)];  // in the window: no branch between the loads
       }
 #pragma unroll
-      for (uint32_t u = 0; u < 8; u++) {
+      for (uint32_t u = 0; u < kLoads; u++) {
         const uint32_t q = q0 + u * 64 + lane;
         if (q < words) sh[q] = t[u];
       }
