@@ -537,7 +537,7 @@ typedef struct {
 } splat_batch_step_params;
 
 /* Launch shapes of the compositing kernels (results do not depend on them): entries per
- * LDS batch (forward 32/64/128, tree-reduced backward 16/32) and persistent launches
+ * LDS batch (forward 32/64/128/256, tree-reduced backward 16/32) and persistent launches
  * (a fixed grid of per_cu workgroups per compute unit taking tiles from a counter). */
 typedef struct {
   int32_t fwd_batch, fwd_persistent, fwd_per_cu;

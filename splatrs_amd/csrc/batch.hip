@@ -518,7 +518,7 @@ int splat_batch_tune(splat_batch* b, const uint32_t* views, uint32_t nviews,
 int splat_batch_set_shapes(splat_batch* b, const splat_tune_result* shapes) {
   return bguard(b, [&] {
     SPLAT_REQUIRE(shapes, "null argument");
-    auto ok_f = [](int v) { return v == 32 || v == 64 || v == 128; };
+    auto ok_f = [](int v) { return v == 32 || v == 64 || v == 128 || v == 256; };
     SPLAT_REQUIRE(ok_f(shapes->fwd_batch) && (shapes->bwd_batch == 16 || shapes->bwd_batch == 32),
                   "unsupported batch shape");
     b->fwd.fwd_batch = shapes->fwd_batch;

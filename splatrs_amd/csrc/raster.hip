@@ -169,6 +169,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
     uint32_t H, float bg0, float bg1, float bg2, float* __restrict__ image,
     float* __restrict__ final_t, uint32_t* __restrict__ n_contrib, BatchDev bd,
     uint32_t* __restrict__ counter) {
+  static_assert(B <= 256, "each of the 256 lanes stages at most one entry of a batch");
   __shared__ uint64_t tab[32];
   __shared__ float4 s0[B], s1[B], s2[B];
   __shared__ uint32_t srect[B];
@@ -699,6 +700,9 @@ void fwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
     case 128:
       return fwd_go<128, BATCH, PERSIST>(ctx, pb, bb, T, width, height, bg, image, final_t,
                                          n_contrib, bd, counter, rv.per_cu);
+    case 256:  // one entry per lane: a whole batch per barrier pair
+      return fwd_go<256, BATCH, PERSIST>(ctx, pb, bb, T, width, height, bg, image, final_t,
+                                         n_contrib, bd, counter, rv.per_cu);
     default:
       return fwd_go<64, BATCH, PERSIST>(ctx, pb, bb, T, width, height, bg, image, final_t,
                                         n_contrib, bd, counter, rv.per_cu);
@@ -814,7 +818,7 @@ RasterTune tune_raster(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& 
     };
     const RasterVariant def;
     out.fwd_ms = out.bwd_ms = 1e30f;
-    for (int fb : {32, 64, 128})
+    for (int fb : {32, 64, 128, 256})
       for (int mode = 0; mode < 4; mode++) {
         RasterVariant rv;
         rv.fwd_batch = fb;

@@ -811,7 +811,7 @@ int splat_trainer_set_shapes(splat_trainer* t, const splat_tune_result* shapes) 
   return tguard(t, [&] {
     SPLAT_REQUIRE(shapes, "null argument");
     SPLAT_REQUIRE((shapes->fwd_batch == 32 || shapes->fwd_batch == 64 ||
-                   shapes->fwd_batch == 128) &&
+                   shapes->fwd_batch == 128 || shapes->fwd_batch == 256) &&
                       (shapes->bwd_batch == 16 || shapes->bwd_batch == 32),
                   "unsupported batch shape");
     drain(t);

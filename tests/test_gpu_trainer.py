@@ -281,7 +281,7 @@ def test_tuned_and_persistent_shapes_give_the_same_model(ctx):
     persistent compositing launches step exactly like the default one (fast backward)."""
     g, cams, views = small_training_problem(ctx)
     extent = scenes.scene_extent(cams)
-    trs = [abi.Trainer(ctx, g) for _ in range(3)]
+    trs = [abi.Trainer(ctx, g) for _ in range(4)]
     for tr in trs:
         for c, v in zip(cams, views):
             tr.add_view(c, v)
@@ -296,6 +296,10 @@ def test_tuned_and_persistent_shapes_give_the_same_model(ctx):
     forced["fwd_batch"], forced["fwd_persistent"], forced["fwd_per_cu"] = 32, 1, 4
     forced["bwd_batch"], forced["bwd_persistent"], forced["bwd_per_cu"] = 16, 1, 2
     trs[2].set_shapes(forced)
+    whole = r.copy()  # a whole tile's entries per forward batch, one workgroup per tile
+    whole["fwd_batch"], whole["fwd_persistent"], whole["fwd_per_cu"] = 256, 0, 1
+    whole["bwd_batch"], whole["bwd_persistent"], whole["bwd_per_cu"] = 32, 0, 1
+    trs[3].set_shapes(whole)
     for it in range(1, 7):
         lr = rec.learning_rates(rec.position_lr(it, extent))
         p = rec.step_params(3, lr, accumulate_stats=True, absgrad=it > 3)
