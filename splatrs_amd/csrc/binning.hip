@@ -621,11 +621,12 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
                       uint32_t* big_list, uint32_t* scan_sums, StepStatus* status,
                       const BatchDev* bd) {
   const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
-  fill_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, 0u);
+  // the scan's 64-bit total, the tile counters and the big-tile count, in one launch (the
+  // scan writes only small[0] and total64)
+  fill3_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, bb.tile_count, kTileSub * T, small + 1,
+            1, 0u);
   exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64);
   bin_guard(ctx, total64, capacity, status);
-  fill_u32(ctx, bb.tile_count, kTileSub * T, 0);
-  fill_u32(ctx, small + 1, 1, 0);
   // The entry kernels cover the capacity and read the step's count from `status`.
   launch(ctx, "bin_marks", block_marks_kernel<>, dim3(blocks_for(n, 256)), dim3(256), 0,
          (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
