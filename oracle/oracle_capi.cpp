@@ -307,6 +307,92 @@ float oracle_train_step_f32(float* params, float* m, float* v, uint64_t n, uint6
   return loss;
 }
 
+// Binary64 accuracy reference of rasterize_backward (rasterize_backward_truth): the binary32
+// forward's decisions, binary64 values.  grads: n x 9 doubles, abs_grads: n x 2.
+int oracle_rasterize_backward_truth(const splat_proj* projs, uint64_t n,
+                                    const splat_binning* binning, const splat_raster_settings* s,
+                                    const uint32_t* n_contrib, const float* dimage,
+                                    double* grads, double* abs_grads) {
+  Binning b = binning_from(binning, n);
+  rasterize_backward_truth(reinterpret_cast<const Proj<float>*>(projs), n, b, s->width,
+                           s->height, s->background, n_contrib, dimage, grads, abs_grads);
+  return SPLAT_OK;
+}
+
+// Parameter gradients of one training step's loss (src/train.rs:211-240, no Adam) at
+// `params`: grads_ref (n x 59 binary32, optional) in the reference's arithmetic, and
+// grads_true (n x 59 binary64, optional) = the binary64 projection backward of the
+// parameters applied to rasterize_backward_truth's screen-space gradients, given the
+// binary32 forward's image and loss gradient.  pg_ref / pg_true (n ProjGrad, n x 9 doubles)
+// optionally receive the screen-space gradients.  Returns 0 or a precondition code.
+int oracle_step_gradients(const float* params, uint64_t n, const splat_camera* cam,
+                          const uint8_t* rgb, int sh_degree, float lambda, const float bg[3],
+                          float* grads_ref, double* grads_true, splat_proj_grad* pg_ref,
+                          double* pg_true) {
+  Camera<float> c = camera_from<float>(cam);
+  ProjOpts<float> o;
+  o.near_plane = 0.2f;
+  o.eps2d = 0.3f;
+  o.antialias = false;
+  o.sh_degree = sh_degree;
+  const uint32_t w = cam->width, h = cam->height;
+  std::vector<Proj<float>> p(n);
+  ORACLE_PAR
+  for (uint64_t i = 0; i < n; i++) p[i] = project_one<float>(params + i * 59, c, o);
+  Binning b;
+  if (!bin<float>(p.data(), n, w, h, b)) return SPLAT_ERR_PRECONDITION;
+  const size_t npx = (size_t)w * h;
+  std::vector<float> image(npx * 3), final_t(npx), target(npx * 3), dimage(npx * 3);
+  std::vector<uint32_t> nc(npx);
+  rasterize_forward<float>(p.data(), b, w, h, bg, image.data(), final_t.data(), nc.data());
+  for (size_t i = 0; i < npx * 3; i++) target[i] = (float)rgb[i] / 255.0f;
+  photometric_loss<float>(image.data(), target.data(), w, h, lambda, dimage.data());
+  if (grads_ref || pg_ref) {
+    std::vector<ProjGrad<float>> pg(n);
+    std::vector<float> ab(2 * n);
+    rasterize_backward<float>(p.data(), n, b, w, h, bg, final_t.data(), nc.data(),
+                              dimage.data(), pg.data(), ab.data());
+    if (pg_ref) std::memcpy(pg_ref, pg.data(), n * sizeof(splat_proj_grad));
+    if (grads_ref) {
+      ORACLE_PAR
+      for (uint64_t i = 0; i < n; i++)
+        project_backward_one<float>(params + i * 59, c, o, pg[i], grads_ref + i * 59);
+    }
+  }
+  if (grads_true || pg_true) {
+    std::vector<double> pg9(n * 9), ab(2 * n);
+    rasterize_backward_truth(p.data(), n, b, w, h, bg, nc.data(), dimage.data(), pg9.data(),
+                             ab.data());
+    if (pg_true) std::memcpy(pg_true, pg9.data(), n * 9 * sizeof(double));
+    if (grads_true) {
+      Camera<double> c64;
+      c64.width = c.width;
+      c64.height = c.height;
+      c64.fx = c.fx;
+      c64.fy = c.fy;
+      c64.cx = c.cx;
+      c64.cy = c.cy;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) c64.rot.m[i][j] = c.rot.m[i][j];
+      c64.trans = V3<double>(c.trans.x, c.trans.y, c.trans.z);
+      ProjOpts<double> o64;
+      o64.near_plane = 0.2;
+      o64.eps2d = 0.3;
+      o64.antialias = false;
+      o64.sh_degree = sh_degree;
+      ORACLE_PAR
+      for (uint64_t i = 0; i < n; i++) {
+        double g64[59];
+        for (int k = 0; k < 59; k++) g64[k] = params[i * 59 + k];
+        ProjGrad<double> dp;
+        std::memcpy(&dp, &pg9[i * 9], sizeof(dp));
+        project_backward_one<double>(g64, c64, o64, dp, grads_true + i * 59);
+      }
+    }
+  }
+  return SPLAT_OK;
+}
+
 // ---------------------------------------------------------------- f64 --------------
 // Layouts: gaussians 59 doubles each; camera 18 doubles [w, h, fx, fy, cx, cy, rot 9,
 // trans 3]; opts [near, eps2d, antialias, sh_degree]; Proj 15 doubles (proj_to64);

@@ -969,6 +969,112 @@ void rasterize_backward(const Proj<R>* projs, size_t np, const Binning& b, uint3
   }
 }
 
+// Accuracy reference for binary32 backward passes (not a restatement of one reference
+// function): the exact gradient of the binary32 forward's piecewise-smooth branch,
+// evaluated in binary64.  It walks the pairs backward_tile (rasterize.rs:354-455) walks and
+// takes every decision as the binary32 code takes it -- the sigma range and alpha_min
+// tests of evaluate (rasterize.rs:231-244), the alpha_max clamp, the last contributor
+// n_contrib of the binary32 forward -- but computes every value (G, alpha, the
+// transmittance products, the accumulated colour, all slot sums) in binary64 from the
+// binary32 inputs.  The final transmittance is recomputed in binary64 by the forward walk
+// over the same pairs.  grads9: n x 9 doubles (ProjGrad order), abs2: n x 2.
+inline void rasterize_backward_truth(const Proj<float>* projs, size_t np, const Binning& b,
+                                     uint32_t width, uint32_t height, const float bgc[3],
+                                     const uint32_t* n_contrib_img, const float* dimage,
+                                     double* grads9, double* abs2) {
+  size_t num_entries = b.sorted.size();
+  std::vector<double> slots(num_entries * SLOTS, 0.0);
+  size_t num_tiles = (size_t)b.tiles_x * b.tiles_y;
+  const double amax32 = (double)Raster<float>::alpha_max();
+  ORACLE_PAR
+  for (size_t tile = 0; tile < num_tiles; tile++) {
+    TileGeom geom = tile_geom(b, width, height, tile);
+    uint32_t start = b.tile_range[tile].first, end = b.tile_range[tile].second;
+    if (end == start) continue;
+    std::vector<TileGauss<float>> list;
+    for (uint32_t i = start; i < end; i++)
+      list.push_back(tile_gauss(projs[b.entry_gaussian[b.sorted[i].second]], geom.x0, geom.y0,
+                                geom.w, geom.h));
+    // Binary64 value of one pair that the binary32 code keeps; false when it skips it.
+    auto pair64 = [&](const TileGauss<float>& g, uint32_t i, uint32_t j, double* G64,
+                      double* a64, bool* unclamped) {
+      if (g.is_empty() || i < g.x0 || i >= g.x1 || j < g.y0 || j >= g.y1) return false;
+      const float px = (float)(geom.x0 + i) + 0.5f, py = (float)(geom.y0 + j) + 0.5f;
+      float G32, a32;
+      if (!evaluate(g, px, py, &G32, &a32)) return false;
+      const double dx = (double)px - g.mean[0], dy = (double)py - g.mean[1];
+      const double sigma = 0.5 * ((double)g.conic[0] * dx * dx + (double)g.conic[2] * dy * dy) +
+                           (double)g.conic[1] * dx * dy;
+      *G64 = exp(-sigma);
+      *unclamped = g.opacity * G32 < Raster<float>::alpha_max();
+      *a64 = *unclamped ? (double)g.opacity * *G64 : amax32;
+      return true;
+    };
+    std::vector<double> acc(list.size() * SLOTS, 0.0);
+    for (uint32_t j = 0; j < geom.h; j++)
+      for (uint32_t i = 0; i < geom.w; i++) {
+        const size_t pix = (size_t)(geom.y0 + j) * width + geom.x0 + i;
+        const uint32_t last = n_contrib_img[pix];
+        const double d[3] = {dimage[pix * 3], dimage[pix * 3 + 1], dimage[pix * 3 + 2]};
+        const double bg_dot = (double)bgc[0] * d[0] + (double)bgc[1] * d[1] + (double)bgc[2] * d[2];
+        double t_final = 1.0;
+        for (uint32_t kk = 0; kk < last && kk < list.size(); kk++) {
+          double G, a;
+          bool un;
+          if (pair64(list[kk], i, j, &G, &a, &un)) t_final *= 1.0 - a;
+        }
+        double t = t_final, accum[3] = {0, 0, 0}, last_alpha = 0, last_color[3] = {0, 0, 0};
+        const double px = (double)((float)(geom.x0 + i) + 0.5f);
+        const double py = (double)((float)(geom.y0 + j) + 0.5f);
+        for (size_t kk = std::min<size_t>(last, list.size()); kk-- > 0;) {
+          const TileGauss<float>& g = list[kk];
+          double G, alpha;
+          bool unclamped;
+          if (!pair64(g, i, j, &G, &alpha, &unclamped)) continue;
+          double* a = &acc[kk * SLOTS];
+          t /= 1.0 - alpha;
+          double dalpha = 0;
+          for (int ch = 0; ch < 3; ch++) {
+            accum[ch] = last_alpha * last_color[ch] + (1.0 - last_alpha) * accum[ch];
+            dalpha += ((double)g.color[ch] - accum[ch]) * d[ch];
+            a[6 + ch] += alpha * t * d[ch];
+          }
+          dalpha *= t;
+          for (int ch = 0; ch < 3; ch++) last_color[ch] = g.color[ch];
+          last_alpha = alpha;
+          dalpha += -t_final / (1.0 - alpha) * bg_dot;
+          if (unclamped) {
+            a[5] += G * dalpha;
+            const double dsigma = -(double)g.opacity * dalpha * G;
+            const double dx = px - g.mean[0], dy = py - g.mean[1];
+            const double ddx = dsigma * ((double)g.conic[0] * dx + (double)g.conic[1] * dy);
+            const double ddy = dsigma * ((double)g.conic[1] * dx + (double)g.conic[2] * dy);
+            a[0] -= ddx;
+            a[1] -= ddy;
+            a[2] += dsigma * 0.5 * dx * dx;
+            a[3] += dsigma * dx * dy;
+            a[4] += dsigma * 0.5 * dy * dy;
+            a[9] += fabs(ddx);
+            a[10] += fabs(ddy);
+          }
+        }
+      }
+    for (size_t kk = 0; kk < list.size(); kk++) {
+      uint32_t entry = b.sorted[start + kk].second;
+      for (int s = 0; s < SLOTS; s++) slots[(size_t)entry * SLOTS + s] = acc[kk * SLOTS + s];
+    }
+  }
+  ORACLE_PAR
+  for (size_t g = 0; g < np; g++) {
+    double sum[SLOTS] = {0};
+    for (size_t entry = b.entry_offset[g]; entry < b.entry_offset[g + 1]; entry++)
+      for (int k = 0; k < SLOTS; k++) sum[k] += slots[entry * SLOTS + k];
+    for (int k = 0; k < 9; k++) grads9[g * 9 + k] = sum[k];
+    abs2[2 * g] = sum[9];
+    abs2[2 * g + 1] = sum[10];
+  }
+}
+
 // ---------------------------------------------------------------- loss.rs ----------
 
 static const int SSIM_WINDOW = 11;  // loss.rs:12

@@ -132,3 +132,30 @@ def naive_render(projs, width, height, bg):
             for ch in range(3):
                 image[y, x, ch] = f(c[ch] + f(t * f(bg[ch])))
     return image
+
+
+ERR_QUANTILES = (0.5, 0.99, 0.999)
+
+
+def error_quantiles(x, truth):
+    """Quantiles of |x - truth| / |truth| over the entries with a non-zero truth."""
+    x = np.asarray(x, np.float64).ravel()
+    t = np.asarray(truth, np.float64).ravel()
+    nz = t != 0
+    if not nz.any():
+        return {q: 0.0 for q in ERR_QUANTILES}
+    rel = np.abs(x[nz] - t[nz]) / np.abs(t[nz])
+    return {q: float(np.quantile(rel, q)) for q in ERR_QUANTILES}
+
+
+def no_less_accurate(name, fast, ref, truth, say=print, slack=1.0):
+    """Asserts that `fast` (a reassociated binary32 computation) is no less accurate than
+    `ref` (the reference's binary32 order) against the binary64 `truth`: at the median,
+    p99 and p99.9 of the relative errors, fast's error is at most `slack` times ref's.
+    Returns both quantile tables."""
+    qf, qr = error_quantiles(fast, truth), error_quantiles(ref, truth)
+    say(f"{name}: relative error vs binary64 truth, fast / reference order: " +
+        ", ".join(f"p{100 * q:g} {qf[q]:.2e} / {qr[q]:.2e}" for q in ERR_QUANTILES))
+    for q in ERR_QUANTILES:
+        assert qf[q] <= slack * qr[q] + 1e-12, (name, q, qf[q], qr[q])
+    return qf, qr

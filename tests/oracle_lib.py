@@ -64,6 +64,8 @@ def lib():
             "oracle_mlp_adam": (None, [P, P, P, P, i32, f32, f32, f32, f32, f32, f32]),
             "oracle_bf16_round": (None, [P, P, C.c_int64]),
             "oracle_set_threads": (i32, [i32]),
+            "oracle_rasterize_backward_truth": (i32, [P, u64, P, P, P, P, P, P]),
+            "oracle_step_gradients": (i32, [P, u64, P, P, i32, f32, P, P, P, P, P]),
             "oracle_scorer_features_f32": (None, [P, C.c_int64, P, P, P, P, P, P, f32, f32, u32,
                                                   P]),
         }
@@ -141,6 +143,42 @@ def rasterize_backward(projs, binning, settings, final_t, n_contrib, dimage):
                                               ptr(final_t), ptr(n_contrib), ptr(dimage),
                                               ptr(grads), ptr(absg)))
     return grads, absg
+
+
+def rasterize_backward_truth(projs, binning, settings, n_contrib, dimage):
+    """Binary64 accuracy reference of rasterize_backward: the binary32 forward's decisions
+    (n_contrib and the per-pair tests) with every value in binary64.  Returns (grads n x 9,
+    abs n x 2) as float64."""
+    n = len(projs)
+    grads = np.zeros((n, 9), np.float64)
+    absg = np.zeros((n, 2), np.float64)
+    s = binning.as_struct()
+    check(lib().oracle_rasterize_backward_truth(
+        ptr(projs), n, C.byref(s), ptr(settings), ptr(np.ascontiguousarray(n_contrib, np.uint32)),
+        ptr(np.ascontiguousarray(dimage, np.float32)), ptr(grads), ptr(absg)), "truth")
+    return grads, absg
+
+
+def step_gradients(params, cam, rgb, sh_degree, lam=0.2, bg=(0, 0, 0), ref=True, truth=True,
+                   screen=False):
+    """Parameter gradients of one training step's loss at `params` (no Adam): the
+    reference's binary32 arithmetic (n x 59 f32) and the binary64 truth (n x 59 f64) of the
+    binary32 forward's branch.  With screen=True also the screen-space gradients
+    (ProjGrad records, n x 9 f64)."""
+    params = np.ascontiguousarray(params, np.float32)
+    n = len(params)
+    gr = np.zeros((n, 59), np.float32) if ref else None
+    gt = np.zeros((n, 59), np.float64) if truth else None
+    pr = np.zeros(n, rec.PROJ_GRAD) if screen else None
+    pt = np.zeros((n, 9), np.float64) if screen else None
+    bgv = np.asarray(bg, np.float32)
+    check(lib().oracle_step_gradients(ptr(params), n, ptr(cam),
+                                      ptr(np.ascontiguousarray(rgb, np.uint8)), sh_degree, lam,
+                                      ptr(bgv), ptr(gr), ptr(gt), ptr(pr), ptr(pt)), "grads")
+    out = {"ref": gr, "true": gt}
+    if screen:
+        out.update(screen_ref=pr, screen_true=pt)
+    return out
 
 
 def blur(src, width, height, channels):

@@ -6,7 +6,10 @@ statistics are bit-exact.  Composited pixels are within 1e-5 (absolute) of the o
 and in practice bitwise equal except where the per-Gaussian alpha-threshold exponent
 ln(opacity * 255) rounds differently (the GPU rounds a binary64 log; glibc's logf
 differs from correct rounding in ~5e-4 of inputs).  Parameter gradients are within 1e-4
-relative (plus an absolute floor of 1e-4 of the field's largest magnitude).
+relative (plus an absolute floor of 1e-4 of the field's largest magnitude) in the
+reference's summation order; the tree-reduced backward is held to a floor-free bar
+instead: at the median, p99 and p99.9 of its relative errors against the binary64
+gradient of the same forward it is no less accurate than the reference's binary32 order.
 """
 import sys
 
@@ -14,7 +17,8 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from helpers import ground_truth, random_gaussian, ref_test_camera, ring_cameras
+from helpers import (ground_truth, no_less_accurate, random_gaussian, ref_test_camera,
+                     ring_cameras)
 from splatrs_amd import abi, scenes
 from splatrs_amd import records as rec
 
@@ -95,17 +99,26 @@ def full_pipeline(ctx, g, cam, opts, bg, ordered=True, check_bwd=True, seed=2):
     dimg = np.random.default_rng(seed).standard_normal(io.shape).astype(np.float32)
     gg, ag = ctx.rasterize_backward(po, bora, st, to, no, dimg, ordered=ordered)
     go, ao = O.rasterize_backward(po, bora, st, to, no, dimg)
-    for f in ("mean2d", "conic", "opacity", "color"):
-        if ordered:
+    if ordered:
+        # the reference's summation order: bitwise, up to the rare ulp of ln(255 o)
+        for f in ("mean2d", "conic", "opacity", "color"):
             same = np.mean(gg[f] == go[f])
             say(f"backward {f}: bitwise-equal fraction {same:.6f}")
-        else:
-            a, b = gg[f].astype(np.float64), go[f].astype(np.float64)
-            rel = np.abs(a - b) / np.maximum(np.maximum(np.abs(a), np.abs(b)), 1e-30)
-            say(f"backward {f}: median relative error {np.median(rel):.2e}, fraction beyond "
-                f"1e-4 before the floor {np.mean(rel > 1e-4):.2e}")
-        grad_close(gg[f], go[f], f)
-    grad_close(ag, ao, "abs")
+            assert same >= 0.9999, f
+            grad_close(gg[f], go[f], f)
+        grad_close(ag, ao, "abs")
+    else:
+        # Reassociated sums: no less accurate than the reference's binary32 order against
+        # the binary64 gradient of the same forward (same decisions), field by field.
+        truth, abs_truth = O.rasterize_backward_truth(po, bora, st, no, dimg)
+        fg = gg.view(np.float32).reshape(-1, 9)
+        fo = go.view(np.float32).reshape(-1, 9)
+        names = ["mean2d.x", "mean2d.y", "conic.a", "conic.b", "conic.c", "opacity",
+                 "color.r", "color.g", "color.b"]
+        for k, name in enumerate(names):
+            no_less_accurate(f"backward {name}", fg[:, k], fo[:, k], truth[:, k], say)
+        for k in range(2):
+            no_less_accurate(f"backward abs[{k}]", ag[:, k], ao[:, k], abs_truth[:, k], say)
     pbg = ctx.project_backward_all(g, cam, opts, go)
     pbo = O.project_backward_all(g, cam, opts, go)
     assert np.array_equal(pbg.view(np.uint32), pbo.view(np.uint32))
@@ -291,7 +304,7 @@ def test_malformed_binning_is_a_precondition_error(ctx):
 
 
 def test_full_size_1m_1080p(ctx):
-    say("1M Gaussians, 1920x1080, forward + ordered backward + loss")
+    say("1M Gaussians, 1920x1080, forward + ordered backward")
     g = scenes.large_scene(0)
     cam = scenes.hd_camera()
     opts = rec.proj_opts(3)

@@ -250,3 +250,65 @@ def test_end_to_end_gradient_f64():
                 return O.chain_loss_f64(m, c64, o64, bg, target, 1.0)
             check(f"gaussian {gi} param {i}", float(grads[gi, i]), central(0.0, hstep, f),
                   1.0, F64)
+
+
+def _projs32(p64):
+    """Binary32 Proj records of f64 screen-space Gaussians (the layout of random_projs)."""
+    out = np.zeros(len(p64), rec.PROJ)
+    out["mean2d"] = p64[:, 0:2]
+    out["conic"] = p64[:, 2:5]
+    out["depth"] = p64[:, 5]
+    out["radius"] = p64[:, 6].astype(np.uint32)
+    out["opacity"] = p64[:, 7]
+    out["color"] = p64[:, 8:11]
+    out["tile_min"] = p64[:, 11:13].astype(np.uint32)
+    out["tile_max"] = p64[:, 13:15].astype(np.uint32)
+    return out
+
+
+def test_truth_backward_equals_f64_backward_of_binary32_inputs():
+    # The binary64 accuracy reference (binary32 decisions, binary64 values) against the
+    # finite-difference-pinned f64 backward (test_rasterizer_gradient_f64) run on the same
+    # binary32 values promoted to f64: with no decision near a threshold they are the same
+    # function, so they agree to binary64 rounding; and the binary32 reference order lies
+    # within binary32 rounding of both.
+    rng = np.random.default_rng(33)
+    w, h = 40, 28
+    bg = np.float32([0.2, 0.5, 0.8]).astype(np.float64)
+    p32 = _projs32(random_projs(rng, w, h, 9))
+    p64 = np.zeros((9, 15))
+    p64[:, 0:2], p64[:, 2:5], p64[:, 5] = p32["mean2d"], p32["conic"], p32["depth"]
+    p64[:, 6], p64[:, 7], p64[:, 8:11] = p32["radius"], p32["opacity"], p32["color"]
+    p64[:, 11:13], p64[:, 13:15] = p32["tile_min"], p32["tile_max"]
+    wt = rng.uniform(-1, 1, (h, w, 3)).astype(np.float32)
+    g64, a64 = O.raster_backward_projs_f64(p64, w, h, bg, wt.astype(np.float64))
+    st = rec.raster_settings(w, h, bg)
+    b = O.bin(p32, w, h)
+    _, ft, nc = O.rasterize_forward(p32, b, st)
+    gt, at = O.rasterize_backward_truth(p32, b, st, nc, wt)
+    scale = np.abs(g64).max(axis=0)
+    assert np.all(np.abs(gt - g64) <= 1e-10 * scale)
+    assert np.all(np.abs(at - a64) <= 1e-10 * np.abs(a64).max(axis=0))
+    g32, _ = O.rasterize_backward(p32, b, st, ft, nc, wt)
+    g32 = g32.view(np.float32).reshape(-1, 9)
+    assert np.all(np.abs(g32 - gt) <= 1e-5 * scale)
+    assert np.abs(g32 - gt).max() > 0  # binary32 rounding is visible against the truth
+
+
+def test_step_gradient_truth_is_close_to_reference_arithmetic():
+    # Parameter gradients of a small training step: the binary32 reference arithmetic is
+    # within binary32 rounding of the binary64 truth, field by field.
+    from splatrs_amd import scenes
+    g = scenes.small_scene(0, 600)
+    cam = scenes.small_scene_cameras()[0]
+    gt_img = np.clip(np.random.default_rng(3).uniform(0, 1, (256, 256, 3)), 0, 1)
+    rgb = np.round(gt_img * 255).astype(np.uint8)
+    out = O.step_gradients(g, cam, rgb, 3, screen=True)
+    ref, true = out["ref"].astype(np.float64), out["true"]
+    scale = np.abs(true).max(axis=0) + 1e-30
+    assert np.abs(true).max() > 0
+    assert np.all(np.abs(ref - true) <= 1e-4 * scale)
+    # the screen-space part equals rasterize_backward_truth's output
+    sr = out["screen_ref"].view(np.float32).reshape(-1, 9).astype(np.float64)
+    st = out["screen_true"]
+    assert np.all(np.abs(sr - st) <= 1e-5 * (np.abs(st).max(axis=0) + 1e-30))
