@@ -325,10 +325,13 @@ typedef struct {
 } splat_colmap_point;
 
 /* read_model, src/colmap/mod.rs:198-238: counts are always set; records are written only if
- * every capacity suffices (else SPLAT_ERR_CAPACITY).  Cameras and images in file order.
- * Missing files: SPLAT_ERR_INVALID; unreadable: SPLAT_ERR_IO; malformed (unknown model,
- * wrong parameter count, truncated binary, implausible counts, bad tokens, names of 256
- * bytes or more): SPLAT_ERR_FORMAT. */
+ * every capacity suffices (else SPLAT_ERR_CAPACITY).  Cameras and images in file order; a
+ * camera id that occurs more than once is one record, with the values read last (the
+ * reference's map by id), at the place of its first occurrence.  Missing files:
+ * SPLAT_ERR_INVALID; unreadable: SPLAT_ERR_IO; malformed (unknown model, wrong parameter
+ * count, truncated binary, implausible counts, bad tokens): SPLAT_ERR_FORMAT.  Image names
+ * are limited to 255 bytes (the record's `name`; the reference has no limit): a longer
+ * name is SPLAT_ERR_FORMAT. */
 int splat_colmap_read(const char* dir, splat_colmap_camera* cameras, uint64_t camera_capacity,
                       uint64_t* num_cameras, splat_colmap_image* images, uint64_t image_capacity,
                       uint64_t* num_images, splat_colmap_point* points, uint64_t point_capacity,
@@ -461,6 +464,14 @@ typedef struct {
   uint32_t num_save_iterations;
   uint32_t log_interval;
   int32_t ordered_backward; /* 1: reference summation order in the backward pass */
+  /* No counterpart in the reference (its train() always starts at iteration 1).  0 or 1:
+   * start at iteration 1.  K + 1: continue a run whose state after iteration K was saved
+   * (splat_checkpoint_write from on_save) and restored (splat_trainer_restore with Adam
+   * step K): the SH degree, view order, densification windows and opacity resets follow
+   * the uninterrupted run's, so the result equals it bitwise.  The loss EMA of the
+   * progress records restarts at the first resumed iteration.  Not available with an
+   * attached scorer (its logged samples are not part of a checkpoint). */
+  uint32_t start_iteration;
 } splat_train_config;
 
 /* Progress, src/train.rs:140-157. */

@@ -199,6 +199,14 @@ int oracle_photometric_loss_f32(const float* pred, const float* target, uint32_t
   return SPLAT_OK;
 }
 
+// The binary64 sums of the binary32 SSIM map and L1 terms of photometric_loss.
+int oracle_loss_sums_f32(const float* pred, const float* target, uint32_t w, uint32_t h,
+                         double* sums) {
+  std::vector<float> grad((size_t)w * h * 3);
+  photometric_loss<float>(pred, target, w, h, 0.2f, grad.data(), sums);
+  return SPLAT_OK;
+}
+
 float oracle_psnr_f32(const float* pred, const float* target, uint64_t len) {
   return psnr<float>(pred, target, len);
 }

@@ -1128,9 +1128,11 @@ void blur(const R* src, size_t width, size_t height, size_t channels, const R ke
 }
 
 // ssim_with_grad, loss.rs:75-134.  The reference sums the SSIM map with rayon's parallel
-// `sum` (thread-dependent order); this sums it sequentially.
+// `sum` (thread-dependent order); this sums it sequentially.  sum64, if given, receives
+// the binary64 sum of the same map values (the accuracy reference for reordered sums).
 template <class R>
-R ssim_with_grad(const R* x, const R* y, size_t width, size_t height, R* grad) {
+R ssim_with_grad(const R* x, const R* y, size_t width, size_t height, R* grad,
+                 double* sum64 = nullptr) {
   size_t n = width * height * 3;
   R kernel[SSIM_WINDOW];
   gaussian_kernel(kernel);
@@ -1170,6 +1172,11 @@ R ssim_with_grad(const R* x, const R* y, size_t width, size_t height, R* grad) {
   }
   R ssim_sum = 0;
   for (size_t i = 0; i < n; i++) ssim_sum += smap[i];
+  if (sum64) {
+    double acc = 0;
+    for (size_t i = 0; i < n; i++) acc += (double)smap[i];
+    *sum64 = acc;
+  }
   R mean = ssim_sum / (R)n;
   std::vector<R> b_mu(n), b_exx(n), b_exy(n);
   blur(d_mu.data(), width, height, 3, kernel, b_mu.data());
@@ -1183,17 +1190,23 @@ R ssim_with_grad(const R* x, const R* y, size_t width, size_t height, R* grad) {
 }
 
 // photometric_loss, loss.rs:137-171 (the L1 sum is sequential here, rayon's `sum` in the
-// reference).
+// reference).  sums64, if given, receives the binary64 sums of the SSIM map and of the
+// |pred - target| terms, each term as the reference rounds it.
 template <class R>
 R photometric_loss(const R* pred, const R* target, size_t width, size_t height, R lambda,
-                   R* grad) {
+                   R* grad, double* sums64 = nullptr) {
   size_t n = width * height * 3;
   R inv_n = RL(1.0) / (R)n;
   R l1 = 0;
   for (size_t i = 0; i < n; i++) l1 += rabs(pred[i] - target[i]);
+  if (sums64) {
+    double acc = 0;
+    for (size_t i = 0; i < n; i++) acc += (double)rabs(pred[i] - target[i]);
+    sums64[1] = acc;
+  }
   l1 = l1 * inv_n;
   std::vector<R> dssim(n);
-  R ssim = ssim_with_grad(pred, target, width, height, dssim.data());
+  R ssim = ssim_with_grad(pred, target, width, height, dssim.data(), sums64);
   R loss = (RL(1.0) - lambda) * l1 + lambda * (RL(1.0) - ssim);
   R w1 = (RL(1.0) - lambda) * inv_n;
   ORACLE_PAR

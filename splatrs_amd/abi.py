@@ -98,7 +98,7 @@ class TrainConfig(C.Structure):
                 ("antialias", i32), ("white_background", i32), ("sh_degree", i32),
                 ("sh_degree_interval", u32), ("near_plane", f32), ("seed", u64),
                 ("save_iterations", P), ("num_save_iterations", u32), ("log_interval", u32),
-                ("ordered_backward", i32)]
+                ("ordered_backward", i32), ("start_iteration", u32)]
 
     @classmethod
     def default(cls, **overrides):

@@ -96,13 +96,15 @@ int splat_checkpoint_read(const char* path, float* gaussians, float* m, float* v
   if (!get(f, &degree, 4, 1) || !get(f, &count, 8, 1) || !get(f, &step, 8, 1) || degree < 0 ||
       degree > 3)
     return set_err(err, err_len, SPLAT_ERR_FORMAT, "malformed checkpoint header");
-  // the file must hold exactly the arrays the header announces
-  const uint64_t body = count * (3ull * kParams * 4 + 12);
+  // the file must hold exactly the arrays the header announces (the count is checked
+  // against the remaining bytes before it is multiplied, so no size can wrap)
+  constexpr uint64_t kRowBytes = 3ull * kParams * 4 + 12;
   const long here = std::ftell(f);
   if (here < 0 || std::fseek(f, 0, SEEK_END) != 0)
     return set_err(err, err_len, SPLAT_ERR_IO, "cannot size checkpoint");
   const long end = std::ftell(f);
-  if (end < here || (uint64_t)(end - here) != body)
+  if (end < here || count > (uint64_t)(end - here) / kRowBytes ||
+      (uint64_t)(end - here) != count * kRowBytes)
     return set_err(err, err_len, SPLAT_ERR_FORMAT, "truncated or oversized checkpoint");
   std::fseek(f, here, SEEK_SET);
   *n = count;

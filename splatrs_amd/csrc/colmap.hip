@@ -392,7 +392,24 @@ int splat_colmap_read(const char* dir, splat_colmap_camera* cameras, uint64_t ca
                  std::string("neither ") + stem + ".bin nor " + stem + ".txt found in " + d};
     };
     const auto pc = pick("cameras"), pi = pick("images"), pp = pick("points3D");
-    const auto cams = pc.second ? read_cameras_bin(pc.first) : read_cameras_txt(pc.first);
+    auto cams = pc.second ? read_cameras_bin(pc.first) : read_cameras_txt(pc.first);
+    // The reference collects the cameras into a map by id (src/colmap/mod.rs:234): a
+    // repeated id keeps one record, the last one read.  Here it keeps the position of the
+    // id's first occurrence.
+    {
+      std::vector<splat_colmap_camera> unique;
+      for (const auto& c : cams) {
+        bool seen = false;
+        for (auto& u : unique)
+          if (u.id == c.id) {
+            u = c;
+            seen = true;
+            break;
+          }
+        if (!seen) unique.push_back(c);
+      }
+      cams.swap(unique);
+    }
     const auto imgs = pi.second ? read_images_bin(pi.first) : read_images_txt(pi.first);
     const auto pts = pp.second ? read_points_bin(pp.first) : read_points_txt(pp.first);
     *num_cameras = cams.size();

@@ -316,7 +316,14 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
 // pair_terms in the tree-reduced form, rearranged for fewer instructions.  Equal up to
 // rounding; the decisions (sigma range, alpha_min, alpha_max) are the exact ones, so the
 // walk visits the same pairs as the forward:
-//  * one hardware reciprocal (1 ulp) stands in for both divisions by 1 - alpha;
+//  * one hardware reciprocal y (1 ulp) of d = 1 - alpha serves both divisions by d.  The
+//    transmittance, which the walk divides again at every contributor, takes one residual
+//    correction, t = q + (s.t - q d) y with q = s.t y and the residual exact in an fma: the
+//    error before the final rounding is O(ulp^2), so t is the correctly rounded quotient
+//    (the reference's) but for inputs within 2^-45 of a rounding midpoint, and no bias
+//    of the reciprocal builds up along the walk.  The background term's T_final / d takes
+//    the same correction; with a black background (BG = false) that term is zero and is
+//    left out;
 //  * the accumulated colour behind the next contributor is updated eagerly,
 //    A <- A + alpha (c - A), the reference's last_alpha * last_color + (1 - last_alpha) * acc
 //    (src/rasterize.rs:420-428) evaluated one entry earlier;
@@ -328,7 +335,7 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
 template <bool ABS>
 __device__ __forceinline__ bool pair_terms_fast(PixBack& s, bool act, float4 r0, float4 r1,
                                                 float4 r2, float px, float py,
-                                                const uint64_t* tab, float v[kSlots]) {
+                                                const uint64_t* tab, bool bg, float v[kSlots]) {
   const float dx = px - r0.x;
   const float dy = py - r0.y;
   const float sigma = 0.5f * (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;
@@ -340,13 +347,21 @@ __device__ __forceinline__ bool pair_terms_fast(PixBack& s, bool act, float4 r0,
   const bool p2 = p1 & !(og < kAlphaMin);
   const bool p3 = p2 & (og < kAlphaMax);
   const float alpha = rmin(og, kAlphaMax);
-  const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
-  const float t = s.t * inv;
+  const float den = 1.0f - alpha;
+  const float inv = __builtin_amdgcn_rcpf(den);
+  const float q0 = s.t * inv;
+  const float t = fmaf(fmaf(-q0, den, s.t), inv, q0);
   const float e0 = r2.y - s.acc0, e1 = r2.z - s.acc1, e2 = r2.w - s.acc2;
   float dalpha = e0 * s.d0;
   dalpha = fmaf(e1, s.d1, dalpha);
   dalpha = fmaf(e2, s.d2, dalpha);
-  dalpha = fmaf(dalpha, t, -(s.t_final * inv) * s.bg_dot);
+  if (bg) {  // uniform: a kernel argument
+    const float f0 = s.t_final * inv;
+    const float tf = fmaf(fmaf(-f0, den, s.t_final), inv, f0);
+    dalpha = fmaf(dalpha, t, -tf * s.bg_dot);
+  } else {
+    dalpha *= t;
+  }
   const float q = p3 ? -(o * dalpha) * G : 0.0f;
   const float w = p2 ? alpha * t : 0.0f;
   const float qdx = q * dx, qdy = q * dy;
@@ -459,6 +474,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums of the SS slots
   // (20 KB at B = 32 without the absolute sums: eight workgroups per CU).
   __shared__ float red[ORDERED ? B * kSlots * 256 : B * SS * 16];
+  // whether the background term (T_final bg . dL) can be non-zero for a finite dL
+  const bool bg_any = (bg0 != 0.0f) | (bg1 != 0.0f) | (bg2 != 0.0f);
   if (threadIdx.x < 32) tab[threadIdx.x] = kExp2Table[threadIdx.x];
   for (;;) {
   const uint32_t tile = next_tile<PERSIST>(counter, &s_tile, order, num_tiles);
@@ -549,7 +566,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         const uint32_t k = lo + kk;
         float v[kSlots];
         const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
-        if (pair_terms_fast<ABS>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, v)) {
+        if (pair_terms_fast<ABS>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, bg_any, v)) {
           float r = row_scatter8(v, lane);
           const float e8 = row_allsum16(v[8]);
           r = l16 == 15 ? e8 : r;

@@ -1063,6 +1063,15 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
     ctx->err = "sh_degree exceeds the maximum of 3";
     return SPLAT_ERR_INVALID;
   }
+  const uint32_t first_it = std::max<uint32_t>(cfg->start_iteration, 1u);
+  if (first_it > cfg->iterations) {
+    ctx->err = "start_iteration is past the last iteration";
+    return SPLAT_ERR_INVALID;
+  }
+  if (first_it > 1 && t->learn) {
+    ctx->err = "a run with an attached scorer cannot resume";
+    return SPLAT_ERR_INVALID;
+  }
   int status = SPLAT_OK;
   const int st = tguard(t, [&] {
     const uint32_t log_iv = cfg->log_interval;
@@ -1073,14 +1082,28 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
     uint32_t last_reset = 0;
     const float bg = cfg->white_background ? 1.0f : 0.0f;
     int32_t sh = 0;  // model.sh_degree starts at 0 (src/train.rs:186-187)
+    const auto sh_steps = [&](uint32_t it) {
+      return cfg->sh_degree_interval > 0 && it % cfg->sh_degree_interval == 0 &&
+             sh < cfg->sh_degree;
+    };
+    const auto resets_at = [&](uint32_t it) {
+      return cfg->densify && it < cfg->densify_until_iter && cfg->opacity_reset_interval > 0 &&
+             (it % cfg->opacity_reset_interval == 0 ||
+              (cfg->white_background && it == cfg->densify_from_iter));
+    };
+    // A resumed run replays the schedule state of iterations 1 .. first_it - 1.
+    for (uint32_t it = 1; it < first_it; it++) {
+      if (sh_steps(it)) sh++;
+      order.next();
+      if (resets_at(it)) last_reset = it;
+    }
     float ema = 0.0f;
     uint64_t dc = 0, ds = 0, dp = 0;
     uint32_t since_log = 0;
     auto timer = std::chrono::steady_clock::now();
     std::vector<float> losses;
-    for (uint32_t it = 1; it <= cfg->iterations; it++) {
-      if (cfg->sh_degree_interval > 0 && it % cfg->sh_degree_interval == 0 && sh < cfg->sh_degree)
-        sh++;
+    for (uint32_t it = first_it; it <= cfg->iterations; it++) {
+      if (sh_steps(it)) sh++;
       const uint32_t view = order.next();
       const bool in_window = cfg->densify && it < cfg->densify_until_iter;
       splat_step_params p;
@@ -1129,10 +1152,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
           ds += rep.split;
           dp += rep.pruned;
         }
-        const bool reset = cfg->opacity_reset_interval > 0 &&
-                           (it % cfg->opacity_reset_interval == 0 ||
-                            (cfg->white_background && it == cfg->densify_from_iter));
-        if (reset) {
+        if (resets_at(it)) {
           const int s3 = splat_trainer_reset_opacity(t, 0.01f);
           if (s3 != SPLAT_OK) fail(s3, ctx->err);
           last_reset = it;
@@ -1148,7 +1168,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
         SPLAT_HIP(hipStreamSynchronize(ctx->stream));
         const uint32_t first = it - since_log + 1;
         for (uint32_t k = 0; k < since_log; k++)
-          ema = (first + k == 1) ? losses[k] : 0.4f * losses[k] + 0.6f * ema;
+          ema = (first + k == first_it) ? losses[k] : 0.4f * losses[k] + 0.6f * ema;
         if (log_now && on_progress) {
           const auto now = std::chrono::steady_clock::now();
           const double ms = std::chrono::duration<double, std::milli>(now - timer).count();
@@ -1167,8 +1187,7 @@ int splat_train(splat_trainer* t, float extent, const splat_train_config* cfg,
           pr.ema_loss = ema;
           pr.psnr = psnr_u8(img, gt);
           pr.num_gaussians = t->n;
-          const uint32_t span = (it % log_iv == 0) ? log_iv : it % log_iv;
-          pr.ms_per_iter = ms / std::max<uint32_t>(span, 1);
+          pr.ms_per_iter = ms / std::max<uint32_t>(since_log, 1);
           pr.cloned = dc;
           pr.split = ds;
           pr.pruned = dp;

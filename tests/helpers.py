@@ -134,28 +134,42 @@ def naive_render(projs, width, height, bg):
     return image
 
 
-ERR_QUANTILES = (0.5, 0.99, 0.999)
+ERR_QUANTILES = (0.5, 0.9, 0.99, 0.999)
+# A quantile is compared only when at least this many samples lie beyond it: the p99.9 of
+# 120 values is a single sample, whose ranking between two summation orders is chance.
+TAIL_SAMPLES = 10
 
 
 def error_quantiles(x, truth):
-    """Quantiles of |x - truth| / |truth| over the entries with a non-zero truth."""
+    """Quantiles of |x - truth| / |truth| over the entries with a non-zero truth; a quantile
+    with fewer than TAIL_SAMPLES samples beyond it is left out."""
     x = np.asarray(x, np.float64).ravel()
     t = np.asarray(truth, np.float64).ravel()
     nz = t != 0
-    if not nz.any():
-        return {q: 0.0 for q in ERR_QUANTILES}
+    n = int(nz.sum())
+    qs = [q for q in ERR_QUANTILES if n * (1.0 - q) >= TAIL_SAMPLES]
+    if not qs:
+        return {}
     rel = np.abs(x[nz] - t[nz]) / np.abs(t[nz])
-    return {q: float(np.quantile(rel, q)) for q in ERR_QUANTILES}
+    return {q: float(np.quantile(rel, q)) for q in qs}
 
 
-def no_less_accurate(name, fast, ref, truth, say=print, slack=1.0):
+# Two binary32 summation orders of the same terms have error distributions that differ
+# only by sampling noise, and where a gradient's error is dominated by the chain both
+# share (the projection backward), their quantiles agree to a few parts in 10^4.  The 5%
+# allowance absorbs that noise; a reduction mistake (a dropped, doubled or misrouted
+# partial) moves the upper quantiles by orders of magnitude.
+ACCURACY_SLACK = 1.05
+
+
+def no_less_accurate(name, fast, ref, truth, say=print, slack=ACCURACY_SLACK):
     """Asserts that `fast` (a reassociated binary32 computation) is no less accurate than
     `ref` (the reference's binary32 order) against the binary64 `truth`: at the median,
     p99 and p99.9 of the relative errors, fast's error is at most `slack` times ref's.
     Returns both quantile tables."""
     qf, qr = error_quantiles(fast, truth), error_quantiles(ref, truth)
     say(f"{name}: relative error vs binary64 truth, fast / reference order: " +
-        ", ".join(f"p{100 * q:g} {qf[q]:.2e} / {qr[q]:.2e}" for q in ERR_QUANTILES))
-    for q in ERR_QUANTILES:
+        ", ".join(f"p{100 * q:g} {qf[q]:.2e} / {qr[q]:.2e}" for q in qf))
+    for q in qf:
         assert qf[q] <= slack * qr[q] + 1e-12, (name, q, qf[q], qr[q])
     return qf, qr

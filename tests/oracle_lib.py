@@ -38,6 +38,7 @@ def lib():
             "oracle_gaussian_kernel_f32": (None, [P]),
             "oracle_ssim_with_grad_f32": (i32, [P, P, u32, u32, P, P]),
             "oracle_photometric_loss_f32": (i32, [P, P, u32, u32, f32, P, P]),
+            "oracle_loss_sums_f32": (i32, [P, P, u32, u32, P]),
             "oracle_psnr_f32": (f32, [P, P, u64]),
             "oracle_adam_step_f32": (i32, [P, P, P, P, u64, P, P]),
             "oracle_stats_accumulate_f32": (i32, [P, P, P, u64, u32, u32, P, P, P]),
@@ -211,6 +212,15 @@ def photometric_loss(pred, target, width, height, lam):
     check(lib().oracle_photometric_loss_f32(ptr(pred), ptr(target), width, height, lam,
                                             ptr(loss), ptr(g)))
     return float(loss[0]), g
+
+
+def loss_sums64(pred, target, width, height):
+    """Binary64 sums of the binary32 SSIM map and |pred - target| terms of the loss."""
+    pred = np.ascontiguousarray(pred, np.float32)
+    target = np.ascontiguousarray(target, np.float32)
+    sums = np.zeros(2, np.float64)
+    check(lib().oracle_loss_sums_f32(ptr(pred), ptr(target), width, height, ptr(sums)))
+    return float(sums[0]), float(sums[1])
 
 
 def psnr(pred, target):

@@ -69,7 +69,8 @@ def test_missing_arrays_are_zeros_and_empty_model(tmp_path):
     assert back["params"].shape == (0, 59) and step == 0
 
 
-@pytest.mark.parametrize("damage", ["magic", "version", "truncated", "padded", "degree"])
+@pytest.mark.parametrize("damage", ["magic", "version", "truncated", "padded", "degree",
+                                    "wrapping_count"])
 def test_malformed_files_are_errors(tmp_path, damage):
     s = state(4, 3)
     p = tmp_path / "d.ckpt"
@@ -83,6 +84,9 @@ def test_malformed_files_are_errors(tmp_path, damage):
         raw = raw[:-1]
     elif damage == "padded":
         raw += b"\0"
+    elif damage == "wrapping_count":
+        # 4 + 2^60 rows of 720 bytes: the product wraps to this file's true body size
+        raw[16:24] = struct.pack("<Q", 4 + (1 << 60))
     else:
         raw[12:16] = struct.pack("<i", 4)
     p.write_bytes(bytes(raw))

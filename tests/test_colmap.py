@@ -244,3 +244,21 @@ def test_malformed_models(tmp_path, case):
     with pytest.raises(abi.ColmapError) as e:
         abi.colmap_read(tmp_path)
     assert e.value.status == want, str(e.value)
+
+
+def test_repeated_camera_id_keeps_the_last_record(tmp_path):
+    # read_model collects cameras into a map by id (src/colmap/mod.rs:234): the record read
+    # last wins; it stays at the place of the id's first occurrence.
+    (tmp_path / "cameras.txt").write_text(
+        "# Camera list\n"
+        "1 PINHOLE 100 80 50 50 40 30\n"
+        "2 SIMPLE_PINHOLE 10 10 5 5 5\n"
+        "1 PINHOLE 200 160 60 61 80 60\n")
+    (tmp_path / "images.txt").write_text("")
+    (tmp_path / "points3D.txt").write_text("")
+    c, i, p = abi.colmap_read(tmp_path)
+    assert len(i) == 0 and len(p) == 0
+    assert list(c["id"]) == [1, 2]
+    assert int(c[0]["width"]) == 200 and int(c[0]["height"]) == 160
+    assert list(c[0]["params"][:4]) == [60.0, 61.0, 80.0, 60.0]
+    assert int(c[1]["model"]) == MODEL_ID["SIMPLE_PINHOLE"]

@@ -18,7 +18,9 @@ Bars:
   * the ordered backward: three continuous steps give params, both moments and the
     statistics bitwise equal to the oracle's (up to the rare ulp of ln(255 o), where the
     device's correctly rounded log and glibc's logf differ);
-  * the loss at 1920x1080: the photometric-loss and SSIM gradients bitwise.
+  * the loss at 1920x1080: the photometric-loss and SSIM gradients bitwise; the scalars
+    within 2e-6 (loss) and 1e-5 (SSIM) of the binary64 sums of the same binary32 terms
+    and no less accurate than the reference's sequential sum.
 """
 import sys
 
@@ -152,9 +154,23 @@ def test_loss_gradient_1080p_bitwise(ctx, problem):
     loss, grad = ctx.photometric_loss(pred, target, w, h, 0.2)
     lo, go = O.photometric_loss(pred, target, w, h, 0.2)
     assert np.array_equal(grad.view(np.uint32), go.view(np.uint32))
-    # the scalar is a sum of 6.2M terms whose order the reference leaves to rayon
-    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    # The scalars are sums of 6.2M terms whose order the reference leaves to rayon's
+    # parallel sum (src/loss.rs:96-121, 146-151); the oracle adds them sequentially in
+    # binary32.  Against the binary64 sums of the same binary32 terms, the device's sums
+    # (binary32 over each workgroup's 8192 terms, by shuffles: error about 13 ulp of the
+    # block's magnitude; binary64 over the blocks) must be within 2e-6 relative for the
+    # loss and 1e-5 for the mean SSIM (its terms change sign), and no less accurate than
+    # the sequential order.
+    ssim64, l164 = O.loss_sums64(pred, target, w, h)
+    n = w * h * 3
+    s64 = ssim64 / n
+    l64 = 0.8 * (l164 / n) + 0.2 * (1.0 - s64)
+    say(f"loss {loss!r}, oracle {lo!r}, binary64 sum {l64!r}")
+    assert abs(loss - l64) <= 2e-6 * abs(l64)
+    assert abs(loss - l64) <= abs(lo - l64) + 1e-7 * abs(l64)
     s, sg = ctx.ssim_with_grad(pred, target, w, h)
     so, sgo = O.ssim_with_grad(pred, target, w, h)
     assert np.array_equal(sg.view(np.uint32), sgo.view(np.uint32))
-    assert abs(s - so) <= 1e-4 * abs(so)
+    say(f"ssim {s!r}, oracle {so!r}, binary64 sum {s64!r}")
+    assert abs(s - s64) <= 1e-5 * abs(s64)
+    assert abs(s - s64) <= abs(so - s64) + 1e-7 * abs(s64)

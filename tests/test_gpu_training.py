@@ -191,6 +191,62 @@ def test_train_loop_matches_restated_schedule(ctx):
         assert np.array_equal(pa[k].view(np.uint32), pb[k].view(np.uint32)), k
 
 
+def test_train_resumes_from_a_checkpoint(ctx, tmp_path):
+    """splat_train to 160 iterations (SH steps at 50 and 100, densification every 20 from
+    30 to 140, opacity resets at 60 and 120) against a run whose state after iteration 90
+    is checkpointed from on_save, restored into a new trainer and trained on from 91:
+    model and moments equal the uninterrupted run's bitwise."""
+    rng = np.random.default_rng(7)
+    gt = ground_truth(rng, 60)
+    cams = ring_cameras(6, 56, 40, 2.6)
+    views = render_views(ctx, gt, cams, (0, 0, 0), 1)
+    extent = scenes.scene_extent(cams)
+    init = ctx.from_points(*init_points(gt, rng))
+    kw = dict(iterations=160, densify=1, densify_from_iter=30, densify_until_iter=140,
+              densification_interval=20, opacity_reset_interval=60,
+              densify_grad_threshold=0.0002, sh_degree=2, sh_degree_interval=50,
+              position_lr_max_steps=160, log_interval=0, seed=11, ordered_backward=1)
+
+    def trainer(model):
+        tr = abi.Trainer(ctx, model)
+        for c, v in zip(cams, views):
+            tr.add_view(c, v)
+        return tr
+    say("uninterrupted splat_train")
+    a = trainer(init)
+    a.train(extent, abi.TrainConfig.default(**kw))
+    want = a.download(moments=True, stats=True)
+    want_step = a.info()[1]
+    a.close()
+    say("splat_train with a checkpoint written at iteration 90")
+    path = tmp_path / "run.ckpt"
+
+    def on_save(it, tr):
+        if it == 90:
+            abi.checkpoint_write(path, tr.download(moments=True, stats=True), tr.info()[1], 1)
+    b = trainer(init)
+    b.train(extent, abi.TrainConfig.default(save_iterations=[90], **kw), on_save=on_save)
+    b.close()
+    saved, step, _ = abi.checkpoint_read(path)
+    assert step == 90
+    say("restore into a new trainer, splat_train from iteration 91")
+    c = trainer(init[:10])
+    c.restore(saved, step)
+    c.train(extent, abi.TrainConfig.default(start_iteration=91, **kw))
+    got = c.download(moments=True, stats=True)
+    assert c.info()[1] == want_step
+    c.close()
+    for k in want:
+        assert got[k].shape == want[k].shape, k
+        assert np.array_equal(got[k].view(np.uint32), want[k].view(np.uint32)), k
+    with pytest.raises(Exception):
+        d = trainer(init)
+        try:
+            d.train(extent, abi.TrainConfig.default(start_iteration=161, **kw))
+        finally:
+            d.close()
+
+
 def test_small_config_matches_oracle(ctx):
     """10k Gaussians (seed 0), 8 views 256x256 at radius 4, targets from seed 1, 200
     iterations of L1 + 0.2 D-SSIM with Adam, no densification."""

@@ -312,3 +312,24 @@ def test_step_gradient_truth_is_close_to_reference_arithmetic():
     sr = out["screen_ref"].view(np.float32).reshape(-1, 9).astype(np.float64)
     st = out["screen_true"]
     assert np.all(np.abs(sr - st) <= 1e-5 * (np.abs(st).max(axis=0) + 1e-30))
+
+
+def test_accuracy_bar_rejects_a_dropped_row_partial():
+    # The bar the tree-reduced backward is held to (no less accurate than the reference
+    # order against the binary64 truth, at p50 / p99 / p99.9) must fail for a plausible
+    # reduction mistake: one of a tile's 16 row partials dropped for 1% of the Gaussians.
+    from helpers import no_less_accurate
+    from splatrs_amd import scenes
+    g = scenes.small_scene(0)  # 10k Gaussians: every quantile has its tail samples
+    cam = scenes.small_scene_cameras()[0]
+    rgb = np.round(np.random.default_rng(3).uniform(0, 1, (256, 256, 3)) * 255).astype(np.uint8)
+    out = O.step_gradients(g, cam, rgb, 3, screen=True)
+    ref = out["screen_ref"].view(np.float32).reshape(-1, 9)
+    truth = out["screen_true"]
+    no_less_accurate("reference against itself", ref, ref, truth, say=lambda m: None)
+    rng = np.random.default_rng(5)
+    bad = ref.copy()
+    rows = rng.choice(len(bad), max(1, len(bad) // 100), replace=False)
+    bad[rows] *= np.float32(15.0 / 16.0)
+    with pytest.raises(AssertionError):
+        no_less_accurate("dropped partial", bad, ref, truth, say=lambda m: None)
