@@ -565,6 +565,16 @@ typedef struct {
 int splat_trainer_tune(splat_trainer* t, uint32_t view, const splat_step_params* p,
                        splat_tune_result* out);
 int splat_trainer_set_shapes(splat_trainer* t, const splat_tune_result* shapes);
+/* No counterpart in the reference (its design notes quote per-pair costs, not counts).  The
+ * compositing half of a step on view `view` (projection, binning, forward, loss and the
+ * tree-reduced backward; the model is not changed) through counting instantiations of the
+ * compositing kernels.  counts[0..4]: forward (entry, wave) visits, visits in which some
+ * pixel evaluates the Gaussian, evaluated (pixel, Gaussian) pairs (the exponential is
+ * computed: 0 <= sigma <= sigma_max), passing pairs (alpha >= 1/255 on a pixel still
+ * compositing), composited pairs; counts[5..8]: the backward walk's visits, active visits,
+ * evaluated and passing pairs.  Synchronises. */
+int splat_trainer_pair_counts(splat_trainer* t, uint32_t view, const splat_step_params* p,
+                              uint64_t counts[9]);
 
 int splat_batch_create(splat_ctx* ctx, splat_batch** out);
 int splat_batch_destroy(splat_batch* b);

@@ -179,6 +179,15 @@ int oracle_rasterize_backward_f32(const splat_proj* projs, uint64_t n, const spl
   return SPLAT_OK;
 }
 
+int oracle_raster_pair_counts(const splat_proj* projs, uint64_t n, const splat_binning* binning,
+                              const splat_raster_settings* s, const uint32_t* n_contrib,
+                              uint64_t* counts) {
+  Binning b = binning_from(binning, n);
+  raster_pair_counts(reinterpret_cast<const Proj<float>*>(projs), b, s->width, s->height,
+                     n_contrib, counts);
+  return SPLAT_OK;
+}
+
 int oracle_blur_f32(const float* src, uint32_t w, uint32_t h, uint32_t ch, float* dst) {
   float k[SSIM_WINDOW];
   gaussian_kernel<float>(k);

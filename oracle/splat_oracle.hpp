@@ -851,6 +851,59 @@ void rasterize_forward(const Proj<R>* projs, const Binning& b, uint32_t width, u
   }
 }
 
+// (Gaussian, pixel) pair counts of the compositing walks, for checking a device's counters:
+// counts[0] pairs forward_tile (rasterize.rs:262-290) evaluates (sigma in [0, sigma_max]
+// on a pixel not yet done, so the exponential is computed), [1] of those passing alpha_min,
+// [2] composited (the transmittance test passes); [3] pairs backward_tile (rasterize.rs:
+// 402-415) evaluates (k < n_contrib), [4] of those passing alpha_min.
+inline void raster_pair_counts(const Proj<float>* projs, const Binning& b, uint32_t width,
+                               uint32_t height, const uint32_t* n_contrib_img,
+                               uint64_t counts[5]) {
+  for (int q = 0; q < 5; q++) counts[q] = 0;
+  size_t num_tiles = (size_t)b.tiles_x * b.tiles_y;
+  for (size_t tile = 0; tile < num_tiles; tile++) {
+    TileGeom geom = tile_geom(b, width, height, tile);
+    uint32_t s = b.tile_range[tile].first, e = b.tile_range[tile].second;
+    size_t n = (size_t)geom.w * geom.h;
+    std::vector<float> trans(n, 1.0f);
+    std::vector<char> done(n, 0);
+    for (uint32_t k = 0; k < e - s; k++) {
+      uint32_t entry = b.sorted[s + k].second;
+      TileGauss<float> g =
+          tile_gauss(projs[b.entry_gaussian[entry]], geom.x0, geom.y0, geom.w, geom.h);
+      if (g.is_empty()) continue;
+      for (uint32_t j = g.y0; j < g.y1; j++) {
+        float py = (float)(geom.y0 + j) + 0.5f;
+        for (uint32_t i = g.x0; i < g.x1; i++) {
+          size_t pix = (size_t)j * geom.w + i;
+          float px = (float)(geom.x0 + i) + 0.5f;
+          float dx = px - g.mean[0], dy = py - g.mean[1];
+          float sigma = 0.5f * (g.conic[0] * dx * dx + g.conic[2] * dy * dy) + g.conic[1] * dx * dy;
+          bool evaluated = !(sigma < 0.0f || sigma > g.sigma_max);
+          float G = 0, alpha = 0;
+          bool passing = evaluate(g, px, py, &G, &alpha);
+          size_t gpix = (size_t)(geom.y0 + j) * width + geom.x0 + i;
+          if (k < n_contrib_img[gpix]) {
+            counts[3] += evaluated;
+            counts[4] += passing;
+          }
+          if (done[pix] || !evaluated) continue;
+          counts[0]++;
+          if (!passing) continue;
+          counts[1]++;
+          float test_t = trans[pix] * (1.0f - alpha);
+          if (test_t < Raster<float>::t_min()) {
+            done[pix] = 1;
+            continue;
+          }
+          counts[2]++;
+          trans[pix] = test_t;
+        }
+      }
+    }
+  }
+}
+
 static const int SLOTS = 11;  // rasterize.rs:341
 
 // rasterize_backward (rasterize.rs:459-505) with backward_tile (rasterize.rs:354-455).

@@ -195,6 +195,7 @@ _SIGS = {
     "splat_scorer_fit": (i32, [P, P, P, P, u64, P, P]),
     "splat_learn_config_default": (None, [P]),
     "splat_trainer_tune": (i32, [P, u32, P, P]),
+    "splat_trainer_pair_counts": (i32, [P, u32, P, P]),
     "splat_trainer_set_shapes": (i32, [P, P]),
     "splat_batch_create": (i32, [P, P]),
     "splat_batch_destroy": (i32, [P]),
@@ -667,6 +668,18 @@ class Trainer:
 
     def set_shapes(self, shapes):
         self.ctx.check(lib().splat_trainer_set_shapes(self._h, _p(shapes)), "trainer_set_shapes")
+
+    PAIR_COUNTERS = ("fwd_visits", "fwd_active_visits", "fwd_evaluated", "fwd_passing",
+                     "fwd_composited", "bwd_visits", "bwd_active_visits", "bwd_evaluated",
+                     "bwd_passing")
+
+    def pair_counts(self, view, params):
+        """(Gaussian, pixel) pair and (entry, wave) visit counts of the compositing kernels on
+        `view` (splat_trainer_pair_counts), as a dict keyed by PAIR_COUNTERS."""
+        c = np.zeros(9, np.uint64)
+        self.ctx.check(lib().splat_trainer_pair_counts(self._h, view, _p(params), _p(c)),
+                       "trainer_pair_counts")
+        return {k: int(v) for k, v in zip(self.PAIR_COUNTERS, c)}
 
     def reserve_entries(self, entries):
         self.ctx.check(lib().splat_trainer_reserve_entries(self._h, entries),

@@ -490,6 +490,17 @@ void raster_backward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb
                      const uint32_t* n_contrib, const float* dimage, bool ordered,
                      float* slots, bool need_abs, const RasterVariant& rv = {},
                      const BatchDev* bd = nullptr, uint32_t* counter = nullptr);
+// The forward and tree-reduced backward compositing of one view with pair counters
+// (9 words at `pairs`; the forward clears them): forward visits, active visits, evaluated,
+// passing and composited pairs; backward visits, active visits, evaluated and passing
+// pairs.  They write what the uncounted kernels write.
+void raster_forward_counted(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                            uint32_t width, uint32_t height, const float bg[3], float* image,
+                            float* final_t, uint32_t* n_contrib, unsigned long long* pairs);
+void raster_backward_counted(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                             uint32_t width, uint32_t height, const float bg[3],
+                             const float* final_t, const uint32_t* n_contrib,
+                             const float* dimage, float* slots, unsigned long long* pairs);
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,
                   splat_proj_grad* grads, float* abs_grads, const StepStatus* status = nullptr);
 // Autotuner of the compositing launch shapes: times every candidate (best of 3, HIP events)

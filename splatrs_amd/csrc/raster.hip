@@ -159,16 +159,35 @@ __device__ __forceinline__ bool evaluate(float4 r0, float4 r1, float px, float p
   return true;
 }
 
+// Pair counters of the counting instantiations (COUNT = true), one 64-bit word each.
+// Forward: (entry, wave) visits, visits in which some lane evaluates the Gaussian,
+// evaluated pairs (sigma in [0, sigma_max]: the exponential is computed), passing pairs
+// (alpha >= 1/255 on a pixel still compositing) and composited pairs.  Backward: visits,
+// active visits, evaluated and passing pairs of the reverse walk.
+enum PairCounter : int {
+  kFwdVisits, kFwdActive, kFwdEvaluated, kFwdPassing, kFwdComposited,
+  kBwdVisits, kBwdActive, kBwdEvaluated, kBwdPassing, kNumPairCounters
+};
+
+// Adds a wave's counts to the global counters (lane 0; the counts are wave-uniform).
+__device__ __forceinline__ void flush_counts(unsigned long long* pairs, int first,
+                                             const uint32_t* c, int n) {
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < n; q++)
+      if (c[q]) atomicAdd(&pairs[first + q], (unsigned long long)c[q]);
+}
+
 // B: entries per LDS batch; BATCH: tiles of many views (BatchDev); PERSIST: a fixed grid
-// that takes tiles from `counter` until `num_tiles` are done.
-template <int B, bool BATCH, bool PERSIST>
+// that takes tiles from `counter` until `num_tiles` are done; COUNT: also count pairs
+// into `pairs` (PairCounter).
+template <int B, bool BATCH, bool PERSIST, bool COUNT = false>
 __global__ void __launch_bounds__(256) raster_fwd_kernel(
     const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
     const uint32_t* __restrict__ order,
     const uint32_t* __restrict__ sorted_gauss, uint32_t num_tiles, uint32_t tiles_x, uint32_t W,
     uint32_t H, float bg0, float bg1, float bg2, float* __restrict__ image,
     float* __restrict__ final_t, uint32_t* __restrict__ n_contrib, BatchDev bd,
-    uint32_t* __restrict__ counter) {
+    uint32_t* __restrict__ counter, unsigned long long* __restrict__ pairs) {
   static_assert(B <= 256, "each of the 256 lanes stages at most one entry of a batch");
   __shared__ uint64_t tab[32];
   __shared__ float4 s0[B], s1[B], s2[B];
@@ -188,6 +207,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
   float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
   uint32_t nc = 0;
   bool done = !valid;
+  uint32_t pcount[5] = {0, 0, 0, 0, 0};
   const uint32_t start = tile_start[tile], end = tile_start[tile + 1];
   for (uint32_t b0 = start; b0 < end; b0 += B) {
     if (__syncthreads_and(done ? 1 : 0)) break;
@@ -216,7 +236,15 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
       const float dy = py - a.y;
       const float sigma = 0.5f * (a.z * dx * dx + b.x * dy * dy) + a.w * dx * dy;
       const bool pass1 = (!done) & in_rect(srect[k], i, j) & !((sigma < 0.0f) | (sigma > b.z));
-      if (!__any(pass1)) continue;
+      if (COUNT) {
+        const uint64_t b1 = __ballot(pass1);
+        pcount[0]++;
+        if (!b1) continue;
+        pcount[1]++;
+        pcount[2] += (uint32_t)__popcll(b1);
+      } else if (!__any(pass1)) {
+        continue;
+      }
       const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
       const float og = b.y * G;
       const bool pass2 = pass1 & !(og < kAlphaMin);
@@ -225,6 +253,10 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
       const bool stop = test_t < kTMin;
       done = done | (pass2 & stop);
       const bool comp = pass2 & !stop;
+      if (COUNT) {
+        pcount[3] += (uint32_t)__popcll(__ballot(pass2));
+        pcount[4] += (uint32_t)__popcll(__ballot(comp));
+      }
       const float4 c = s2[k];
       const float w = alpha * T;
       c0 = comp ? c0 + w * c.y : c0;
@@ -243,6 +275,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
     final_t[pix] = T;
     n_contrib[pix] = nc;
   }
+  if (COUNT) flush_counts(pairs, kFwdVisits, pcount, 5);
   if (!PERSIST) break;
   }
 }
@@ -332,20 +365,31 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
 //    diagonal conic slots once per entry, when the batch's slots are written;
 //  * products with a zero factor stand in for the per-slot selects.
 // The pixel state uses acc0..2 for A; lc and last_alpha are unused.
-template <bool ABS>
+// COUNT: cnt[0..3] += visit, active visit, evaluated and passing pairs.
+template <bool ABS, bool COUNT = false>
 __device__ __forceinline__ bool pair_terms_fast(PixBack& s, bool act, float4 r0, float4 r1,
                                                 float4 r2, float px, float py,
-                                                const uint64_t* tab, bool bg, float v[kSlots]) {
+                                                const uint64_t* tab, bool bg, float v[kSlots],
+                                                uint32_t* cnt = nullptr) {
   const float dx = px - r0.x;
   const float dy = py - r0.y;
   const float sigma = 0.5f * (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;
   const bool p1 = act & !((sigma < 0.0f) | (sigma > r1.z));
-  if (!__any(p1)) return false;
+  if (COUNT) {
+    const uint64_t b1 = __ballot(p1);
+    cnt[0]++;
+    if (!b1) return false;
+    cnt[1]++;
+    cnt[2] += (uint32_t)__popcll(b1);
+  } else if (!__any(p1)) {
+    return false;
+  }
   const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
   const float o = r1.y;
   const float og = o * G;
   const bool p2 = p1 & !(og < kAlphaMin);
   const bool p3 = p2 & (og < kAlphaMax);
+  if (COUNT) cnt[3] += (uint32_t)__popcll(__ballot(p2));
   const float alpha = rmin(og, kAlphaMax);
   const float den = 1.0f - alpha;
   const float inv = __builtin_amdgcn_rcpf(den);
@@ -454,7 +498,8 @@ __device__ __forceinline__ float row_allsum16(float v) {
 // Otherwise: 16-lane row sums by DPP, 16 row partials per entry summed in a fixed order.
 // ABS: also accumulate the AbsGS slots (9, 10); otherwise they are written as zero.
 // BF: entries per LDS batch of the tree-reduced form (the ordered form uses kOrdBatch).
-template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST>
+// COUNT (tree-reduced form only): also count pairs into `pairs` (PairCounter).
+template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST, bool COUNT = false>
 __global__ void __launch_bounds__(256) raster_bwd_kernel(
     const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
     const uint32_t* __restrict__ order,
@@ -462,7 +507,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     uint32_t num_tiles, uint32_t tiles_x, uint32_t W, uint32_t H, float bg0, float bg1,
     float bg2, const float* __restrict__ final_t, const uint32_t* __restrict__ n_contrib,
     const float* __restrict__ dimage, float* __restrict__ slots, BatchDev bd,
-    uint32_t* __restrict__ counter) {
+    uint32_t* __restrict__ counter, unsigned long long* __restrict__ pairs) {
+  static_assert(!(COUNT && ORDERED), "pairs are counted in the tree-reduced form");
   constexpr int B = ORDERED ? kOrdBatch : BF;
   __shared__ uint64_t tab[32];
   __shared__ float4 s0[B], s1[B], s2[B];
@@ -509,6 +555,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   }
   const uint32_t start = tile_start[tile], end = tile_start[tile + 1];
   const uint32_t count = end - start;
+  uint32_t pcount[4] = {0, 0, 0, 0};
   __syncthreads();
   if (valid && s.last > 0) atomicMax(&smax_last, s.last);
   __syncthreads();
@@ -566,7 +613,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         const uint32_t k = lo + kk;
         float v[kSlots];
         const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
-        if (pair_terms_fast<ABS>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, bg_any, v)) {
+        if (pair_terms_fast<ABS, COUNT>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, bg_any, v,
+                                        pcount)) {
           float r = row_scatter8(v, lane);
           const float e8 = row_allsum16(v[8]);
           r = l16 == 15 ? e8 : r;
@@ -609,6 +657,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       slots[(size_t)sorted_entry[start + lo + kk] * SS + slot] = acc;
     }
   }
+  if (COUNT) flush_counts(pairs, kBwdVisits, pcount, 4);
   if (!PERSIST) break;
   }
 }
@@ -688,7 +737,7 @@ void fwd_go(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, unsigne
   launch(ctx, "raster_fwd", raster_fwd_kernel<B, BATCH, PERSIST>, dim3(grid), dim3(256), 0,
          (const float4*)pb.rec, (const uint32_t*)bb.tile_start, (const uint32_t*)bb.tile_order,
          (const uint32_t*)bb.sorted_gauss, T, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
-         image, final_t, n_contrib, bd, counter);
+         image, final_t, n_contrib, bd, counter, (unsigned long long*)nullptr);
 }
 
 template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST>
@@ -702,7 +751,8 @@ void bwd_go(splat_ctx* ctx, const char* name, const ProjBuffers& pb, const BinBu
          0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start,
          (const uint32_t*)bb.tile_order, (const uint32_t*)bb.sorted_gauss,
          (const uint32_t*)bb.sorted_entry, T, bb.tiles_x, width,
-         height, bg[0], bg[1], bg[2], final_t, n_contrib, dimage, slots, bd, counter);
+         height, bg[0], bg[1], bg[2], final_t, n_contrib, dimage, slots, bd, counter,
+         (unsigned long long*)nullptr);
 }
 
 template <bool BATCH, bool PERSIST>
@@ -761,6 +811,30 @@ void bwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
 }
 
 }  // namespace
+
+void raster_forward_counted(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                            uint32_t width, uint32_t height, const float bg[3], float* image,
+                            float* final_t, uint32_t* n_contrib, unsigned long long* pairs) {
+  const unsigned T = bb.tiles_x * bb.tiles_y;
+  fill_u32(ctx, reinterpret_cast<uint32_t*>(pairs), 2 * kNumPairCounters, 0u);
+  launch(ctx, "raster_fwd_count", raster_fwd_kernel<64, false, false, true>, dim3(T), dim3(256),
+         0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start, (const uint32_t*)nullptr,
+         (const uint32_t*)bb.sorted_gauss, T, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
+         image, final_t, n_contrib, BatchDev{}, (uint32_t*)nullptr, pairs);
+}
+
+void raster_backward_counted(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb,
+                             uint32_t width, uint32_t height, const float bg[3],
+                             const float* final_t, const uint32_t* n_contrib,
+                             const float* dimage, float* slots, unsigned long long* pairs) {
+  const unsigned T = bb.tiles_x * bb.tiles_y;
+  launch(ctx, "raster_bwd_count", raster_bwd_kernel<false, false, 32, false, false, true>,
+         dim3(T), dim3(256), 0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start,
+         (const uint32_t*)nullptr, (const uint32_t*)bb.sorted_gauss,
+         (const uint32_t*)bb.sorted_entry, T, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
+         (const float*)final_t, (const uint32_t*)n_contrib, dimage, slots, BatchDev{},
+         (uint32_t*)nullptr, pairs);
+}
 
 void raster_forward(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, uint32_t width,
                     uint32_t height, const float bg[3], float* image, float* final_t,

@@ -807,6 +807,39 @@ int splat_trainer_tune(splat_trainer* t, uint32_t view, const splat_step_params*
   });
 }
 
+int splat_trainer_pair_counts(splat_trainer* t, uint32_t view, const splat_step_params* p,
+                              uint64_t counts[9]) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(p && counts && view < t->views.size(), "bad argument");
+    SPLAT_REQUIRE(p->sh_degree >= 0 && p->sh_degree <= 3, "sh_degree must be 0..3");
+    drain(t);
+    splat_ctx* ctx = t->ctx;
+    const TrainerView& tv = *t->views[view];
+    const uint32_t W = tv.cam.width, H = tv.cam.height;
+    const size_t npx = (size_t)W * H;
+    // the compositing half of a step; the model is not changed
+    ProjBuffers pb = proj_buffers(t);
+    project_forward(ctx, t->params.as<float>(), t->n, t->stride, tv.cam, step_opts(*p), pb);
+    BinBuffers bb{};
+    const size_t E = bin_build(ctx, pb, t->n, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile,
+                               bb, &t->entry_cap, t->store, nullptr);
+    float* image = (float*)t->store["image"].reserve(npx * 12 + 4);
+    float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
+    uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
+    float* dimage = (float*)t->store["dimage"].reserve(npx * 12 + 4);
+    float* scratch = (float*)t->store["loss_scratch"].reserve(loss_scratch_floats(W, H) * 4);
+    float* slots = (float*)t->store["slots"].reserve((std::max(E, t->entry_cap) + 1) * kSlots * 4);
+    auto* pairs = (unsigned long long*)t->store["pair_counts"].reserve(16 * 8);
+    raster_forward_counted(ctx, pb, bb, W, H, p->background, image, final_t, n_contrib, pairs);
+    loss_forward_backward(ctx, image, nullptr, tv.rgb.as<uint8_t>(), W, H, p->lambda_dssim, true,
+                          dimage, scratch, nullptr);
+    raster_backward_counted(ctx, pb, bb, W, H, p->background, final_t, n_contrib, dimage, slots,
+                            pairs);
+    SPLAT_HIP(hipMemcpyAsync(counts, pairs, 9 * 8, hipMemcpyDeviceToHost, ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 int splat_trainer_set_shapes(splat_trainer* t, const splat_tune_result* shapes) {
   return tguard(t, [&] {
     SPLAT_REQUIRE(shapes, "null argument");

@@ -38,6 +38,7 @@ def lib():
             "oracle_gaussian_kernel_f32": (None, [P]),
             "oracle_ssim_with_grad_f32": (i32, [P, P, u32, u32, P, P]),
             "oracle_photometric_loss_f32": (i32, [P, P, u32, u32, f32, P, P]),
+            "oracle_raster_pair_counts": (i32, [P, u64, P, P, P, P]),
             "oracle_loss_sums_f32": (i32, [P, P, u32, u32, P]),
             "oracle_psnr_f32": (f32, [P, P, u64]),
             "oracle_adam_step_f32": (i32, [P, P, P, P, u64, P, P]),
@@ -144,6 +145,18 @@ def rasterize_backward(projs, binning, settings, final_t, n_contrib, dimage):
                                               ptr(final_t), ptr(n_contrib), ptr(dimage),
                                               ptr(grads), ptr(absg)))
     return grads, absg
+
+
+def raster_pair_counts(projs, binning, settings, n_contrib):
+    """Pair counts of the compositing walks: forward evaluated, passing and composited
+    pairs; backward evaluated and passing pairs (a dict)."""
+    c = np.zeros(5, np.uint64)
+    s = binning.as_struct()
+    check(lib().oracle_raster_pair_counts(ptr(projs), len(projs), C.byref(s), ptr(settings),
+                                          ptr(np.ascontiguousarray(n_contrib, np.uint32)),
+                                          ptr(c)))
+    keys = ("fwd_evaluated", "fwd_passing", "fwd_composited", "bwd_evaluated", "bwd_passing")
+    return {k: int(v) for k, v in zip(keys, c)}
 
 
 def rasterize_backward_truth(projs, binning, settings, n_contrib, dimage):

@@ -174,3 +174,35 @@ def test_loss_gradient_1080p_bitwise(ctx, problem):
     say(f"ssim {s!r}, oracle {so!r}, binary64 sum {s64!r}")
     assert abs(s - s64) <= 1e-5 * abs(s64)
     assert abs(s - s64) <= abs(so - s64) + 1e-7 * abs(s64)
+
+
+def test_pair_counts_1080p_equal_the_oracle(ctx, problem):
+    """The counting compositing kernels at 1M / 1080p: forward evaluated, passing and
+    composited pairs and backward evaluated and passing pairs equal the oracle's walk; the
+    backward passes exactly the composited pairs."""
+    g, cam, rgb, extent = problem
+    w, h = int(cam["width"]), int(cam["height"])
+    tr = abi.Trainer(ctx, g)
+    tr.add_view(cam, rgb)
+    p, _ = step_params(1, extent, False)
+    got = tr.pair_counts(0, p)
+    tr.close()
+    say(f"device pair counts: {got}")
+    opts = rec.proj_opts(3)
+    po = O.project_all(g, cam, opts)
+    bo = O.bin(po, w, h)
+    st = rec.raster_settings(w, h)
+    _, _, no = O.rasterize_forward(po, bo, st)
+    want = O.raster_pair_counts(po, bo, st, no)
+    say(f"oracle pair counts: {want}")
+    for k in ("fwd_passing", "fwd_composited", "bwd_passing"):
+        assert got[k] == want[k], k
+    # sigma_max = ln(255 o): where the device's correctly rounded log and glibc's logf
+    # differ by an ulp, a pair at sigma = sigma_max is evaluated on one side only (and fails
+    # alpha_min on both)
+    for k in ("fwd_evaluated", "bwd_evaluated"):
+        assert abs(got[k] - want[k]) <= 1e-6 * want[k], k
+    assert got["bwd_passing"] == got["fwd_composited"]
+    for d in ("fwd", "bwd"):
+        assert got[f"{d}_active_visits"] <= got[f"{d}_visits"]
+        assert got[f"{d}_active_visits"] <= got[f"{d}_evaluated"] <= 64 * got[f"{d}_active_visits"]

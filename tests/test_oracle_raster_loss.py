@@ -333,3 +333,56 @@ def test_accuracy_bar_rejects_a_dropped_row_partial():
     bad[rows] *= np.float32(15.0 / 16.0)
     with pytest.raises(AssertionError):
         no_less_accurate("dropped partial", bad, ref, truth, say=lambda m: None)
+
+
+def test_pair_counts_match_a_naive_compositor():
+    # The oracle's pair counts against a per-pixel walk over every Gaussian whose tile
+    # rectangle covers the pixel's tile (tests/pipeline.rs:134-186 with counters): passing
+    # pairs (alpha >= 1/255 before the pixel is done) and composited pairs.  The backward
+    # walk (k < n_contrib) passes exactly the composited pairs.
+    rng = np.random.default_rng(21)
+    w, h = 40, 28
+    gt = ground_truth(rng, 60)
+    from helpers import ref_test_camera
+    cam = ref_test_camera(w, h)
+    opts = rec.proj_opts(1)
+    projs = O.project_all(gt, cam, opts)
+    b = O.bin(projs, w, h)
+    st = rec.raster_settings(w, h)
+    _, _, nc = O.rasterize_forward(projs, b, st)
+    got = O.raster_pair_counts(projs, b, st, nc)
+    f = np.float32
+    vis = [i for i in range(len(projs)) if projs[i]["radius"] > 0]
+    bits = projs["depth"].astype(np.float32).view(np.uint32)
+    order = sorted(vis, key=lambda i: (int(bits[i]), i))
+    passing = composited = 0
+    for y in range(h):
+        for x in range(w):
+            tx, ty = x // 16, y // 16
+            px, py = f(x) + f(0.5), f(y) + f(0.5)
+            t = f(1.0)
+            for i in order:
+                p = projs[i]
+                if not (p["tile_min"][0] <= tx < p["tile_max"][0] and
+                        p["tile_min"][1] <= ty < p["tile_max"][1]):
+                    continue
+                m, cn, o = p["mean2d"].astype(f), p["conic"].astype(f), f(p["opacity"])
+                dx, dy = f(px - m[0]), f(py - m[1])
+                sigma = f(f(f(0.5) * f(f(f(cn[0] * dx) * dx) + f(f(cn[2] * dy) * dy))) +
+                          f(f(cn[1] * dx) * dy))
+                if sigma < 0:
+                    continue
+                og = f(o * np.exp(f(-sigma), dtype=np.float32))
+                if og < f(f(1.0) / f(255.0)):
+                    continue
+                passing += 1
+                test_t = f(t * f(f(1.0) - min(og, f(0.99))))
+                if test_t < f(1e-4):
+                    break
+                composited += 1
+                t = test_t
+    assert composited > 100
+    assert got["fwd_passing"] == passing
+    assert got["fwd_composited"] == composited
+    assert got["bwd_passing"] == composited
+    assert got["fwd_evaluated"] >= passing and got["bwd_evaluated"] >= got["bwd_passing"]
