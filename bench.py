@@ -11,7 +11,9 @@ resident in HBM.  `value` is iterations/s over K timed steps with inputs residen
 
 A second pass of K steps runs with HIP events around every kernel (on the trainer's
 stream) to attribute time per kernel; the roofline entry uses the dominant kernel's
-algorithmic bytes per launch over its mean duration.  The CPU baseline runs the oracle's
+algorithmic bytes per launch over its mean duration.  Counting instantiations of the
+compositing kernels give the exact evaluated (Gaussian, pixel) pairs, which put the
+compositing kernels on the FP32 roofline as well.  The CPU baseline runs the oracle's
 train step (a scalar C++ restatement of the reference) on the host.
 """
 import argparse
@@ -25,7 +27,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "train iters/s @1M Gaussians 1080p (plain vs batched+learned) at PSNR parity"
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_GINST = 256 * 2.4  # CUs x GHz: wave64 VALU instructions per second (G)
+FP32_PEAK_TFLOPS = 157.3  # MI355X vector FP32
+VALU_PEAK_GINST = 256 * 2.4  # CUs x GHz: wave64 VALU quad-cycles per second per SIMD (G)
+# Algorithmic FP32 operations per evaluated (Gaussian, pixel) pair of the compositing walks
+# (the reference's evaluate / forward_tile / backward_tile arithmetic, src/rasterize.rs:
+# 231-290, 394-448): about 30 in the forward and 70 in the backward.
+FLOPS_PER_PAIR = {"raster_fwd": 30, "raster_bwd": 70}
 
 
 def algorithmic_bytes(name, N, V, E, P):
@@ -362,26 +369,32 @@ def main():
         t = json.load(open(tpath))["kernels"].get(roof_name)
         if t:
             traffic = int(t["read_bytes"] + t["write_bytes"])
-            if "valu_insts" in t:
+            if "valu_insts" in t and "valu_dual_quadcycles" in t:
                 # The compositing kernels are bound by instruction issue, not by HBM: wave64
-                # VALU instructions per launch (rocprofv3 SQ_INSTS_VALU on this workload) per
-                # second of this run, against one instruction per SIMD per quad-cycle (256 CUs
-                # x 4 SIMDs at 2.4 GHz, the non-packed FP32 rate of the 157.3 TFLOP/s peak).
-                # gfx950 issues a second VALU instruction in some quad-cycles
-                # (SQ_ACTIVE_INST_VALU2 counts them), so `frac` can pass 1; `busy_frac` is
-                # the share of SIMD quad-cycles that issued at least one VALU instruction.
+                # VALU instructions per launch (rocprofv3 SQ_INSTS_VALU on this workload),
+                # of which some issue as a second instruction in a quad-cycle
+                # (SQ_ACTIVE_INST_VALU2).  busy_frac = quad-cycles that issued a VALU
+                # instruction / (256 CUs x 4 SIMDs x 2.4 GHz x launch time), at most 1.
                 rate = t["valu_insts"] / mean_s / 1e9
-                valu = {"insts_per_launch": int(t["valu_insts"]), "achieved": round(rate, 1),
-                        "peak": VALU_PEAK_GINST, "unit": "G wave-instructions/s",
-                        "frac": round(rate / VALU_PEAK_GINST, 3)}
-                if "valu_dual_quadcycles" in t:
-                    single = 1.0 - t["valu_dual_quadcycles"] / t["valu_insts"]
-                    valu["dual_issue_quadcycles_per_launch"] = int(t["valu_dual_quadcycles"])
-                    valu["busy_frac"] = round(rate * single / VALU_PEAK_GINST, 3)
+                single = 1.0 - t["valu_dual_quadcycles"] / t["valu_insts"]
+                valu = {"insts_per_launch": int(t["valu_insts"]),
+                        "dual_issue_quadcycles_per_launch": int(t["valu_dual_quadcycles"]),
+                        "busy_frac": round(rate * single / VALU_PEAK_GINST, 3)}
+    # exact (Gaussian, pixel) pair counts of this view (counting kernels, outside the timed
+    # region): the compositing kernels' FP32 roofline
+    pairs = trainer.pair_counts(view, params(it + 1))
+    evaluated = {"raster_fwd": pairs["fwd_evaluated"], "raster_bwd": pairs["bwd_evaluated"]}
+    for name, ops in FLOPS_PER_PAIR.items():
+        if name in kernels:
+            tf = evaluated[name] * ops / (kernels[name]["mean_ms"] * 1e-3) / 1e12
+            kernels[name]["fp32"] = {"evaluated_pairs": evaluated[name], "ops_per_pair": ops,
+                                     "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS,
+                                     "unit": "TFLOP/s", "frac": round(tf / FP32_PEAK_TFLOPS, 4)}
     roofline = {"kernel": roof_name, "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes": int(b),
-                "mean_ms": kernels[roof_name]["mean_ms"], "valu_issue": valu}
+                "mean_ms": kernels[roof_name]["mean_ms"],
+                "fp32": kernels[roof_name].get("fp32"), "valu_issue": valu}
 
     cpu = None
     if args.cpu_steps > 0:
@@ -424,7 +437,7 @@ def main():
                    "step_mode": "hipGraph replay",
                    "eager_it_per_s": round(args.steps / eager_sec, 3),
                    "graph_captures_replays_aborts": list(graph_stats),
-                   "tuned_shapes": tuned},
+                   "tuned_shapes": tuned, "pair_counts": pairs},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,
