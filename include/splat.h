@@ -222,6 +222,20 @@ int splat_photometric_loss(splat_ctx* ctx, const float* pred, const float* targe
                            uint32_t width, uint32_t height, float lambda_dssim,
                            float* loss, float* grad);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight evaluation loops, harnesses that forbid hipMalloc).
+ * splat_photometric_loss on device buffers: pred, target, grad are device pointers of
+ * width*height*3 floats; loss (device, one float) may be NULL.  Runs on the context's
+ * stream and returns without waiting.  Nothing is allocated: the kernels' scratch is one
+ * static device array of 73 MiB per GPU, held until the process ends, which limits images
+ * to width*height <= 2097152 pixels (1920x1080 fits; larger: SPLAT_ERR_INVALID).  Every call
+ * on one GPU shares that array, so calls that overlap on one GPU (from any context or
+ * stream) race: serialise them.  Results equal splat_photometric_loss's bitwise. */
+int splat_photometric_loss_device_noalloc(splat_ctx* ctx, const float* pred,
+                                          const float* target, uint32_t width,
+                                          uint32_t height, float lambda_dssim, float* loss,
+                                          float* grad);
+
 /* ssim_with_grad, src/loss.rs:75-134. */
 int splat_ssim_with_grad(splat_ctx* ctx, const float* x, const float* y, uint32_t width,
                          uint32_t height, float* ssim, float* grad);

@@ -146,6 +146,7 @@ _SIGS = {
     "splat_render": (i32, [P, P, u64, P, P, P, P]),
     "splat_to_rgb8": (i32, [P, u64, P]),
     "splat_photometric_loss": (i32, [P, P, P, u32, u32, f32, P, P]),
+    "splat_photometric_loss_device_noalloc": (i32, [P, P, P, u32, u32, f32, P, P]),
     "splat_ssim_with_grad": (i32, [P, P, P, u32, u32, P, P]),
     "splat_blur": (i32, [P, P, u32, u32, u32, P]),
     "splat_psnr": (i32, [P, P, u64, P]),
@@ -383,6 +384,15 @@ class Context:
                                                 lambda_dssim, _p(loss), _p(grad)),
                    "photometric_loss")
         return float(loss[0]), grad
+
+    def photometric_loss_device_noalloc(self, pred_ptr, target_ptr, width, height,
+                                        lambda_dssim, loss_ptr, grad_ptr):
+        """splat_photometric_loss_device_noalloc on device addresses (integers): enqueued on
+        the context's stream; nothing is allocated.  Call sync() before reading."""
+        self.check(lib().splat_photometric_loss_device_noalloc(
+            self._h, C.c_void_p(pred_ptr), C.c_void_p(target_ptr), width, height,
+            lambda_dssim, C.c_void_p(loss_ptr) if loss_ptr else None, C.c_void_p(grad_ptr)),
+            "photometric_loss_device_noalloc")
 
     def ssim_with_grad(self, x, y, width, height):
         """ssim_with_grad, src/loss.rs:75-134 -> (ssim, grad)."""

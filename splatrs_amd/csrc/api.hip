@@ -168,6 +168,12 @@ int splat_ctx_create(int device, splat_ctx** out) {
     delete ctx;
     return SPLAT_ERR_HIP;
   }
+  // this device's address of the no-allocation entry point's scratch array
+  if (splat::noalloc_setup(device) != SPLAT_OK) {
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SPLAT_ERR_HIP;
+  }
   *out = ctx;
   return SPLAT_OK;
 }

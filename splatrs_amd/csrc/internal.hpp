@@ -527,6 +527,11 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
                            float lambda, bool want_l1, float* dimage, float* scratch,
                            double* loss_parts_host, const StepDyn* dyn = nullptr);
 size_t loss_scratch_floats(uint32_t width, uint32_t height);
+hipError_t loss_launch_bare(hipStream_t stream, const float* pred, const float* target,
+                            uint32_t width, uint32_t height, float lambda, float* dimage,
+                            float* loss_out, float* scratch);
+// Resolves this device's address of the no-allocation scratch array (once per device).
+int noalloc_setup(int device);
 // Blocks of the loss kernels for one image (the batched grid is the sum over pairs).
 unsigned loss_blocks(uint32_t width, uint32_t height);
 // Batched: loss and dL/dimage of every pair in two launches; scratch holds 3 maps of

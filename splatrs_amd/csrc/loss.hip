@@ -471,6 +471,37 @@ void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_
          reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, out_dev, dyn);
 }
 
+// Both passes and the loss value on caller-provided device buffers, with `scratch` of
+// loss_scratch_floats(width, height) floats: plain launches on `stream` (no timing events,
+// no host-side state), so nothing is allocated on the device or the host.
+hipError_t loss_launch_bare(hipStream_t stream, const float* pred, const float* target,
+                            uint32_t width, uint32_t height, float lambda, float* dimage,
+                            float* loss_out, float* scratch) {
+  const size_t n = (size_t)width * height * 3;
+  Taps taps;
+  gaussian_kernel_host(taps.k);
+  const float c1 = 0.01f * 0.01f, c2 = 0.03f * 0.03f;
+  float* d_mu = scratch;
+  float* d_exx = scratch + n;
+  float* d_exy = scratch + 2 * n;
+  float2* partial = reinterpret_cast<float2*>(scratch + 3 * n);
+  const dim3 grid(blocks_for((size_t)width * 3, kStrip), blocks_for(height, kRowsPerBlock));
+  const float inv_n = 1.0f / (float)n;
+  const float w1 = (1.0f - lambda) * inv_n;
+  hipLaunchKernelGGL((ssim_stats_kernel<false, false>), grid, dim3(256), 0, stream, pred,
+                     (const void*)target, width, height, taps, c1, c2, d_mu, d_exx, d_exy,
+                     partial, (const StepDyn*)nullptr, BatchDev{});
+  hipLaunchKernelGGL((ssim_grad_kernel<false, 0, false>), grid, dim3(256), 0, stream, pred,
+                     (const void*)target, width, height, taps, (const float*)d_mu,
+                     (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage,
+                     (const StepDyn*)nullptr, BatchDev{});
+  if (loss_out)
+    hipLaunchKernelGGL(loss_value_kernel, dim3(1), dim3(256), 0, stream,
+                       (const float2*)partial, grid.x * grid.y, lambda, (float)n, loss_out,
+                       (const StepDyn*)nullptr);
+  return hipGetLastError();
+}
+
 // gaussian_kernel, src/loss.rs:14-27 (host, glibc-compatible expf).
 void gaussian_kernel_host(float k[11]) {
   const float sigma = 1.5f;

@@ -173,3 +173,39 @@ def no_less_accurate(name, fast, ref, truth, say=print, slack=ACCURACY_SLACK):
     for q in qf:
         assert qf[q] <= slack * qr[q] + 1e-12, (name, q, qf[q], qr[q])
     return qf, qr
+
+
+class DeviceArrays:
+    """Device buffers for tests of entry points that take device pointers.  The HIP calls
+    are looked up through libsplat's own handle (dlsym searches its dependencies), so
+    buffers and kernels share the runtime libsplat is bound to, whichever copy of it the
+    process loaded first.  upload() returns a device address (int)."""
+
+    def __init__(self):
+        import ctypes
+        from splatrs_amd import abi
+        self.C = ctypes
+        self.hip = abi.lib()
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_int]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        p = self.C.c_void_p()
+        assert self.hip.hipMalloc(self.C.byref(p), max(a.nbytes, 4)) == 0
+        self.ptrs.append(p.value)
+        assert self.hip.hipMemcpy(p.value, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return p.value
+
+    def download(self, ptr, shape, dtype):
+        out = np.zeros(shape, dtype)
+        assert self.hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self):
+        for p in self.ptrs:
+            self.hip.hipFree(p)
+        self.ptrs = []
