@@ -118,6 +118,32 @@ SPLAT_HD float expf_nonpos_tab(float x, const uint64_t* tab) {
   return x < -0x1.9fe368p6f ? 0.0f : (float)y;
 }
 
+// expf_nonpos_tab on the compositing walks' arguments: x = -sigma with 0 <= sigma <=
+// sigma_max = ln(255 o) on every pixel that uses the result; sigma_max < 94.5 for any
+// finite opacity (ln 255 for a sigmoid one), above the underflow cut at -103.97, so the
+// cut's compare and select are left out.  Other lanes' results are discarded by their
+// predicates.
+SPLAT_HD float expf_walk_tab(float x, const uint64_t* tab) {
+  const double inv_ln2_n = 0x1.71547652b82fep+0 * 32.0;
+  const double shift = 0x1.8p+52;
+  const double c0 = 0x1.c6af84b912394p-5 / 32.0 / 32.0 / 32.0;
+  const double c1 = 0x1.ebfce50fac4f3p-3 / 32.0 / 32.0;
+  const double c2 = 0x1.62e42ff0c52d6p-1 / 32.0;
+  const double z = inv_ln2_n * (double)x;
+  double kd = z + shift;
+  const uint64_t ki = d2bits(kd);
+  kd -= shift;
+  const double r = z - kd;
+  const uint64_t t = tab[ki % 32] + (ki << 47);
+  const double s = bits2d(t);
+  const double zz = __builtin_fma(c0, r, c1);
+  const double r2 = r * r;
+  double y = __builtin_fma(c2, r, 1.0);
+  y = __builtin_fma(zz, r2, y);
+  y = y * s;
+  return (float)y;
+}
+
 __device__ __forceinline__ float splat_expf(float x) { return expf_tab(x, kExp2Table); }
 __device__ __forceinline__ float splat_logf(float x) { return (float)log((double)x); }
 

@@ -122,13 +122,38 @@ __device__ __forceinline__ uint32_t entry_rect(float4 r0, float4 r1, float4 r2, 
   return x0 | (x1 << 8) | (y0 << 16) | (y1 << 24);
 }
 
+// The records as staged in LDS: conic A (r0.z) and C (r1.x) halved.  Scaling by 1/2
+// commutes with every rounding of sigma = 0.5 (A dx dx + C dy dy) + B dx dy
+// (src/rasterize.rs:235-236) as long as the products stay normal: a nonzero dx or dy is
+// at least 2^-25 (an ulp of a coordinate in [0.5, 2^24)), so for conic entries above 2^-76
+// (2D covariances below about 10^22 px^2) (A/2) dx dx + (C/2) dy dy is exactly half of
+// A dx dx + C dy dy and the walks get the reference's sigma bit for bit with one multiply
+// less.  Uses of the full A and C double them back (exact).
+__device__ __forceinline__ float4 half_conic0(float4 r0) {
+  r0.z *= 0.5f;
+  return r0;
+}
+__device__ __forceinline__ float4 half_conic1(float4 r1) {
+  r1.x *= 0.5f;
+  return r1;
+}
+
 // Pixel of thread t in the 16x16 tile: wave w = t / 64 covers the 8x8 block
-// (w % 2, w / 2) and lane l its pixel (l % 8, l / 8).  Square blocks let the rectangle
-// test below cull in both directions (10% fewer (entry, wave) pairs than 4x16 strips).
+// (w % 2, w / 2); its 16-lane row r = l / 16 the 4x4 sub-block (r % 2, r / 2) of that
+// block, and lane l the pixel (l % 4, (l / 4) % 4) of the sub-block.  Square blocks let the
+// rectangle tests below cull in both directions, and the backward walks a list of entries
+// per row (the DPP reductions are per row).
 __device__ __forceinline__ void tile_pixel(uint32_t t, uint32_t* i, uint32_t* j) {
-  const uint32_t w = t >> 6, l = t & 63u;
-  *i = (w & 1u) * 8u + (l & 7u);
-  *j = (w >> 1) * 8u + (l >> 3);
+  const uint32_t w = t >> 6, r = (t >> 4) & 3u, q = t & 15u;
+  *i = (w & 1u) * 8u + (r & 1u) * 4u + (q & 3u);
+  *j = (w >> 1) * 8u + (r >> 1) * 4u + (q >> 2);
+}
+
+// Whether a packed rectangle has pixels in the 4x4 sub-block of row r of wave w.
+__device__ __forceinline__ bool row_touches(uint32_t rc, uint32_t w, uint32_t r) {
+  const uint32_t x0 = rc & 0xffu, x1 = (rc >> 8) & 0xffu, y0 = (rc >> 16) & 0xffu, y1 = rc >> 24;
+  const uint32_t bx = (w & 1u) * 8u + (r & 1u) * 4u, by = (w >> 1) * 8u + (r >> 1) * 4u;
+  return (x0 < x1) & (y0 < y1) & (x0 < bx + 4u) & (x1 > bx) & (y0 < by + 4u) & (y1 > by);
 }
 
 // Whether a packed rectangle has pixels in the 8x8 block of wave w.
@@ -215,8 +240,8 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
     if (threadIdx.x < cnt) {
       const uint32_t g = sorted_gauss[b0 + threadIdx.x];
       const float4 a = rec[4 * g], b = rec[4 * g + 1], c = rec[4 * g + 2];
-      s0[threadIdx.x] = a;
-      s1[threadIdx.x] = b;
+      s0[threadIdx.x] = half_conic0(a);
+      s1[threadIdx.x] = half_conic1(b);
       s2[threadIdx.x] = c;
       srect[threadIdx.x] = entry_rect(a, b, c, ti);
     }
@@ -234,7 +259,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
       const float4 a = s0[k], b = s1[k];
       const float dx = px - a.x;
       const float dy = py - a.y;
-      const float sigma = 0.5f * (a.z * dx * dx + b.x * dy * dy) + a.w * dx * dy;
+      const float sigma = (a.z * dx * dx + b.x * dy * dy) + a.w * dx * dy;  // halved A, C
       const bool pass1 = (!done) & in_rect(srect[k], i, j) & !((sigma < 0.0f) | (sigma > b.z));
       if (COUNT) {
         const uint64_t b1 = __ballot(pass1);
@@ -245,7 +270,7 @@ __global__ void __launch_bounds__(256) raster_fwd_kernel(
       } else if (!__any(pass1)) {
         continue;
       }
-      const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
+      const float G = expf_walk_tab(-sigma, tab);  // used where 0 <= sigma <= sigma_max
       const float og = b.y * G;
       const bool pass2 = pass1 & !(og < kAlphaMin);
       const float alpha = rmin(og, kAlphaMax);
@@ -299,10 +324,10 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
   for (int q = 0; q < kSlots; q++) v[q] = 0.0f;
   const float dx = px - r0.x;
   const float dy = py - r0.y;
-  const float sigma = 0.5f * (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;
+  const float sigma = (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;  // halved A, C
   const bool p1 = act & !((sigma < 0.0f) | (sigma > r1.z));
   if (!__any(p1)) return false;
-  const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
+  const float G = expf_walk_tab(-sigma, tab);  // used where 0 <= sigma <= sigma_max
   const float o = r1.y;
   const float og = o * G;
   const bool p2 = p1 & !(og < kAlphaMin);
@@ -320,8 +345,8 @@ __device__ __forceinline__ bool pair_terms(PixBack& s, bool act, float4 r0, floa
   const bool p3 = p2 & (og < kAlphaMax);
   const float dgauss = o * dalpha;
   const float dsigma = -dgauss * G;
-  const float ddx = dsigma * (r0.z * dx + r0.w * dy);
-  const float ddy = dsigma * (r0.w * dx + r1.x * dy);
+  const float ddx = dsigma * ((2.0f * r0.z) * dx + r0.w * dy);
+  const float ddy = dsigma * (r0.w * dx + (2.0f * r1.x) * dy);
   v[6] = p2 ? alpha * t * s.d0 : 0.0f;
   v[7] = p2 ? alpha * t * s.d1 : 0.0f;
   v[8] = p2 ? alpha * t * s.d2 : 0.0f;
@@ -373,7 +398,7 @@ __device__ __forceinline__ bool pair_terms_fast(PixBack& s, bool act, float4 r0,
                                                 uint32_t* cnt = nullptr) {
   const float dx = px - r0.x;
   const float dy = py - r0.y;
-  const float sigma = 0.5f * (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;
+  const float sigma = (r0.z * dx * dx + r1.x * dy * dy) + r0.w * dx * dy;  // halved A, C
   const bool p1 = act & !((sigma < 0.0f) | (sigma > r1.z));
   if (COUNT) {
     const uint64_t b1 = __ballot(p1);
@@ -384,7 +409,7 @@ __device__ __forceinline__ bool pair_terms_fast(PixBack& s, bool act, float4 r0,
   } else if (!__any(p1)) {
     return false;
   }
-  const float G = expf_nonpos_tab(-sigma, tab);  // -sigma <= 0 or NaN here
+  const float G = expf_walk_tab(-sigma, tab);  // used where 0 <= sigma <= sigma_max
   const float o = r1.y;
   const float og = o * G;
   const bool p2 = p1 & !(og < kAlphaMin);
@@ -419,8 +444,8 @@ __device__ __forceinline__ bool pair_terms_fast(PixBack& s, bool act, float4 r0,
   v[7] = w * s.d1;
   v[8] = w * s.d2;
   if (ABS) {
-    v[9] = fabsf(fmaf(r0.z, qdx, r0.w * qdy));
-    v[10] = fabsf(fmaf(r0.w, qdx, r1.x * qdy));
+    v[9] = fabsf(fmaf(2.0f * r0.z, qdx, r0.w * qdy));
+    v[10] = fabsf(fmaf(r0.w, qdx, (2.0f * r1.x) * qdy));
   } else {
     v[9] = v[10] = 0.0f;
   }
@@ -515,6 +540,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   __shared__ uint32_t srect[B];
   __shared__ uint32_t smax_last;
   __shared__ uint32_t s_tile;
+  // per (wave, row): the batch entries the row walks, back to front
+  __shared__ uint8_t slist[ORDERED ? 1 : 16][ORDERED ? 1 : B];
   // words per entry in `slots`: the absolute sums (slots 9, 10) only when asked for
   constexpr uint32_t SS = ABS ? kSlots : 9;
   // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums of the SS slots
@@ -565,10 +592,16 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     const uint32_t k = q / SS;
     slots[(size_t)sorted_entry[start + k] * SS + q % SS] = 0.0f;
   }
-  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  // the last contributor of any pixel of this wave
-  uint32_t wave_last = valid ? s.last : 0u;
-  for (int d = 32; d >= 1; d >>= 1) wave_last = max(wave_last, (uint32_t)__shfl_xor((int)wave_last, d, 64));
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, row = lane >> 4;
+  // the last contributor of any pixel of each 16-lane row of this wave
+  uint32_t row_last[4];
+  {
+    uint32_t m = valid ? s.last : 0u;
+#pragma unroll
+    for (int d = 8; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
+#pragma unroll
+    for (int r = 0; r < 4; r++) row_last[r] = (uint32_t)__builtin_amdgcn_readlane((int)m, 16 * r);
+  }
   // Batches walk [0, kmax) from the back.
   for (int32_t hi = (int32_t)kmax; hi > 0; hi -= B) {
     const int32_t lo = max(hi - B, 0);
@@ -577,8 +610,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
     if (threadIdx.x < cnt) {
       const uint32_t g = sorted_gauss[start + lo + threadIdx.x];
       const float4 a = rec[4 * g], b = rec[4 * g + 1], c = rec[4 * g + 2];
-      s0[threadIdx.x] = a;
-      s1[threadIdx.x] = b;
+      s0[threadIdx.x] = half_conic0(a);
+      s1[threadIdx.x] = half_conic1(b);
       s2[threadIdx.x] = c;
       srect[threadIdx.x] = entry_rect(a, b, c, ti);
     }
@@ -595,24 +628,39 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         for (int q = 0; q < kSlots; q++) red[(kk * kSlots + q) * 256 + j * kTile + i] = v[q];
       }
     } else {
-      // each wave walks, back to front, only the entries that reach its pixel block and
-      // precede its last contributor (B <= 64: one ballot covers the batch).
-      // Sums over each 16-lane row: slots 0-7 by one reduce-scatter (even lane 2m holds slot m), slot 8 (and
-      // the AbsGS slots 9, 10) by a butterfly read from lane 15 (13, 11); one LDS store per
-      // writer lane.
+      // Each 16-lane row walks, back to front, only the batch entries that reach its 4x4
+      // sub-block and precede the row's last contributor; the four rows of a wave walk
+      // their own lists side by side (B <= 64: one ballot per row covers the batch).
+      // Sums over each 16-lane row: slots 0-7 by one reduce-scatter (even lane 2m holds
+      // slot m), slot 8 (and the AbsGS slots 9, 10) by a butterfly read from lane 15 (13,
+      // 11); one LDS store per writer lane.
       const uint32_t l16 = lane & 15u;
       const bool even = (l16 & 1u) == 0;
       const bool writer = even | (l16 == 15) | (ABS & ((l16 == 13) | (l16 == 11)));
       const uint32_t wslot = even ? (l16 >> 1) : (l16 == 15 ? 8u : (l16 == 13 ? 9u : 10u));
-      const uint32_t woff = wslot * 16 + wave * 4 + (lane >> 4);
-      uint64_t todo = __ballot(lane < cnt && lo + lane < wave_last &&
-                               wave_touches(srect[min(lane, cnt - 1)], wave));
-      while (todo) {
-        const int kk = 63 - __builtin_clzll(todo);
-        todo &= ~(1ull << kk);
+      const uint32_t woff = wslot * 16 + wave * 4 + row;
+      // the rows' lists: entry kk sits at position popcount(mask >> (kk + 1))
+      uint8_t* my_list = slist[wave * 4 + row];
+      uint32_t len[4];
+      const uint32_t kl = min(lane, cnt - 1);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint64_t m = __ballot(lane < cnt && lo + lane < row_last[r] &&
+                                    row_touches(srect[kl], wave, (uint32_t)r));
+        len[r] = (uint32_t)__popcll(m);
+        if ((m >> lane) & 1ull)
+          slist[wave * 4 + r][__popcll(m >> lane) - 1] = (uint8_t)lane;
+      }
+      const uint32_t steps = max(max(len[0], len[1]), max(len[2], len[3]));
+      const uint32_t my_len = row == 0 ? len[0] : (row == 1 ? len[1] : (row == 2 ? len[2] : len[3]));
+      // the lists are written and read by this wave only
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      for (uint32_t t = 0; t < steps; t++) {
+        const bool has = t < my_len;
+        const uint32_t kk = has ? (uint32_t)my_list[t] : 0u;
         const uint32_t k = lo + kk;
         float v[kSlots];
-        const bool act = valid & (k < s.last) & in_rect(srect[kk], i, j);
+        const bool act = has & valid & (k < s.last) & in_rect(srect[kk], i, j);
         if (pair_terms_fast<ABS, COUNT>(s, act, s0[kk], s1[kk], s2[kk], px, py, tab, bg_any, v,
                                         pcount)) {
           float r = row_scatter8(v, lane);
@@ -624,7 +672,7 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
             r = l16 == 13 ? e9 : r;
             r = l16 == 11 ? e10 : r;
           }
-          if (writer) red[kk * SS * 16 + woff] = r;
+          if (writer & has) red[kk * SS * 16 + woff] = r;
         }
       }
     }
@@ -648,7 +696,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
         if (slot < 2) {
           const float sx = part_sum(0), sy = part_sum(1);
           const float4 a = s0[kk];
-          acc = slot == 0 ? -fmaf(a.z, sx, a.w * sy) : -fmaf(a.w, sx, s1[kk].x * sy);
+          acc = slot == 0 ? -fmaf(2.0f * a.z, sx, a.w * sy)
+                          : -fmaf(a.w, sx, (2.0f * s1[kk].x) * sy);
         } else if (ABS || slot < 9) {
           acc = part_sum(slot);
           if (slot == 2 || slot == 4) acc *= 0.5f;
