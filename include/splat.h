@@ -369,6 +369,27 @@ int splat_camera_from_colmap(const splat_colmap_camera* cameras, uint64_t num_ca
 int splat_camera_match_image_size(const splat_camera* camera, uint32_t width, uint32_t height,
                                   splat_camera* out);
 
+/* ---- dataset loading (src/dataset.rs:131-205, 239-250) ---------------------------------
+ * The reference decodes each image file with the `image` crate; here the caller decodes
+ * and passes RGB8 pixels, and the library does the rest of dataset::load on the GPU. */
+
+/* image::imageops::resize(.., FilterType::Lanczos3) (image 0.25) of an RGB8 image
+ * (height x width x 3) to out_width x out_height, as dataset::load downscales views
+ * (src/dataset.rs:176-180): separable, binary32 taps and sums, clamped and rounded to
+ * nearest.  Host buffers; the resampling runs on the GPU. */
+int splat_resize_rgb8(splat_ctx* ctx, const uint8_t* src, uint32_t width, uint32_t height,
+                      uint32_t out_width, uint32_t out_height, uint8_t* dst);
+/* The view size of a downscaled image, src/dataset.rs:172-175: each side divided by
+ * `downscale` and rounded, at least 1.  downscale 0: SPLAT_ERR_INVALID (:136-138). */
+int splat_downscaled_size(uint32_t width, uint32_t height, uint32_t downscale,
+                          uint32_t* out_width, uint32_t* out_height);
+/* split_eval, src/dataset.rs:239-250: is_test[i] = 1 for every hold-th view (i % hold ==
+ * 0, none when hold is 0), the training set being the others, in order. */
+int splat_split_eval(uint64_t n, uint32_t hold, uint8_t* is_test);
+/* The view order of dataset::load (src/dataset.rs:141): images sorted by name, byte-wise
+ * and stable. */
+int splat_colmap_sort_images(splat_colmap_image* images, uint64_t n);
+
 /* ---- checkpoints ---------------------------------------------------------------------
  * No counterpart in the reference, whose train loop saves only the model (.ply,
  * src/ply.rs:19-69, called from src/train.rs:275-279): a checkpoint adds the Adam moments
@@ -425,6 +446,20 @@ int splat_trainer_destroy(splat_trainer* t);
  * bytes and converted in the loss kernel (View::to_real, src/dataset.rs:23-25). */
 int splat_trainer_add_view(splat_trainer* t, const splat_camera* cam, const uint8_t* rgb,
                            uint32_t* index);
+/* The per-image step of dataset::load on a trainer (src/dataset.rs:146-185): the image's
+ * camera (camera_from_colmap at the model's resolution) matched to the decoded size
+ * (match_image_size), the view size for `downscale`, the camera resized to it
+ * (Camera::resized) and written to *used (optional), and the RGB8 pixels resized on the
+ * GPU (Lanczos3, as splat_resize_rgb8; copied when downscale is 1) into the new view. */
+int splat_trainer_add_view_downscaled(splat_trainer* t, const splat_camera* cam,
+                                      const uint8_t* rgb, uint32_t width, uint32_t height,
+                                      uint32_t downscale, splat_camera* used,
+                                      uint32_t* index);
+/* No counterpart in the reference (its views are plain host vectors): a view's camera and,
+ * if rgb is given, its RGB8 pixels (height x width x 3; capacity in bytes, else
+ * SPLAT_ERR_CAPACITY). */
+int splat_trainer_view(splat_trainer* t, uint32_t view, splat_camera* cam, uint8_t* rgb,
+                       uint64_t capacity);
 /* One iteration on view `view`: project -> bin -> composite -> loss -> backward ->
  * statistics -> Adam (src/train.rs:211-247).  In graph mode (the default) the step is a
  * replay of a captured HIP graph and returns without waiting for the GPU when r is NULL

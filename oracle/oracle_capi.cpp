@@ -188,6 +188,12 @@ int oracle_raster_pair_counts(const splat_proj* projs, uint64_t n, const splat_b
   return SPLAT_OK;
 }
 
+int oracle_resize_rgb8(const uint8_t* src, uint32_t w, uint32_t h, uint32_t ow, uint32_t oh,
+                       uint8_t* dst) {
+  resize_rgb8(src, w, h, ow, oh, dst);
+  return SPLAT_OK;
+}
+
 int oracle_blur_f32(const float* src, uint32_t w, uint32_t h, uint32_t ch, float* dst) {
   float k[SSIM_WINDOW];
   gaussian_kernel<float>(k);

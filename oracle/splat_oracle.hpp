@@ -904,6 +904,81 @@ inline void raster_pair_counts(const Proj<float>* projs, const Binning& b, uint3
   }
 }
 
+// image::imageops::resize(.., FilterType::Lanczos3) of an RGB8 image, as dataset::load
+// calls it (src/dataset.rs:176-180).  The `image` crate (0.25.10, Cargo.lock) is not
+// vendored; this restates its published resampler (imageops/sample.rs): a copy for equal
+// sizes, else a vertical pass into binary32 RGBA, then a horizontal pass that clamps to
+// [0, 255] and rounds half away from zero; per output index o the filter centre is
+// (o + 0.5) * ratio, the support 3 * max(ratio, 1), taps lanczos3((i - centre + 0.5) /
+// max(ratio, 1)) on the clamped range [floor(c - s), ceil(c + s)), normalised by their sum.
+inline float resize_lanczos3(float x) {
+  auto sinc = [](float t) {
+    if (t == 0.0f) return 1.0f;
+    float a = t * 3.14159265358979323846f;
+    return std::sin(a) / a;
+  };
+  return std::fabs(x) < 3.0f ? sinc(x) * sinc(x / 3.0f) : 0.0f;
+}
+
+inline void resize_taps(uint32_t in, uint32_t out, uint32_t o, int64_t* left_out,
+                        std::vector<float>* ws) {
+  float ratio = (float)in / (float)out;
+  float sratio = ratio < 1.0f ? 1.0f : ratio;
+  float support = 3.0f * sratio;
+  float centre = ((float)o + 0.5f) * ratio;
+  int64_t left = std::min<int64_t>(std::max<int64_t>((int64_t)std::floor(centre - support), 0),
+                                   (int64_t)in - 1);
+  int64_t right = std::min<int64_t>(
+      std::max<int64_t>((int64_t)std::ceil(centre + support), left + 1), (int64_t)in);
+  float c = centre - 0.5f;
+  ws->clear();
+  float sum = 0.0f;
+  for (int64_t i = left; i < right; i++) {
+    float w = resize_lanczos3(((float)i - c) / sratio);
+    ws->push_back(w);
+    sum += w;
+  }
+  for (auto& w : *ws) w /= sum;
+  *left_out = left;
+}
+
+inline void resize_rgb8(const uint8_t* src, uint32_t w, uint32_t h, uint32_t ow, uint32_t oh,
+                        uint8_t* dst) {
+  if (w == 0 || h == 0) {
+    std::fill(dst, dst + (size_t)ow * oh * 3, (uint8_t)0);
+    return;
+  }
+  if (w == ow && h == oh) {
+    std::copy(src, src + (size_t)w * h * 3, dst);
+    return;
+  }
+  std::vector<float> tmp((size_t)w * oh * 3);
+  std::vector<float> ws;
+  for (uint32_t oy = 0; oy < oh; oy++) {  // vertical_sample
+    int64_t left;
+    resize_taps(h, oh, oy, &left, &ws);
+    for (uint32_t x = 0; x < w; x++)
+      for (int c = 0; c < 3; c++) {
+        float t = 0.0f;
+        for (size_t i = 0; i < ws.size(); i++)
+          t += (float)src[((size_t)(left + i) * w + x) * 3 + c] * ws[i];
+        tmp[((size_t)oy * w + x) * 3 + c] = t;
+      }
+  }
+  for (uint32_t ox = 0; ox < ow; ox++) {  // horizontal_sample
+    int64_t left;
+    resize_taps(w, ow, ox, &left, &ws);
+    for (uint32_t y = 0; y < oh; y++)
+      for (int c = 0; c < 3; c++) {
+        float t = 0.0f;
+        for (size_t i = 0; i < ws.size(); i++)
+          t += tmp[((size_t)y * w + left + i) * 3 + c] * ws[i];
+        t = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t);
+        dst[((size_t)y * ow + ox) * 3 + c] = (uint8_t)std::round(t);
+      }
+  }
+}
+
 static const int SLOTS = 11;  // rasterize.rs:341
 
 // rasterize_backward (rasterize.rs:459-505) with backward_tile (rasterize.rs:354-455).

@@ -173,6 +173,12 @@ _SIGS = {
                                  C.c_size_t]),
     "splat_camera_from_colmap": (i32, [P, u64, P, P, C.c_char_p, C.c_size_t]),
     "splat_camera_match_image_size": (i32, [P, u32, u32, P]),
+    "splat_resize_rgb8": (i32, [P, P, u32, u32, u32, u32, P]),
+    "splat_downscaled_size": (i32, [u32, u32, u32, P, P]),
+    "splat_trainer_add_view_downscaled": (i32, [P, P, P, u32, u32, u32, P, P]),
+    "splat_split_eval": (i32, [u64, u32, P]),
+    "splat_trainer_view": (i32, [P, u32, P, P, u64]),
+    "splat_colmap_sort_images": (i32, [P, u64]),
     "splat_trainer_evaluate": (i32, [P, P, u32, i32, i32, P, P, P]),
     "splat_trainer_create": (i32, [P, P, u64, u64, P]),
     "splat_trainer_destroy": (i32, [P]),
@@ -394,6 +400,15 @@ class Context:
             lambda_dssim, C.c_void_p(loss_ptr) if loss_ptr else None, C.c_void_p(grad_ptr)),
             "photometric_loss_device_noalloc")
 
+    def resize_rgb8(self, image, out_width, out_height):
+        """image::imageops::resize(.., Lanczos3) of an RGB8 image on the GPU."""
+        image = np.ascontiguousarray(image, np.uint8)
+        h, w = image.shape[:2]
+        out = np.zeros((out_height, out_width, 3), np.uint8)
+        self.check(lib().splat_resize_rgb8(self._h, _p(image), w, h, out_width, out_height,
+                                           _p(out)), "resize_rgb8")
+        return out
+
     def ssim_with_grad(self, x, y, width, height):
         """ssim_with_grad, src/loss.rs:75-134 -> (ssim, grad)."""
         x = np.ascontiguousarray(x, np.float32)
@@ -603,6 +618,75 @@ def match_image_size(camera, width, height):
     return out
 
 
+def downscaled_size(width, height, downscale):
+    """The view size of dataset::load for an integer downscale (src/dataset.rs:172-175)."""
+    w, h = C.c_uint32(), C.c_uint32()
+    st = lib().splat_downscaled_size(width, height, downscale, C.byref(w), C.byref(h))
+    if st != OK:
+        raise SplatError(st, "downscale must be at least 1")
+    return w.value, h.value
+
+
+def split_eval(n, hold):
+    """split_eval (src/dataset.rs:239-250) -> (train indices, held-out indices)."""
+    t = np.zeros(n, np.uint8)
+    st = lib().splat_split_eval(n, hold, _p(t))
+    if st != OK:
+        raise SplatError(st, "split_eval")
+    idx = np.arange(n)
+    return idx[t == 0], idx[t == 1]
+
+
+def colmap_sort_images(images):
+    """The image order of dataset::load (src/dataset.rs:141): by name, stable."""
+    out = np.ascontiguousarray(images, rec.COLMAP_IMAGE).copy()
+    st = lib().splat_colmap_sort_images(_p(out), len(out))
+    if st != OK:
+        raise SplatError(st, "colmap_sort_images")
+    return out
+
+
+def locate_sparse(root, sparse=None):
+    """locate_sparse (src/dataset.rs:40-59): root/sparse/0, then root/sparse."""
+    if sparse is not None:
+        return sparse
+    for d in (os.path.join(root, "sparse", "0"), os.path.join(root, "sparse")):
+        if os.path.isdir(d):
+            return d
+    raise SplatError(ERR_INVALID, f"no sparse/0 or sparse directory under {root}")
+
+
+def locate(root, sparse=None, images=None):
+    """locate (src/dataset.rs:61-80) -> (sparse dir, images dir)."""
+    s = locate_sparse(root, sparse)
+    i = images if images is not None else os.path.join(root, "images")
+    if not os.path.isdir(i):
+        raise SplatError(ERR_INVALID, f"images directory {i} does not exist")
+    return s, i
+
+
+def load_dataset(trainer, sparse, images, downscale, decode):
+    """dataset::load (src/dataset.rs:131-205) into a trainer's views: the COLMAP model's
+    images in name order, each decoded by `decode(path) -> (h, w, 3) uint8` (the reference
+    uses the `image` crate), matched to its camera and downscaled on the GPU.  Returns
+    (views' cameras, points xyz float32, points rgb uint8, extent)."""
+    if downscale == 0:
+        raise SplatError(ERR_INVALID, "downscale must be at least 1")
+    cams, imgs, pts = colmap_read(sparse)
+    if len(imgs) == 0:
+        raise SplatError(ERR_INVALID, f"no registered images in {sparse}")
+    used = []
+    for im in colmap_sort_images(imgs):
+        cam = camera_from_colmap(cams, im)
+        name = im["name"].decode() if isinstance(im["name"], bytes) else str(im["name"])
+        used.append(trainer.add_view_downscaled(cam, decode(os.path.join(images, name)),
+                                                downscale))
+    xyz = np.ascontiguousarray(pts["xyz"], np.float64).astype(np.float32)
+    rgb = np.ascontiguousarray(pts["rgb"], np.uint8)
+    from . import scenes
+    return used, xyz, rgb, scenes.scene_extent(used)
+
+
 def to_rgb8(image):
     """to_rgb8, src/render.rs:26-31."""
     image = np.ascontiguousarray(image, np.float32)
@@ -656,6 +740,28 @@ class Trainer:
                        "trainer_add_view")
         self.views.append(cam)
         return idx.value
+
+    def add_view_downscaled(self, cam, rgb, downscale):
+        """The per-image step of dataset::load (src/dataset.rs:146-185): cam at the model's
+        resolution, rgb the decoded image; returns the view's camera."""
+        rgb = np.ascontiguousarray(rgb, np.uint8)
+        h, w = rgb.shape[:2]
+        used = np.zeros((), rec.CAMERA)
+        idx = C.c_uint32()
+        self.ctx.check(lib().splat_trainer_add_view_downscaled(
+            self._h, _p(cam), _p(rgb), w, h, downscale, _p(used), C.byref(idx)),
+            "trainer_add_view_downscaled")
+        self.views.append(used)
+        return used
+
+    def view(self, index):
+        """A view's camera and RGB8 pixels as stored on the device."""
+        cam = np.zeros((), rec.CAMERA)
+        self.ctx.check(lib().splat_trainer_view(self._h, index, _p(cam), None, 0), "trainer_view")
+        rgb = np.zeros((int(cam["height"]), int(cam["width"]), 3), np.uint8)
+        self.ctx.check(lib().splat_trainer_view(self._h, index, _p(cam), _p(rgb), rgb.nbytes),
+                       "trainer_view")
+        return cam, rgb
 
     def step(self, view, params, result=True):
         """One iteration; with result=False the call returns without waiting (graph mode)."""

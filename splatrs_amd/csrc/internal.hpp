@@ -516,6 +516,11 @@ RasterTune tune_raster(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& 
                        float* slots, bool need_abs, const BatchDev* bd, uint32_t* counter);
 void fill_tune_result(const RasterTune& t, splat_tune_result* out);
 
+// dataset.hip: image::imageops::resize(.., Lanczos3) of a device RGB8 image into dst
+// (out_w x out_h), as dataset::load downscales views; synchronises.
+void resize_rgb8_device(splat_ctx* ctx, const uint8_t* src, uint32_t w, uint32_t h,
+                        uint32_t out_w, uint32_t out_h, uint8_t* dst);
+
 // loss.hip
 struct LossOut {
   float* dimage;     // width*height*3

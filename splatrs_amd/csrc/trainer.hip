@@ -605,6 +605,65 @@ int splat_trainer_add_view(splat_trainer* t, const splat_camera* cam, const uint
   });
 }
 
+int splat_trainer_add_view_downscaled(splat_trainer* t, const splat_camera* cam,
+                                      const uint8_t* rgb, uint32_t width, uint32_t height,
+                                      uint32_t downscale, splat_camera* used,
+                                      uint32_t* index) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(cam && rgb, "null argument");
+    if (downscale == 0) fail(SPLAT_ERR_INVALID, "downscale must be at least 1");
+    // src/dataset.rs:159-184: the camera matched to the decoded image, then both scaled
+    splat_camera matched;
+    if (splat_camera_match_image_size(cam, width, height, &matched) != SPLAT_OK)
+      fail(SPLAT_ERR_INVALID, "image of " + std::to_string(width) + "x" +
+                                  std::to_string(height) + " does not match camera of " +
+                                  std::to_string(cam->width) + "x" + std::to_string(cam->height));
+    uint32_t w = 0, h = 0;
+    (void)splat_downscaled_size(matched.width, matched.height, downscale, &w, &h);
+    SPLAT_REQUIRE(w <= 65535 * kTile && h <= 65535 * kTile, "bad image size");
+    splat_camera c = matched;  // Camera::resized, src/camera.rs:48-62
+    const float sx = (float)w / (float)matched.width, sy = (float)h / (float)matched.height;
+    c.width = w;
+    c.height = h;
+    c.fx = matched.fx * sx;
+    c.fy = matched.fy * sy;
+    c.cx = matched.cx * sx;
+    c.cy = matched.cy * sy;
+    splat_ctx* ctx = t->ctx;
+    drain(t);
+    auto v = std::make_unique<TrainerView>();
+    v->cam = c;
+    v->rgb.reserve((size_t)w * h * 3);
+    const size_t in_bytes = (size_t)width * height * 3;
+    if (downscale > 1) {
+      uint8_t* staged = (uint8_t*)t->store["view_upload"].reserve(in_bytes + 4);
+      SPLAT_HIP(hipMemcpyAsync(staged, rgb, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+      resize_rgb8_device(ctx, staged, width, height, w, h, v->rgb.as<uint8_t>());
+    } else {
+      SPLAT_HIP(hipMemcpyAsync(v->rgb.ptr, rgb, in_bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+    t->views.push_back(std::move(v));
+    if (used) *used = c;
+    if (index) *index = (uint32_t)(t->views.size() - 1);
+  });
+}
+
+int splat_trainer_view(splat_trainer* t, uint32_t view, splat_camera* cam, uint8_t* rgb,
+                       uint64_t capacity) {
+  return tguard(t, [&] {
+    SPLAT_REQUIRE(view < t->views.size(), "bad view index");
+    const TrainerView& tv = *t->views[view];
+    if (cam) *cam = tv.cam;
+    if (!rgb) return;
+    const size_t bytes = (size_t)tv.cam.width * tv.cam.height * 3;
+    if (capacity < bytes) fail(SPLAT_ERR_CAPACITY, "output capacity too small");
+    drain(t);
+    SPLAT_HIP(hipMemcpyAsync(rgb, tv.rgb.ptr, bytes, hipMemcpyDeviceToHost, t->ctx->stream));
+    SPLAT_HIP(hipStreamSynchronize(t->ctx->stream));
+  });
+}
+
 int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params* p,
                        splat_step_result* r) {
   return tguard(t, [&] {

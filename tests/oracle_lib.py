@@ -39,6 +39,7 @@ def lib():
             "oracle_ssim_with_grad_f32": (i32, [P, P, u32, u32, P, P]),
             "oracle_photometric_loss_f32": (i32, [P, P, u32, u32, f32, P, P]),
             "oracle_raster_pair_counts": (i32, [P, u64, P, P, P, P]),
+            "oracle_resize_rgb8": (i32, [P, u32, u32, u32, u32, P]),
             "oracle_loss_sums_f32": (i32, [P, P, u32, u32, P]),
             "oracle_psnr_f32": (f32, [P, P, u64]),
             "oracle_adam_step_f32": (i32, [P, P, P, P, u64, P, P]),
@@ -145,6 +146,15 @@ def rasterize_backward(projs, binning, settings, final_t, n_contrib, dimage):
                                               ptr(final_t), ptr(n_contrib), ptr(dimage),
                                               ptr(grads), ptr(absg)))
     return grads, absg
+
+
+def resize_rgb8(image, out_width, out_height):
+    """image::imageops::resize(.., Lanczos3) of an RGB8 image (dataset::load's downscale)."""
+    image = np.ascontiguousarray(image, np.uint8)
+    h, w = image.shape[:2]
+    out = np.zeros((out_height, out_width, 3), np.uint8)
+    check(lib().oracle_resize_rgb8(ptr(image), w, h, out_width, out_height, ptr(out)))
+    return out
 
 
 def raster_pair_counts(projs, binning, settings, n_contrib):
