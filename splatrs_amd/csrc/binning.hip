@@ -484,18 +484,33 @@ void tile_starts(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint32_t* total
   exclusive_scan_u32(ctx, totals, bb.tile_start, T, total_dev, scan_sums);
 }
 
-// Dispatch order of the compositing kernels: tiles by decreasing entry count, in
-// kOrderBuckets buckets relative to the largest tile (order within a bucket unspecified),
-// so the largest tiles start first and the smallest fill the end of the launch.  One
-// workgroup; the result is a permutation of [0, T) whatever the counts.
+// Dispatch order of the compositing kernels.  Workgroup b of a launch runs on die
+// (accelerator complex) b % 8, each with its own L2, so the order gives die x a contiguous
+// band of tiles -- neighbouring tiles share Gaussian records -- as the blocks x, x + 8,
+// x + 16, ...: band x holds the ceil((T - x) / 8) tiles after the bands before it, and its
+// tiles go out by decreasing entry count (kOrderBuckets buckets relative to the largest
+// tile, order within a bucket unspecified) so the largest start first and the smallest
+// fill the end of the launch.  One workgroup; the result is a permutation of [0, T)
+// whatever the counts.
 constexpr uint32_t kOrderBuckets = 64;
+constexpr uint32_t kOrderBands = 8;
 
 __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ tile_start,
                                                           uint32_t T, uint32_t* __restrict__ order) {
-  __shared__ uint32_t smax, hist[kOrderBuckets], cursor[kOrderBuckets];
+  __shared__ uint32_t smax, hist[kOrderBands][kOrderBuckets], cursor[kOrderBands][kOrderBuckets];
+  __shared__ uint32_t band_lo[kOrderBands + 1];
   const uint32_t t = threadIdx.x;
-  if (t == 0) smax = 0;
-  if (t < kOrderBuckets) hist[t] = 0;
+  if (t == 0) {
+    smax = 0;
+    uint32_t run = 0;
+    for (uint32_t x = 0; x < kOrderBands; x++) {
+      band_lo[x] = run;
+      run += T > x ? (T - x + kOrderBands - 1) / kOrderBands : 0u;
+    }
+    band_lo[kOrderBands] = run;  // == T
+  }
+  for (uint32_t q = t; q < kOrderBands * kOrderBuckets; q += blockDim.x)
+    hist[q / kOrderBuckets][q % kOrderBuckets] = 0;
   __syncthreads();
   uint32_t m = 0;
   for (uint32_t q = t; q < T; q += blockDim.x) m = max(m, tile_start[q + 1] - tile_start[q]);
@@ -507,17 +522,26 @@ __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __rest
     const uint64_t c = tile_start[q + 1] - tile_start[q];
     return (uint32_t)(((mx - c) * kOrderBuckets) / (mx + 1));  // 0: the largest tiles
   };
-  for (uint32_t q = t; q < T; q += blockDim.x) atomicAdd(&hist[bucket(q)], 1u);
+  const auto band = [&](uint32_t q) {
+    uint32_t x = 0;
+    while (x + 1 < kOrderBands && q >= band_lo[x + 1]) x++;
+    return x;
+  };
+  for (uint32_t q = t; q < T; q += blockDim.x) atomicAdd(&hist[band(q)][bucket(q)], 1u);
   __syncthreads();
-  if (t == 0) {
+  if (t < kOrderBands) {
     uint32_t run = 0;
     for (uint32_t b = 0; b < kOrderBuckets; b++) {
-      cursor[b] = run;
-      run += hist[b];
+      cursor[t][b] = run;
+      run += hist[t][b];
     }
   }
   __syncthreads();
-  for (uint32_t q = t; q < T; q += blockDim.x) order[atomicAdd(&cursor[bucket(q)], 1u)] = q;
+  for (uint32_t q = t; q < T; q += blockDim.x) {
+    const uint32_t x = band(q);
+    const uint32_t p = atomicAdd(&cursor[x][bucket(q)], 1u);  // rank within the band
+    order[p * kOrderBands + x] = q;
+  }
 }
 
 void tile_order(splat_ctx* ctx, const BinBuffers& bb, size_t T) {
