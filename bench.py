@@ -137,6 +137,66 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
                                              "gaussians") if k in r} for r in frontier]}
 
 
+def densify_bench(n, rounds, log=print):
+    """Density control at scale: one densify_and_prune (src/densify.rs:125-221) of n
+    Gaussians (the 1M-Gaussian scene) with densification statistics drawn so that about
+    12% of the rows pass the 2e-4 gradient threshold (split or cloned by their size) and
+    about 3% are pruned, restored before each of `rounds` timed rounds.  Reports the mean
+    time of the compaction kernels and their algorithmic bytes: densify_flags reads the
+    rows' log-scales, opacity and statistics (28 B) and writes 5 flags (20 B); the
+    scatter reads parameters and both moments of every row (708 B) and writes them for
+    every output row (708 B x N_out)."""
+    import numpy as np
+    from splatrs_amd import abi, scenes
+    from splatrs_amd import records as rec
+    ctx = abi.Context(0)
+    g = scenes.large_scene(0, n)
+    rng = np.random.default_rng(5)
+    denom = rng.integers(1, 60, n).astype(np.uint32)
+    mean_grad = np.exp(rng.normal(np.log(1.2e-4), 0.55, n)).astype(np.float32)
+    state = {"params": g, "grad_accum": (mean_grad * denom).astype(np.float32), "denom": denom,
+             "max_radii": rng.integers(0, 40, n).astype(np.uint32)}
+    g[rng.random(n) < 0.03, 10] = -7.0  # opacity below the 0.005 prune limit
+    tr = abi.Trainer(ctx, g)
+    extent = 6.0
+    cfg = rec.densify_config(grad_threshold=0.0002)
+    tr.restore(state, 1)
+    tr.densify(cfg, extent, 1)  # warm-up: sizes the output planes
+    ctx.set_timing(True)
+    reports = []
+    walls = []
+    for r in range(rounds):
+        tr.restore(state, 1)
+        ctx.reset_timing()
+        t0 = time.perf_counter()
+        rep = tr.densify(cfg, extent, 2 + r)
+        walls.append(time.perf_counter() - t0)
+        times = ctx.kernel_times()
+        reports.append((rep, times))
+    ctx.set_timing(False)
+    n_out = int(tr.info()[0])
+    rep = reports[-1][0]
+    out = {"workload": f"densify_and_prune of {n} Gaussians (1M-Gaussian scene, drawn "
+                       "statistics), threshold 2e-4, extent 6",
+           "n_in": n, "n_out": n_out, "cloned": int(rep["cloned"]), "split": int(rep["split"]),
+           "pruned": int(rep["pruned"]),
+           "call_ms": round(float(np.mean(walls)) * 1e3, 3), "kernels": {}}
+    alg = {"densify_flags": 48 * n, "densify_scatter": 708 * n + 708 * n_out}
+    for name in sorted(reports[-1][1]):
+        ms = np.mean([t[name][1] / max(t[name][0], 1) for _, t in reports if name in t])
+        k = {"mean_ms": round(float(ms), 4),
+             "launches": int(reports[-1][1][name][0])}
+        if name in alg:
+            k["algorithmic_bytes"] = alg[name]
+            k["gbs"] = round(alg[name] / (ms * 1e-3) / 1e9, 1)
+            k["frac_of_hbm_peak"] = round(alg[name] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)
+        out["kernels"][name] = k
+    tr.close()
+    ctx.close()
+    log(f"densify: {out}")
+    return out
+
+
 def batched_bench(steps, log=print):
     """Batched configuration: 16 scenes of 50k-800k Gaussians (log-uniform; the small
     configuration's generator) and 64 views (4 per scene, 320x240 .. 1600x1200) trained
@@ -216,6 +276,7 @@ def batched_bench(steps, log=print):
                              "bwd_batch": int(tune["bwd_batch"]),
                              "bwd_persistent": bool(tune["bwd_persistent"]),
                              "bwd_per_cu": int(tune["bwd_per_cu"]),
+                             "bwd_row_lists": bool(tune["bwd_row_lists"]),
                              "fwd_ms": round(float(tune["fwd_ms"]), 3),
                              "bwd_ms": round(float(tune["bwd_ms"]), 3),
                              "default_fwd_ms": round(float(tune["default_fwd_ms"]), 3),
@@ -246,10 +307,16 @@ def main():
                     help="extra plain runs at these densify gradient thresholds")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched many-scene configuration")
+    ap.add_argument("--densify-only", action="store_true",
+                    help="only time density control (compaction) at 1M Gaussians")
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the batched and plain-vs-learned sections of the JSON line")
     args = ap.parse_args()
+    if args.densify_only:
+        r = densify_bench(args.gaussians, 5, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(json.dumps(r), flush=True)
+        return
     if args.batched_only:
         r = batched_bench(args.batched_steps, log=lambda m: print(m, file=sys.stderr, flush=True))
         print(json.dumps(r), flush=True)
@@ -310,6 +377,7 @@ def main():
     tuned = {"fwd_batch": int(tune["fwd_batch"]), "fwd_persistent": bool(tune["fwd_persistent"]),
              "fwd_per_cu": int(tune["fwd_per_cu"]), "bwd_batch": int(tune["bwd_batch"]),
              "bwd_persistent": bool(tune["bwd_persistent"]), "bwd_per_cu": int(tune["bwd_per_cu"]),
+             "bwd_row_lists": bool(tune["bwd_row_lists"]),
              "fwd_ms": round(float(tune["fwd_ms"]), 4), "bwd_ms": round(float(tune["bwd_ms"]), 4),
              "default_fwd_ms": round(float(tune["default_fwd_ms"]), 4),
              "default_bwd_ms": round(float(tune["default_bwd_ms"]), 4)}

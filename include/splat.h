@@ -597,15 +597,18 @@ typedef struct {
 } splat_batch_step_params;
 
 /* Launch shapes of the compositing kernels (results do not depend on them): entries per
- * LDS batch (forward 32/64/128/256, tree-reduced backward 16/32) and persistent launches
- * (a fixed grid of per_cu workgroups per compute unit taking tiles from a counter). */
+ * LDS batch (forward 32/64/128/256, tree-reduced backward 16/32), persistent launches
+ * (a fixed grid of per_cu workgroups per compute unit taking tiles from a counter), and
+ * whether each 16-lane row of a tree-reduced backward wave walks its own entry list
+ * (bwd_row_lists = 1) or the wave walks every entry that reaches one of its rows (0). */
 typedef struct {
   int32_t fwd_batch, fwd_persistent, fwd_per_cu;
   int32_t bwd_batch, bwd_persistent, bwd_per_cu;
   float fwd_ms, bwd_ms;                  /* best measured times */
   float default_fwd_ms, default_bwd_ms;  /* times of the default shape (64 / 32, one
-                                            workgroup per tile) */
-  uint32_t candidates, pad_;
+                                            workgroup per tile, row lists) */
+  uint32_t candidates;
+  int32_t bwd_row_lists;
 } splat_tune_result;
 
 /* No counterpart in the reference.  Autotuner of the single-view trainer: times every

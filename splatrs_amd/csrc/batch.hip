@@ -64,6 +64,7 @@ struct BatchKey {
            fwd.fwd_batch == o.fwd.fwd_batch && fwd.persistent == o.fwd.persistent &&
            fwd.per_cu == o.fwd.per_cu && bwd.bwd_batch == o.bwd.bwd_batch &&
            bwd.persistent == o.bwd.persistent && bwd.per_cu == o.bwd.per_cu &&
+           bwd.rows == o.bwd.rows &&
            entry_cap == o.entry_cap && params == o.params;
   }
 };
@@ -527,6 +528,7 @@ int splat_batch_set_shapes(splat_batch* b, const splat_tune_result* shapes) {
     b->bwd.bwd_batch = shapes->bwd_batch;
     b->bwd.persistent = shapes->bwd_persistent != 0;
     b->bwd.per_cu = (unsigned)std::max(1, shapes->bwd_per_cu);
+    b->bwd.rows = shapes->bwd_row_lists != 0;
   });
 }
 

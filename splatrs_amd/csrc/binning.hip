@@ -490,25 +490,20 @@ void tile_starts(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint32_t* total
 // x + 16, ...: band x holds the ceil((T - x) / 8) tiles after the bands before it, and its
 // tiles go out by decreasing entry count (kOrderBuckets buckets relative to the largest
 // tile, order within a bucket unspecified) so the largest start first and the smallest
-// fill the end of the launch.  One workgroup; the result is a permutation of [0, T)
-// whatever the counts.
+// fill the end of the launch.  Bands fix each die's share of the work; below kBandTiles
+// tiles (a few per compute unit) that share is too coarse to balance, and the tiles go
+// out heaviest first over the whole view instead (one band).  One workgroup; the result is
+// a permutation of [0, T) whatever the counts.
 constexpr uint32_t kOrderBuckets = 64;
 constexpr uint32_t kOrderBands = 8;
+constexpr uint32_t kBandTiles = 4096;
 
 __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ tile_start,
                                                           uint32_t T, uint32_t* __restrict__ order) {
   __shared__ uint32_t smax, hist[kOrderBands][kOrderBuckets], cursor[kOrderBands][kOrderBuckets];
-  __shared__ uint32_t band_lo[kOrderBands + 1];
   const uint32_t t = threadIdx.x;
-  if (t == 0) {
-    smax = 0;
-    uint32_t run = 0;
-    for (uint32_t x = 0; x < kOrderBands; x++) {
-      band_lo[x] = run;
-      run += T > x ? (T - x + kOrderBands - 1) / kOrderBands : 0u;
-    }
-    band_lo[kOrderBands] = run;  // == T
-  }
+  const uint32_t nb = T >= kBandTiles ? kOrderBands : 1u;
+  if (t == 0) smax = 0;
   for (uint32_t q = t; q < kOrderBands * kOrderBuckets; q += blockDim.x)
     hist[q / kOrderBuckets][q % kOrderBuckets] = 0;
   __syncthreads();
@@ -517,15 +512,16 @@ __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __rest
   for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
   if ((t & 63u) == 0) atomicMax(&smax, m);
   __syncthreads();
-  const uint64_t mx = smax;
-  const auto bucket = [&](uint32_t q) {
-    const uint64_t c = tile_start[q + 1] - tile_start[q];
-    return (uint32_t)(((mx - c) * kOrderBuckets) / (mx + 1));  // 0: the largest tiles
+  const uint32_t mx = smax;
+  const float scale = (float)kOrderBuckets / ((float)mx + 1.0f);
+  const auto bucket = [&](uint32_t q) {  // 0: the largest tiles (any rounding is harmless)
+    const uint32_t c = tile_start[q + 1] - tile_start[q];
+    return min((uint32_t)((float)(mx - c) * scale), kOrderBuckets - 1);
   };
+  // bands 0 .. r-1 hold s + 1 tiles, the others s (s = T / nb, r = T % nb)
+  const uint32_t bs = T / nb, br = T % nb, big = br * (bs + 1);
   const auto band = [&](uint32_t q) {
-    uint32_t x = 0;
-    while (x + 1 < kOrderBands && q >= band_lo[x + 1]) x++;
-    return x;
+    return q < big ? q / (bs + 1) : br + (bs ? (q - big) / bs : 0u);
   };
   for (uint32_t q = t; q < T; q += blockDim.x) atomicAdd(&hist[band(q)][bucket(q)], 1u);
   __syncthreads();
@@ -540,7 +536,7 @@ __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __rest
   for (uint32_t q = t; q < T; q += blockDim.x) {
     const uint32_t x = band(q);
     const uint32_t p = atomicAdd(&cursor[x][bucket(q)], 1u);  // rank within the band
-    order[p * kOrderBands + x] = q;
+    order[p * nb + x] = q;
   }
 }
 

@@ -471,13 +471,16 @@ int radix_sort_pairs(splat_ctx* ctx, uint32_t* k0, uint32_t* v0, uint32_t* k1, u
 
 // raster.hip
 // Launch shape of the compositing kernels (results do not depend on it): entries per LDS
-// batch of the forward (32, 64, 128) and of the tree-reduced backward (16, 32), and
-// whether a fixed grid of per_cu workgroups per compute unit takes tiles from a counter.
+// batch of the forward (32, 64, 128, 256) and of the tree-reduced backward (16, 32),
+// whether a fixed grid of per_cu workgroups per compute unit takes tiles from a counter,
+// and whether each 16-lane row of a backward wave walks its own list of entries (rows)
+// or the wave walks the entries that reach any of its rows.
 struct RasterVariant {
   int fwd_batch = 64;
   int bwd_batch = 32;
   bool persistent = false;
   unsigned per_cu = 4;
+  bool rows = true;
 };
 // With `bd` the tiles are the global tile space of a batched step (bb.tiles_x * bb.tiles_y
 // tiles; width/height unused).  `counter` (one word) is needed by persistent launches.

@@ -523,8 +523,11 @@ __device__ __forceinline__ float row_allsum16(float v) {
 // Otherwise: 16-lane row sums by DPP, 16 row partials per entry summed in a fixed order.
 // ABS: also accumulate the AbsGS slots (9, 10); otherwise they are written as zero.
 // BF: entries per LDS batch of the tree-reduced form (the ordered form uses kOrdBatch).
-// COUNT (tree-reduced form only): also count pairs into `pairs` (PairCounter).
-template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST, bool COUNT = false>
+// ROWS (tree-reduced form): every 16-lane row walks its own entry list; otherwise the wave
+// walks the entries that reach any of its rows (the same sums either way).  COUNT
+// (tree-reduced form only): also count pairs into `pairs` (PairCounter).
+template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST, bool ROWS,
+          bool COUNT = false>
 __global__ void __launch_bounds__(256) raster_bwd_kernel(
     const float4* __restrict__ rec, const uint32_t* __restrict__ tile_start,
     const uint32_t* __restrict__ order,
@@ -641,23 +644,40 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
       const uint32_t woff = wslot * 16 + wave * 4 + row;
       // the rows' lists: entry kk sits at position popcount(mask >> (kk + 1))
       uint8_t* my_list = slist[wave * 4 + row];
-      uint32_t len[4];
       const uint32_t kl = min(lane, cnt - 1);
+      uint32_t steps = 0, my_len = 0;
+      uint64_t wave_todo = 0;  // !ROWS: the wave's entries, walked from the top bit down
+      if (ROWS) {
+        uint32_t len[4];
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const uint64_t m = __ballot(lane < cnt && lo + lane < row_last[r] &&
-                                    row_touches(srect[kl], wave, (uint32_t)r));
-        len[r] = (uint32_t)__popcll(m);
-        if ((m >> lane) & 1ull)
-          slist[wave * 4 + r][__popcll(m >> lane) - 1] = (uint8_t)lane;
+        for (int r = 0; r < 4; r++) {
+          const uint64_t m = __ballot(lane < cnt && lo + lane < row_last[r] &&
+                                      row_touches(srect[kl], wave, (uint32_t)r));
+          len[r] = (uint32_t)__popcll(m);
+          if ((m >> lane) & 1ull)
+            slist[wave * 4 + r][__popcll(m >> lane) - 1] = (uint8_t)lane;
+        }
+        steps = max(max(len[0], len[1]), max(len[2], len[3]));
+        my_len = row == 0 ? len[0] : (row == 1 ? len[1] : (row == 2 ? len[2] : len[3]));
+        // the lists are written and read by this wave only
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      } else {
+        const uint32_t wave_last =
+            max(max(row_last[0], row_last[1]), max(row_last[2], row_last[3]));
+        wave_todo = __ballot(lane < cnt && lo + lane < wave_last &&
+                             wave_touches(srect[kl], wave));
+        steps = (uint32_t)__popcll(wave_todo);
+        my_len = steps;
       }
-      const uint32_t steps = max(max(len[0], len[1]), max(len[2], len[3]));
-      const uint32_t my_len = row == 0 ? len[0] : (row == 1 ? len[1] : (row == 2 ? len[2] : len[3]));
-      // the lists are written and read by this wave only
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       for (uint32_t t = 0; t < steps; t++) {
         const bool has = t < my_len;
-        const uint32_t kk = has ? (uint32_t)my_list[t] : 0u;
+        uint32_t kk;
+        if (ROWS) {
+          kk = has ? (uint32_t)my_list[t] : 0u;
+        } else {
+          kk = 63u - (uint32_t)__builtin_clzll(wave_todo);
+          wave_todo &= ~(1ull << kk);
+        }
         const uint32_t k = lo + kk;
         float v[kSlots];
         const bool act = has & valid & (k < s.last) & in_rect(srect[kk], i, j);
@@ -789,14 +809,14 @@ void fwd_go(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& bb, unsigne
          image, final_t, n_contrib, bd, counter, (unsigned long long*)nullptr);
 }
 
-template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST>
+template <bool ORDERED, bool ABS, int BF, bool BATCH, bool PERSIST, bool ROWS>
 void bwd_go(splat_ctx* ctx, const char* name, const ProjBuffers& pb, const BinBuffers& bb,
             unsigned T, uint32_t width, uint32_t height, const float bg[3], const float* final_t,
             const uint32_t* n_contrib, const float* dimage, float* slots, const BatchDev& bd,
             uint32_t* counter, unsigned per_cu) {
   if (PERSIST) fill_u32(ctx, counter, kXcds, 0u);
   const unsigned grid = PERSIST ? persistent_grid(ctx, per_cu, T) : T;
-  launch(ctx, name, raster_bwd_kernel<ORDERED, ABS, BF, BATCH, PERSIST>, dim3(grid), dim3(256),
+  launch(ctx, name, raster_bwd_kernel<ORDERED, ABS, BF, BATCH, PERSIST, ROWS>, dim3(grid), dim3(256),
          0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start,
          (const uint32_t*)bb.tile_order, (const uint32_t*)bb.sorted_gauss,
          (const uint32_t*)bb.sorted_entry, T, bb.tiles_x, width,
@@ -825,6 +845,31 @@ void fwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
   }
 }
 
+template <bool BATCH, bool PERSIST, bool ROWS>
+void bwd_rows_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
+                   const BinBuffers& bb, unsigned T, uint32_t width, uint32_t height,
+                   const float bg[3], const float* final_t, const uint32_t* n_contrib,
+                   const float* dimage, float* slots, bool need_abs, const BatchDev& bd,
+                   uint32_t* counter) {
+  const unsigned pc = rv.per_cu;
+  if (rv.bwd_batch == 16) {
+    if (need_abs)
+      return bwd_go<false, true, 16, BATCH, PERSIST, ROWS>(ctx, "raster_bwd", pb, bb, T, width,
+                                                           height, bg, final_t, n_contrib,
+                                                           dimage, slots, bd, counter, pc);
+    return bwd_go<false, false, 16, BATCH, PERSIST, ROWS>(ctx, "raster_bwd", pb, bb, T, width,
+                                                          height, bg, final_t, n_contrib, dimage,
+                                                          slots, bd, counter, pc);
+  }
+  if (need_abs)
+    return bwd_go<false, true, 32, BATCH, PERSIST, ROWS>(ctx, "raster_bwd", pb, bb, T, width,
+                                                         height, bg, final_t, n_contrib, dimage,
+                                                         slots, bd, counter, pc);
+  return bwd_go<false, false, 32, BATCH, PERSIST, ROWS>(ctx, "raster_bwd", pb, bb, T, width,
+                                                        height, bg, final_t, n_contrib, dimage,
+                                                        slots, bd, counter, pc);
+}
+
 template <bool BATCH, bool PERSIST>
 void bwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
               const BinBuffers& bb, unsigned T, uint32_t width, uint32_t height,
@@ -834,29 +879,18 @@ void bwd_pick(splat_ctx* ctx, const RasterVariant& rv, const ProjBuffers& pb,
   const unsigned pc = rv.per_cu;
   // the record width (9 or 11 words per entry) follows need_abs in both modes
   if (ordered && need_abs)
-    return bwd_go<true, true, 8, BATCH, PERSIST>(ctx, "raster_bwd_ordered", pb, bb, T, width,
-                                                 height, bg, final_t, n_contrib, dimage, slots,
-                                                 bd, counter, pc);
+    return bwd_go<true, true, 8, BATCH, PERSIST, false>(ctx, "raster_bwd_ordered", pb, bb, T,
+                                                        width, height, bg, final_t, n_contrib,
+                                                        dimage, slots, bd, counter, pc);
   if (ordered)
-    return bwd_go<true, false, 8, BATCH, PERSIST>(ctx, "raster_bwd_ordered", pb, bb, T, width,
-                                                  height, bg, final_t, n_contrib, dimage, slots,
-                                                  bd, counter, pc);
-  if (rv.bwd_batch == 16) {
-    if (need_abs)
-      return bwd_go<false, true, 16, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
-                                                     bg, final_t, n_contrib, dimage, slots, bd,
-                                                     counter, pc);
-    return bwd_go<false, false, 16, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
-                                                    bg, final_t, n_contrib, dimage, slots, bd,
-                                                    counter, pc);
-  }
-  if (need_abs)
-    return bwd_go<false, true, 32, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
-                                                   bg, final_t, n_contrib, dimage, slots, bd,
-                                                   counter, pc);
-  return bwd_go<false, false, 32, BATCH, PERSIST>(ctx, "raster_bwd", pb, bb, T, width, height,
-                                                  bg, final_t, n_contrib, dimage, slots, bd,
-                                                  counter, pc);
+    return bwd_go<true, false, 8, BATCH, PERSIST, false>(ctx, "raster_bwd_ordered", pb, bb, T,
+                                                         width, height, bg, final_t, n_contrib,
+                                                         dimage, slots, bd, counter, pc);
+  if (rv.rows)
+    return bwd_rows_pick<BATCH, PERSIST, true>(ctx, rv, pb, bb, T, width, height, bg, final_t,
+                                               n_contrib, dimage, slots, need_abs, bd, counter);
+  return bwd_rows_pick<BATCH, PERSIST, false>(ctx, rv, pb, bb, T, width, height, bg, final_t,
+                                              n_contrib, dimage, slots, need_abs, bd, counter);
 }
 
 }  // namespace
@@ -877,7 +911,7 @@ void raster_backward_counted(splat_ctx* ctx, const ProjBuffers& pb, const BinBuf
                              const float* final_t, const uint32_t* n_contrib,
                              const float* dimage, float* slots, unsigned long long* pairs) {
   const unsigned T = bb.tiles_x * bb.tiles_y;
-  launch(ctx, "raster_bwd_count", raster_bwd_kernel<false, false, 32, false, false, true>,
+  launch(ctx, "raster_bwd_count", raster_bwd_kernel<false, false, 32, false, false, true, true>,
          dim3(T), dim3(256), 0, (const float4*)pb.rec, (const uint32_t*)bb.tile_start,
          (const uint32_t*)nullptr, (const uint32_t*)bb.sorted_gauss,
          (const uint32_t*)bb.sorted_entry, T, bb.tiles_x, width, height, bg[0], bg[1], bg[2],
@@ -976,17 +1010,19 @@ RasterTune tune_raster(splat_ctx* ctx, const ProjBuffers& pb, const BinBuffers& 
         }
       }
     for (int bw : {16, 32})
-      for (int mode = 0; mode < 4; mode++) {
+      for (int mode = 0; mode < 8; mode++) {
         RasterVariant rv;
         rv.bwd_batch = bw;
-        rv.persistent = mode > 0;
-        rv.per_cu = mode == 1 ? 2 : (mode == 2 ? 4 : 8);
+        rv.rows = mode < 4 || ordered;  // the ordered form walks per entry
+        if (ordered && mode >= 4) continue;
+        rv.persistent = mode % 4 > 0;
+        rv.per_cu = mode % 4 == 1 ? 2 : (mode % 4 == 2 ? 4 : 8);
         const float ms = time_it([&] {
           raster_backward(ctx, pb, bb, width, height, bg, final_t, n_contrib, dimage, ordered,
                           slots, need_abs, rv, bd, counter);
         });
         out.candidates++;
-        if (rv.bwd_batch == def.bwd_batch && !rv.persistent) out.default_bwd_ms = ms;
+        if (rv.bwd_batch == def.bwd_batch && !rv.persistent && rv.rows) out.default_bwd_ms = ms;
         if (ms < out.bwd_ms) {
           out.bwd_ms = ms;
           out.bwd = rv;
@@ -1016,7 +1052,7 @@ void fill_tune_result(const RasterTune& t, splat_tune_result* out) {
   out->default_fwd_ms = t.default_fwd_ms;
   out->default_bwd_ms = t.default_bwd_ms;
   out->candidates = t.candidates;
-  out->pad_ = 0;
+  out->bwd_row_lists = t.bwd.rows ? 1 : 0;
 }
 
 void reduce_slots(splat_ctx* ctx, const uint32_t* entry_offset, const float* slots, size_t n,

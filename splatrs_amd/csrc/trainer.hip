@@ -222,6 +222,7 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   P.bwd_batch = t->bwd_shape.bwd_batch;
   P.bwd_persistent = t->bwd_shape.persistent;
   P.bwd_per_cu = (int32_t)t->bwd_shape.per_cu;
+  P.bwd_rows = t->bwd_shape.rows ? 1 : 0;
   P.counter = (uint32_t*)st["raster_counter"].reserve(64);
   P.pb = proj_buffers(t);
   BinBuffers& bb = P.bb;
@@ -279,6 +280,7 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   bv.bwd_batch = P.bwd_batch;
   bv.persistent = P.bwd_persistent != 0;
   bv.per_cu = (unsigned)P.bwd_per_cu;
+  bv.rows = P.bwd_rows != 0;
   raster_forward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.image, P.final_t, P.n_contrib, fv,
                  nullptr, P.counter);
   loss_forward_backward(ctx, P.image, nullptr, any_target, P.width, P.height, P.lambda, true,
@@ -913,6 +915,7 @@ int splat_trainer_set_shapes(splat_trainer* t, const splat_tune_result* shapes) 
     t->bwd_shape.bwd_batch = shapes->bwd_batch;
     t->bwd_shape.persistent = shapes->bwd_persistent != 0;
     t->bwd_shape.per_cu = (unsigned)std::max(1, shapes->bwd_per_cu);
+    t->bwd_shape.rows = shapes->bwd_row_lists != 0;
   });
 }
 

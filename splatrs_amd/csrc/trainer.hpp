@@ -30,6 +30,7 @@ struct StepPlan {
   int32_t sh_degree, antialias, ordered, accumulate, absgrad, pad0;
   float near_plane, lambda, bg[3], strong_threshold;
   int32_t fwd_batch, fwd_persistent, fwd_per_cu, bwd_batch, bwd_persistent, bwd_per_cu;
+  int32_t bwd_rows, pad1;
   float *params, *m, *v, *grad_accum;
   uint32_t *denom, *max_radii;
   float* max_grad;   // extra statistics for the learned scorer (null when not tracked)

@@ -52,7 +52,8 @@ BATCH_STEP_PARAMS = np.dtype([("sh_degree", "<i4"), ("ordered_backward", "<i4"),
 TUNE_RESULT = np.dtype([("fwd_batch", "<i4"), ("fwd_persistent", "<i4"), ("fwd_per_cu", "<i4"),
                         ("bwd_batch", "<i4"), ("bwd_persistent", "<i4"), ("bwd_per_cu", "<i4"),
                         ("fwd_ms", "<f4"), ("bwd_ms", "<f4"), ("default_fwd_ms", "<f4"),
-                        ("default_bwd_ms", "<f4"), ("candidates", "<u4"), ("pad_", "<u4")])
+                        ("default_bwd_ms", "<f4"), ("candidates", "<u4"),
+                        ("bwd_row_lists", "<i4")])
 SCORER_PARAMS = 9347
 SCORER_FEATURES = 12
 
