@@ -49,8 +49,8 @@ def algorithmic_bytes(name, N, V, E, P):
         "loss_ssim_stats": 51 * P,           # pred 12 + target 3 read; 3 partial maps written
         "loss_grad": 63 * P,                 # 3 maps + pred + target read; dL/dimage written
         "bin_keygen": 12 * E + 16 * N,
-        "bin_scatter": 28 * E,
-        "bin_sort_small": 24 * E,
+        "bin_scatter": 32 * E,               # gaussian, tile, slot, depth read; key + gaussian written
+        "bin_sort_tiles": 20 * E,            # key + gaussian read; entry + gaussian written
         "stats": 24 * N,
         # slots + params + both moments read; params and moments written (no gradient planes)
         "grad_adam": 36 * E + 1416 * N,

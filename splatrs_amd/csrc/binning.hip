@@ -9,16 +9,19 @@
 //      in turn, so a counter set stays in one XCD's cache lines), laid out [kSub][T];
 //   3. per tile, the counters' offsets within the tile and its total; tile_start =
 //      exclusive scan of the totals;
-//   4. scatter (depth bits << 32 | entry) into its tile's bucket;
-//   5. sort each bucket by that 64-bit value: bitonic in LDS (tiles up to 1024 entries in
-//      256-lane workgroups; up to 16384 in 1024-lane workgroups with 128 KB of LDS) and
-//      in global memory beyond.  This equals the reference's (key, entry) order.
+//   4. scatter one 16-byte record per entry into its tile's bucket: the key (depth bits
+//      << 32 | entry) and the entry's Gaussian;
+//   5. sort each bucket by that 64-bit value, the Gaussian carried along: tiles of up to
+//      1024 entries by one wave each (a bitonic network in registers, lanes exchanging
+//      by DPP and the gfx950 row swaps), up to 16384 by a bitonic sort in LDS (1024-lane workgroups,
+//      128 KB), in global memory beyond.  This equals the reference's (key, entry) order.
 #include "internal.hpp"
 
 namespace splat {
 namespace {
 
-constexpr int kSmallCap = 1024;   // entries per tile sorted by the 256-lane kernel
+constexpr int kWaveCap = 512;     // entries per tile sorted by one wave (8 per lane)
+constexpr int kMidCap = 1024;     // ... by one wave of the second kernel (16 per lane)
 constexpr uint32_t kKeygenThreads = 256;
 constexpr uint32_t kKeygenPerLane = 1;
 constexpr uint32_t kKeygenEntries = kKeygenThreads * kKeygenPerLane;  // per keygen workgroup
@@ -215,7 +218,7 @@ __global__ void __launch_bounds__(256) scatter_kernel(uint32_t e_total,
                                                       const uint32_t* __restrict__ tile_sub,
                                                       uint32_t num_tiles,
                                                       const float* __restrict__ depth,
-                                                      uint64_t* __restrict__ bucket,
+                                                      uint4* __restrict__ bucket,
                                                       const StepStatus* __restrict__ status) {
   if (status) {
     if (status->abort) return;
@@ -224,8 +227,9 @@ __global__ void __launch_bounds__(256) scatter_kernel(uint32_t e_total,
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= e_total) return;
   const uint32_t tile = entry_tile[e];
+  const uint32_t g = entry_gaussian[e];
   const uint32_t pos = tile_start[tile] + tile_sub[entry_sub(e) * num_tiles + tile] + entry_slot[e];
-  bucket[pos] = ((uint64_t)f2bits(depth[entry_gaussian[e]]) << 32) | e;
+  bucket[pos] = make_uint4(e, f2bits(depth[g]), g, 0u);  // key low word, high word, Gaussian
 }
 
 // Entry count of the step and the capacity check of a graph-captured step.
@@ -296,35 +300,202 @@ __device__ __forceinline__ void write_sorted(uint32_t pos, uint64_t v, uint32_t 
   if (keys_out) keys_out[pos] = ((uint64_t)tile << 32) | (v >> 32);
 }
 
-// Tiles with at most kSmallCap entries; larger tiles are queued for the big kernel.
-__global__ void __launch_bounds__(256) sort_small_kernel(uint32_t num_tiles,
+// ---- per-tile sorts in one wave ----------------------------------------------------
+// A tile of at most 64 K entries is sorted by one wave holding K (key, Gaussian) pairs per
+// lane in registers: a bitonic network over N = 64 K positions (padding keys all ones),
+// position i = lane K + k.  Compare-exchanges of positions less than K apart stay inside
+// the lane; the others pair lane l with lane l ^ (stride / K), exchanged by ds_bpermute,
+// each lane keeping the smaller or the larger of its pair.  No LDS array, no barrier.
+
+// The value of lane ^ LX for LX = 1, 2, 4, 8: DPP within 16-lane rows (quad permutes, the
+// half-row mirror, the rotation by 8).
+template <uint32_t LX>
+__device__ __forceinline__ uint32_t from_lane_xor(uint32_t v) {
+  if constexpr (LX == 1) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);  // [1,0,3,2]
+  } else if constexpr (LX == 2) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);  // [2,3,0,1]
+  } else if constexpr (LX == 4) {  // quad reversal, then the half-row mirror
+    const int r = __builtin_amdgcn_update_dpp(0, (int)v, 0x1B, 0xF, 0xF, true);
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, r, 0x141, 0xF, 0xF, true);
+  } else {
+    static_assert(LX == 8, "lane distance");
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);  // row_ror 8
+  }
+}
+
+// For LX = 16, 32, the gfx950 swaps: permlane16_swap exchanges the odd rows of its first
+// operand with the even rows of its second, permlane32_swap the upper half of the first
+// with the lower half of the second.  Given the same value twice, the first result holds
+// in every lane the value of the lower lane of its pair (lane & ~LX), the second the
+// value of the upper one.
+template <uint32_t LX>
+__device__ __forceinline__ uint2 pair_values(uint32_t v) {
+  if constexpr (LX == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return make_uint2(r[0], r[1]);
+  } else {
+    static_assert(LX == 32, "lane distance");
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return make_uint2(r[0], r[1]);
+  }
+}
+
+// Compare-exchange of the in-lane pairs (k, k | S) at network size `size`.
+template <int K, int S>
+__device__ __forceinline__ void wave_sort_in_lane(uint64_t (&key)[K], uint32_t (&gid)[K],
+                                                  uint32_t lane, uint32_t size) {
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    if (k & S) continue;
+    const int k2 = k | S;
+    // keys are distinct except the all-ones padding, whose order does not matter, so
+    // "a > b when ascending, a < b when descending" is (a > b) == up
+    const bool up = ((lane * K + k) & size) == 0;
+    const uint64_t a = key[k], b = key[k2];
+    const bool sw = (a > b) == up;
+    key[k] = sw ? b : a;
+    key[k2] = sw ? a : b;
+    const uint32_t ga = gid[k], gb = gid[k2];
+    gid[k] = sw ? gb : ga;
+    gid[k2] = sw ? ga : gb;
+  }
+}
+
+// Compare-exchange with lane ^ LX (stride = LX K >= K): the pair's direction and this
+// lane's side depend on the lane only (size > stride >= K).  Keys are distinct except the
+// all-ones padding, whose order does not matter, so "take the smaller when keeping the
+// minimum, else the larger" is one comparison equal to keep_min.
+template <int K, uint32_t LX>
+__device__ __forceinline__ void wave_sort_across(uint64_t (&key)[K], uint32_t (&gid)[K],
+                                                 uint32_t lane, uint32_t size) {
+  const bool up = ((lane * K) & size) == 0;
+  const bool lower = (lane & LX) == 0;
+  const bool keep_min = lower == up;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    if constexpr (LX <= 8) {
+      const uint32_t lo = from_lane_xor<LX>((uint32_t)key[k]);
+      const uint32_t hi = from_lane_xor<LX>((uint32_t)(key[k] >> 32));
+      const uint32_t pg = from_lane_xor<LX>(gid[k]);
+      const uint64_t pk = ((uint64_t)hi << 32) | lo;
+      const bool take = (pk < key[k]) == keep_min;
+      key[k] = take ? pk : key[k];
+      gid[k] = take ? pg : gid[k];
+    } else {
+      const uint2 lo = pair_values<LX>((uint32_t)key[k]);
+      const uint2 hi = pair_values<LX>((uint32_t)(key[k] >> 32));
+      const uint2 g = pair_values<LX>(gid[k]);
+      const uint64_t a = ((uint64_t)hi.x << 32) | lo.x, b = ((uint64_t)hi.y << 32) | lo.y;
+      const bool take_a = (a < b) == keep_min;  // a: the lower lane's key, b: the upper's
+      key[k] = take_a ? a : b;
+      gid[k] = take_a ? g.x : g.y;
+    }
+  }
+}
+
+// The network's stages from (SIZE, STRIDE) on, unrolled so that every register index is
+// a constant.
+template <int K, uint32_t SIZE, uint32_t STRIDE>
+__device__ __forceinline__ void wave_sort_stages(uint64_t (&key)[K], uint32_t (&gid)[K],
+                                                 uint32_t lane) {
+  if constexpr (STRIDE >= (uint32_t)K)
+    wave_sort_across<K, STRIDE / K>(key, gid, lane, SIZE);
+  else
+    wave_sort_in_lane<K, (int)STRIDE>(key, gid, lane, SIZE);
+  if constexpr (STRIDE > 1)
+    wave_sort_stages<K, SIZE, STRIDE / 2>(key, gid, lane);
+  else if constexpr (SIZE < 64u * K)
+    wave_sort_stages<K, SIZE * 2, SIZE>(key, gid, lane);
+}
+
+template <int K>
+__device__ __forceinline__ void wave_sort_tile(uint32_t lane, uint32_t tile, uint32_t s,
+                                               uint32_t n, const uint4* __restrict__ bucket,
+                                               uint32_t* __restrict__ sorted_entry,
+                                               uint32_t* __restrict__ sorted_gauss,
+                                               uint64_t* __restrict__ keys_out) {
+  uint64_t key[K];
+  uint32_t gid[K];
+  // any assignment of the inputs to positions sorts; this one reads coalesced
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const uint32_t p = (uint32_t)k * 64u + lane;
+    const uint4 r = p < n ? bucket[s + p] : make_uint4(~0u, ~0u, 0u, 0u);
+    key[k] = ((uint64_t)r.y << 32) | r.x;
+    gid[k] = r.z;
+  }
+  wave_sort_stages<K, 2, 1>(key, gid, lane);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const uint32_t r = lane * K + k;
+    if (r < n) {
+      sorted_entry[s + r] = (uint32_t)key[k];
+      sorted_gauss[s + r] = gid[k];
+      if (keys_out) keys_out[s + r] = ((uint64_t)tile << 32) | (key[k] >> 32);
+    }
+  }
+}
+
+// One wave per tile (four per workgroup).  Larger tiles are queued in `queue` (T words):
+// up to kMidCap entries from the end (count in *mid_count), beyond from the start (count
+// in *big_count); a tile is queued at most once, so the two never meet.
+__global__ void __launch_bounds__(256) sort_tiles_kernel(uint32_t num_tiles,
                                                          const uint32_t* __restrict__ tile_start,
-                                                         uint64_t* __restrict__ bucket,
-                                                         const uint32_t* __restrict__ entry_gaussian,
+                                                         const uint4* __restrict__ bucket,
                                                          uint32_t* __restrict__ sorted_entry,
                                                          uint32_t* __restrict__ sorted_gauss,
                                                          uint64_t* __restrict__ keys_out,
-                                                         uint32_t* __restrict__ big_list,
-                                                         uint32_t* __restrict__ big_count) {
-  __shared__ uint64_t sh[kSmallCap];
-  const uint32_t tile = blockIdx.x;
+                                                         uint32_t* __restrict__ queue,
+                                                         uint32_t* __restrict__ big_count,
+                                                         uint32_t* __restrict__ mid_count) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t tile = blockIdx.x * 4u + (threadIdx.x >> 6);
+  if (tile >= num_tiles) return;
   const uint32_t s = tile_start[tile], n = tile_start[tile + 1] - s;
   if (n == 0) return;
-  if (n > kSmallCap) {
-    if (threadIdx.x == 0) big_list[atomicAdd(big_count, 1u)] = tile;
+  if (n > (uint32_t)kWaveCap) {
+    if (lane == 0) {
+      if (n > (uint32_t)kMidCap)
+        queue[atomicAdd(big_count, 1u)] = tile;
+      else
+        queue[num_tiles - 1 - atomicAdd(mid_count, 1u)] = tile;
+    }
     return;
   }
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) sh[i] = bucket[s + i];
-  __syncthreads();
-  bitonic_sort(n, [&](uint32_t i) { return sh[i]; }, [&](uint32_t i, uint64_t v) { sh[i] = v; });
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-    write_sorted(s + i, sh[i], tile, entry_gaussian, sorted_entry, sorted_gauss, keys_out);
+  if (n <= 64)
+    wave_sort_tile<1>(lane, tile, s, n, bucket, sorted_entry, sorted_gauss, keys_out);
+  else if (n <= 128)
+    wave_sort_tile<2>(lane, tile, s, n, bucket, sorted_entry, sorted_gauss, keys_out);
+  else if (n <= 256)
+    wave_sort_tile<4>(lane, tile, s, n, bucket, sorted_entry, sorted_gauss, keys_out);
+  else
+    wave_sort_tile<8>(lane, tile, s, n, bucket, sorted_entry, sorted_gauss, keys_out);
+}
+
+// The queued tiles of kWaveCap < n <= kMidCap entries, one wave each (the grid's waves
+// stride over the queue; a wave leaves when it is exhausted).
+__global__ void __launch_bounds__(256) sort_mid_kernel(uint32_t num_tiles,
+                                                       const uint32_t* __restrict__ tile_start,
+                                                       const uint4* __restrict__ bucket,
+                                                       uint32_t* __restrict__ sorted_entry,
+                                                       uint32_t* __restrict__ sorted_gauss,
+                                                       uint64_t* __restrict__ keys_out,
+                                                       const uint32_t* __restrict__ queue,
+                                                       const uint32_t* __restrict__ mid_count) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t count = *mid_count;
+  for (uint32_t q = blockIdx.x * 4u + (threadIdx.x >> 6); q < count; q += gridDim.x * 4u) {
+    const uint32_t tile = queue[num_tiles - 1 - q];
+    const uint32_t s = tile_start[tile], n = tile_start[tile + 1] - s;
+    wave_sort_tile<16>(lane, tile, s, n, bucket, sorted_entry, sorted_gauss, keys_out);
+  }
 }
 
 // Queued tiles: LDS bitonic up to kBigCap entries, global-memory bitonic beyond.  The
 // grid strides over the queue; every workgroup exits once the queue is exhausted.
 __global__ void __launch_bounds__(1024) sort_big_kernel(const uint32_t* __restrict__ tile_start,
-                                                        uint64_t* __restrict__ bucket,
+                                                        uint4* __restrict__ bucket,
                                                         const uint32_t* __restrict__ entry_gaussian,
                                                         uint32_t* __restrict__ sorted_entry,
                                                         uint32_t* __restrict__ sorted_gauss,
@@ -333,22 +504,30 @@ __global__ void __launch_bounds__(1024) sort_big_kernel(const uint32_t* __restri
                                                         const uint32_t* __restrict__ big_count) {
   __shared__ uint64_t sh[kBigCap];
   const uint32_t count = *big_count;
+  // the records' keys (their Gaussians come from entry_gaussian)
+  const auto rec_key = [&](uint32_t i) {
+    const uint2 k = *reinterpret_cast<const uint2*>(&bucket[i]);
+    return ((uint64_t)k.y << 32) | k.x;
+  };
   for (uint32_t q = blockIdx.x; q < count; q += gridDim.x) {
     const uint32_t tile = big_list[q];
     const uint32_t s = tile_start[tile], n = tile_start[tile + 1] - s;
     if (n <= kBigCap) {
-      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) sh[i] = bucket[s + i];
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) sh[i] = rec_key(s + i);
       __syncthreads();
       bitonic_sort(n, [&](uint32_t i) { return sh[i]; },
                    [&](uint32_t i, uint64_t v) { sh[i] = v; });
       for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
         write_sorted(s + i, sh[i], tile, entry_gaussian, sorted_entry, sorted_gauss, keys_out);
     } else {
-      uint64_t* seg = bucket + s;
-      bitonic_sort(n, [&](uint32_t i) { return seg[i]; },
-                   [&](uint32_t i, uint64_t v) { seg[i] = v; });
+      bitonic_sort(n, [&](uint32_t i) { return rec_key(s + i); },
+                   [&](uint32_t i, uint64_t v) {
+                     *reinterpret_cast<uint2*>(&bucket[s + i]) =
+                         make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+                   });
       for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-        write_sorted(s + i, seg[i], tile, entry_gaussian, sorted_entry, sorted_gauss, keys_out);
+        write_sorted(s + i, rec_key(s + i), tile, entry_gaussian, sorted_entry, sorted_gauss,
+                     keys_out);
     }
     __syncthreads();
   }
@@ -497,53 +676,171 @@ void tile_starts(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint32_t* total
 constexpr uint32_t kOrderBuckets = 64;
 constexpr uint32_t kOrderBands = 8;
 constexpr uint32_t kBandTiles = 4096;
+constexpr uint32_t kPlanThreads = 1024;
+constexpr uint32_t kPlanItems = 16;
+constexpr uint32_t kPlanTiles = kPlanThreads * kPlanItems;  // tiles whose counts fit in LDS
 
-__global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ tile_start,
-                                                          uint32_t T, uint32_t* __restrict__ order) {
-  __shared__ uint32_t smax, hist[kOrderBands][kOrderBuckets], cursor[kOrderBands][kOrderBuckets];
+// LDS index of tile q in a count array padded by one word per 32 (a thread reading its 16
+// consecutive tiles then spreads over the banks).
+__device__ __forceinline__ uint32_t plan_slot(uint32_t q) { return q + (q >> 5); }
+
+struct OrderShared {
+  uint32_t smax;
+  uint32_t hist[kOrderBands][kOrderBuckets];
+  uint32_t cursor[kOrderBands][kOrderBuckets];
+};
+
+// The dispatch order from the tiles' entry counts (count(q), any source) and their
+// maximum m_lane reduced here; every thread of the 1024-lane workgroup calls it.
+template <class Count>
+__device__ __forceinline__ void order_from_counts(uint32_t T, uint32_t m_lane, Count count,
+                                                  OrderShared& o, uint32_t* __restrict__ order) {
   const uint32_t t = threadIdx.x;
   const uint32_t nb = T >= kBandTiles ? kOrderBands : 1u;
-  if (t == 0) smax = 0;
+  if (t == 0) o.smax = 0;
   for (uint32_t q = t; q < kOrderBands * kOrderBuckets; q += blockDim.x)
-    hist[q / kOrderBuckets][q % kOrderBuckets] = 0;
+    o.hist[q / kOrderBuckets][q % kOrderBuckets] = 0;
   __syncthreads();
-  uint32_t m = 0;
-  for (uint32_t q = t; q < T; q += blockDim.x) m = max(m, tile_start[q + 1] - tile_start[q]);
+  uint32_t m = m_lane;
   for (int d = 32; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, 64));
-  if ((t & 63u) == 0) atomicMax(&smax, m);
+  if ((t & 63u) == 0) atomicMax(&o.smax, m);
   __syncthreads();
-  const uint32_t mx = smax;
+  const uint32_t mx = o.smax;
   const float scale = (float)kOrderBuckets / ((float)mx + 1.0f);
   const auto bucket = [&](uint32_t q) {  // 0: the largest tiles (any rounding is harmless)
-    const uint32_t c = tile_start[q + 1] - tile_start[q];
-    return min((uint32_t)((float)(mx - c) * scale), kOrderBuckets - 1);
+    return min((uint32_t)((float)(mx - count(q)) * scale), kOrderBuckets - 1);
   };
   // bands 0 .. r-1 hold s + 1 tiles, the others s (s = T / nb, r = T % nb)
   const uint32_t bs = T / nb, br = T % nb, big = br * (bs + 1);
   const auto band = [&](uint32_t q) {
     return q < big ? q / (bs + 1) : br + (bs ? (q - big) / bs : 0u);
   };
-  for (uint32_t q = t; q < T; q += blockDim.x) atomicAdd(&hist[band(q)][bucket(q)], 1u);
+  for (uint32_t q = t; q < T; q += blockDim.x) atomicAdd(&o.hist[band(q)][bucket(q)], 1u);
   __syncthreads();
   if (t < kOrderBands) {
     uint32_t run = 0;
     for (uint32_t b = 0; b < kOrderBuckets; b++) {
-      cursor[t][b] = run;
-      run += hist[t][b];
+      o.cursor[t][b] = run;
+      run += o.hist[t][b];
     }
   }
   __syncthreads();
   for (uint32_t q = t; q < T; q += blockDim.x) {
     const uint32_t x = band(q);
-    const uint32_t p = atomicAdd(&cursor[x][bucket(q)], 1u);  // rank within the band
+    const uint32_t p = atomicAdd(&o.cursor[x][bucket(q)], 1u);  // rank within the band
     order[p * nb + x] = q;
   }
+}
+
+// The order from tile_start (views binned by radix sorts).  Counts of up to kPlanTiles
+// tiles are staged in LDS first, so the three passes over them read no global memory.
+__global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t* __restrict__ tile_start,
+                                                          uint32_t T, uint32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[kPlanTiles + kPlanTiles / 32];
+  __shared__ OrderShared o;
+  const bool staged = T <= kPlanTiles;
+  uint32_t m = 0;
+  if (staged) {
+    for (uint32_t q = threadIdx.x; q < T; q += blockDim.x) {
+      const uint32_t c = tile_start[q + 1] - tile_start[q];
+      cnt[plan_slot(q)] = c;
+      m = max(m, c);
+    }
+    __syncthreads();
+    order_from_counts(T, m, [&](uint32_t q) { return cnt[plan_slot(q)]; }, o, order);
+  } else {
+    for (uint32_t q = threadIdx.x; q < T; q += blockDim.x)
+      m = max(m, tile_start[q + 1] - tile_start[q]);
+    order_from_counts(T, m, [&](uint32_t q) { return tile_start[q + 1] - tile_start[q]; }, o,
+                      order);
+  }
+}
+
+// Views of at most kPlanTiles tiles, after the counting pass: one workgroup writes each
+// counter set's offset within its tile (tile_sub), tile_start (the exclusive scan of the
+// tile totals, tile_start[T] and *total_out the entry count) and the dispatch order.
+__global__ void __launch_bounds__(kPlanThreads) tile_plan_kernel(
+    uint32_t T, const uint32_t* __restrict__ tile_count, uint32_t* __restrict__ tile_sub,
+    uint32_t* __restrict__ tile_start, uint32_t* __restrict__ total_out,
+    uint32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[kPlanTiles + kPlanTiles / 32];
+  __shared__ uint32_t sh[kPlanThreads / 64 + 1];
+  __shared__ OrderShared o;
+  const uint32_t t = threadIdx.x;
+  // 1. per tile (coalesced over tiles): the counter sets' offsets and the total
+  uint32_t m = 0;
+  for (uint32_t q = t; q < T; q += kPlanThreads) {
+    uint32_t c[kTileSub];
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSub; k++) c[k] = tile_count[k * T + q];
+    uint32_t acc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kTileSub; k++) {
+      tile_sub[k * T + q] = acc;
+      acc += c[k];
+    }
+    cnt[plan_slot(q)] = acc;
+    m = max(m, acc);
+  }
+  __syncthreads();
+  // 2. tile_start: thread t scans tiles [16 t, 16 t + 16)
+  uint32_t vals[kPlanItems];
+  uint32_t sum = 0;
+  const uint32_t base = t * kPlanItems;
+#pragma unroll
+  for (uint32_t k = 0; k < kPlanItems; k++) {
+    vals[k] = base + k < T ? cnt[plan_slot(base + k)] : 0u;
+    sum += vals[k];
+  }
+  uint32_t total;
+  const uint32_t incl = block_inclusive_scan<kPlanThreads>(sum, sh, &total);
+  uint32_t run = incl - sum;
+#pragma unroll
+  for (uint32_t k = 0; k < kPlanItems; k++) {
+    if (base + k < T) tile_start[base + k] = run;
+    run += vals[k];
+  }
+  if (t == 0) {
+    tile_start[T] = total;
+    if (total_out) *total_out = total;
+  }
+  // 3. the dispatch order
+  if (order) order_from_counts(T, m, [&](uint32_t q) { return cnt[plan_slot(q)]; }, o, order);
 }
 
 void tile_order(splat_ctx* ctx, const BinBuffers& bb, size_t T) {
   if (!bb.tile_order || T == 0) return;
   launch(ctx, "bin_tile_order", tile_order_kernel, dim3(1), dim3(1024), 0,
          (const uint32_t*)bb.tile_start, (uint32_t)T, bb.tile_order);
+}
+
+// After the counting pass: tile_sub, tile_start (total to *total_dev) and the dispatch order.
+void tile_layout(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint32_t* total_dev,
+                 uint32_t* scan_sums) {
+  if (T <= kPlanTiles) {
+    launch(ctx, "bin_tile_plan", tile_plan_kernel, dim3(1), dim3(kPlanThreads), 0, (uint32_t)T,
+           (const uint32_t*)bb.tile_count, bb.tile_sub, bb.tile_start, total_dev, bb.tile_order);
+    return;
+  }
+  tile_starts(ctx, bb, T, total_dev, scan_sums);
+  tile_order(ctx, bb, T);
+}
+
+// counts[0] (big tiles) and counts[2] (mid tiles) start at zero; `queue` holds T words.
+void sort_tiles(splat_ctx* ctx, const BinBuffers& bb, size_t T, uint64_t* keys_out,
+                uint32_t* queue, uint32_t* counts) {
+  launch(ctx, "bin_sort_tiles", sort_tiles_kernel, dim3(blocks_for(T, 4)), dim3(256), 0,
+         (uint32_t)T, (const uint32_t*)bb.tile_start, (const uint4*)bb.bucket,
+         bb.sorted_entry, bb.sorted_gauss, keys_out, queue,
+         counts, counts + 2);
+  launch(ctx, "bin_sort_mid", sort_mid_kernel, dim3(512), dim3(256), 0, (uint32_t)T,
+         (const uint32_t*)bb.tile_start, (const uint4*)bb.bucket,
+         bb.sorted_entry, bb.sorted_gauss, keys_out,
+         (const uint32_t*)queue, (const uint32_t*)(counts + 2));
+  launch(ctx, "bin_sort_big", sort_big_kernel, dim3(256), dim3(1024), 0,
+         (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
+         bb.sorted_entry, bb.sorted_gauss, keys_out, (const uint32_t*)queue,
+         (const uint32_t*)counts);
 }
 
 }  // namespace
@@ -578,14 +875,14 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
     store["entry_gaussian"].reserve(cap * 4);
     store["entry_tile"].reserve(cap * 4);
     store["entry_slot"].reserve(cap * 4);
-    store["bucket"].reserve(cap * 8);
+    store["bucket"].reserve(cap * 16);
     store["sorted_entry"].reserve(cap * 4);
     store["sorted_gauss"].reserve(cap * 4);
   }
   bb.entry_gaussian = store["entry_gaussian"].as<uint32_t>();
   bb.entry_tile = store["entry_tile"].as<uint32_t>();
   bb.entry_slot = store["entry_slot"].as<uint32_t>();
-  bb.bucket = store["bucket"].as<uint64_t>();
+  bb.bucket = store["bucket"].as<uint4>();
   bb.sorted_entry = store["sorted_entry"].as<uint32_t>();
   bb.sorted_gauss = store["sorted_gauss"].as<uint32_t>();
   if (ctx->bin_method == 1) {  // radix sorts (same result)
@@ -601,7 +898,7 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
   bb.block_marks = (uint32_t*)store["block_marks"].reserve((E / kKeygenEntries + 2) * 4);
   // 2. keygen + per-tile counts
   fill_u32(ctx, bb.tile_count, kTileSub * T, 0);
-  fill_u32(ctx, small + 1, 1, 0);
+  fill_u32(ctx, small + 1, 3, 0);  // the queued-tile counts small[1] and small[3]
   launch(ctx, "bin_marks", block_marks_kernel<>, dim3(blocks_for(n, 256)), dim3(256), 0,
          (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
          (const StepStatus*)nullptr);
@@ -610,8 +907,8 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
          (const uint2*)pb.rect, tiles_x, bb.entry_gaussian, bb.entry_tile, bb.entry_slot,
          bb.tile_count, (uint32_t)T, (const uint32_t*)bb.block_marks,
          (const StepStatus*)nullptr);
-  // 3. tile starts
-  tile_starts(ctx, bb, T, small + 2, nullptr);
+  // 3. tile starts and the dispatch order
+  tile_layout(ctx, bb, T, small + 2, nullptr);
   // 4. scatter into buckets
   launch(ctx, "bin_scatter", scatter_kernel, dim3(blocks_for(E, 256)), dim3(256), 0,
          (uint32_t)E, (const uint32_t*)bb.entry_gaussian, (const uint32_t*)bb.entry_tile,
@@ -619,14 +916,7 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
          (const uint32_t*)bb.tile_sub, (uint32_t)T, (const float*)pb.depth, bb.bucket,
          (const StepStatus*)nullptr);
   // 5. per-tile sorts
-  launch(ctx, "bin_sort_small", sort_small_kernel, dim3((unsigned)T), dim3(256), 0, (uint32_t)T,
-         (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
-         bb.sorted_entry, bb.sorted_gauss, keys_out, big_list, small + 1);
-  launch(ctx, "bin_sort_big", sort_big_kernel, dim3(256), dim3(1024), 0,
-         (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
-         bb.sorted_entry, bb.sorted_gauss, keys_out, (const uint32_t*)big_list,
-         (const uint32_t*)(small + 1));
-  tile_order(ctx, bb, T);
+  sort_tiles(ctx, bb, T, keys_out, big_list, small + 1);
   return E;
 }
 
@@ -641,10 +931,10 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
                       uint32_t* big_list, uint32_t* scan_sums, StepStatus* status,
                       const BatchDev* bd) {
   const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
-  // the scan's 64-bit total, the tile counters and the big-tile count, in one launch (the
-  // scan writes only small[0] and total64)
+  // the scan's 64-bit total, the tile counters and the queued-tile counts small[1] and
+  // small[3], in one launch (the scan writes only small[0] and total64)
   fill3_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, bb.tile_count, kTileSub * T, small + 1,
-            1, 0u);
+            3, 0u);
   exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64);
   bin_guard(ctx, total64, capacity, status);
   // The entry kernels cover the capacity and read the step's count from `status`.
@@ -663,20 +953,13 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
            (const uint2*)pb.rect, bb.tiles_x, bb.entry_gaussian, bb.entry_tile, bb.entry_slot,
            bb.tile_count, (uint32_t)T, (const uint32_t*)bb.block_marks,
            (const StepStatus*)status);
-  tile_starts(ctx, bb, T, small + 2, scan_sums);
+  tile_layout(ctx, bb, T, small + 2, scan_sums);
   launch(ctx, "bin_scatter", scatter_kernel, dim3(blocks_for(capacity, 256)), dim3(256), 0, 0u,
          (const uint32_t*)bb.entry_gaussian, (const uint32_t*)bb.entry_tile,
          (const uint32_t*)bb.entry_slot, (const uint32_t*)bb.tile_start,
          (const uint32_t*)bb.tile_sub, (uint32_t)T, (const float*)pb.depth, bb.bucket,
          (const StepStatus*)status);
-  launch(ctx, "bin_sort_small", sort_small_kernel, dim3((unsigned)T), dim3(256), 0, (uint32_t)T,
-         (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
-         bb.sorted_entry, bb.sorted_gauss, (uint64_t*)nullptr, big_list, small + 1);
-  launch(ctx, "bin_sort_big", sort_big_kernel, dim3(256), dim3(1024), 0,
-         (const uint32_t*)bb.tile_start, bb.bucket, (const uint32_t*)bb.entry_gaussian,
-         bb.sorted_entry, bb.sorted_gauss, (uint64_t*)nullptr, (const uint32_t*)big_list,
-         (const uint32_t*)(small + 1));
-  tile_order(ctx, bb, T);
+  sort_tiles(ctx, bb, T, nullptr, big_list, small + 1);
 }
 
 

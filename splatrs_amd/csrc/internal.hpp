@@ -280,6 +280,35 @@ __device__ __forceinline__ uint32_t find_segment(const uint32_t* base, uint32_t 
   return lo;
 }
 
+// Inclusive scan of one value per thread across the workgroup (THREADS lanes, sh holds
+// THREADS / 64 + 1 words); returns the inclusive prefix and writes the workgroup total to
+// *total.
+template <int THREADS>
+__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(v, d, 64);
+    if (lane >= d) v += o;
+  }
+  if (lane == 63) sh[wave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < THREADS / 64; w++) {
+      const uint32_t t = sh[w];
+      sh[w] = run;
+      run += t;
+    }
+    sh[THREADS / 64] = run;
+  }
+  __syncthreads();
+  v += sh[wave];
+  *total = sh[THREADS / 64];
+  __syncthreads();
+  return v;
+}
+
 // Learning rate of parameter plane j (LearningRates, src/adam.rs:11-26).
 __device__ __forceinline__ float plane_lr(int j, const splat_learning_rates& lr) {
   if (j < 3) return lr.mean;
@@ -339,7 +368,8 @@ struct BinBuffers {
   uint32_t* tile_start;      // T + 1
   uint32_t* block_marks;     // >= capacity / 256 + 2: Gaussian of each keygen workgroup's
                              // first entry (a workgroup handles 256 entries)
-  uint64_t* bucket;          // E: (depth bits << 32 | entry), grouped by tile
+  uint4* bucket;             // E records {entry, depth bits, Gaussian, 0} (the 64-bit key
+                             // depth bits << 32 | entry in .x/.y), grouped by tile
   uint32_t* sorted_entry;    // E: entry ids in (tile, depth, entry) order
   uint32_t* sorted_gauss;    // E: their Gaussian ids
   uint32_t tiles_x, tiles_y;

@@ -11,34 +11,6 @@ constexpr int kScanTile = kScanThreads * kScanItems;  // 4096 items per block
 constexpr int kTopThreads = 1024;
 constexpr int kTopItems = 16;  // one workgroup scans up to 16384 block sums
 
-// Inclusive scan of one value per thread across the workgroup; returns the inclusive
-// prefix and writes the workgroup total to *total.
-template <int THREADS>
-__device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(v, d, 64);
-    if (lane >= d) v += o;
-  }
-  if (lane == 63) sh[wave] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int w = 0; w < THREADS / 64; w++) {
-      const uint32_t t = sh[w];
-      sh[w] = run;
-      run += t;
-    }
-    sh[THREADS / 64] = run;
-  }
-  __syncthreads();
-  v += sh[wave];
-  *total = sh[THREADS / 64];
-  __syncthreads();
-  return v;
-}
-
 // Sum of one 64-bit value per thread over the workgroup, added to *total (one atomic per
 // workgroup; *total starts at zero).
 template <int THREADS>

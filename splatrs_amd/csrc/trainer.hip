@@ -237,7 +237,7 @@ StepPlan plan_step(splat_trainer* t, const splat_camera& cam, const splat_step_p
   bb.entry_gaussian = (uint32_t*)st["entry_gaussian"].reserve(ec * 4);
   bb.entry_tile = (uint32_t*)st["entry_tile"].reserve(ec * 4);
   bb.entry_slot = (uint32_t*)st["entry_slot"].reserve(ec * 4);
-  bb.bucket = (uint64_t*)st["bucket"].reserve(ec * 8);
+  bb.bucket = (uint4*)st["bucket"].reserve(ec * 16);
   bb.sorted_entry = (uint32_t*)st["sorted_entry"].reserve(ec * 4);
   bb.sorted_gauss = (uint32_t*)st["sorted_gauss"].reserve(ec * 4);
   P.small = (uint32_t*)st["bin_small"].reserve(64);

@@ -186,10 +186,12 @@ def test_duplicates_break_ties_by_entry(ctx):
     full_pipeline(ctx, g, ring_cameras(1, 48, 40, 2.4)[0], rec.proj_opts(1), (0.0, 0.0, 0.0))
 
 
-@pytest.mark.parametrize("n", [3000, 20000])
+@pytest.mark.parametrize("n", [40, 64, 65, 200, 400, 512, 513, 700, 1024, 1025, 3000, 20000])
 def test_crowded_tiles_use_large_sorts(ctx, n):
     # n Gaussians around one point: every tile of a 64x64 image gets n entries, which
-    # exercises the 1024-lane LDS sort (3000) and the global-memory sort (20000).
+    # exercises each per-tile sort at and beyond its size limits: one wave with 1 .. 8
+    # keys per lane (n <= 512), the queued one-wave sort with 16 keys per lane (n <= 1024),
+    # the 1024-lane LDS sort (3000) and the global-memory sort (20000).
     say(f"crowded tiles, {n} entries per tile")
     rng = np.random.default_rng(n)
     g = np.zeros((n, 59), np.float32)
