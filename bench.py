@@ -50,7 +50,7 @@ def algorithmic_bytes(name, N, V, E, P):
         "loss_grad": 63 * P,                 # 3 maps + pred + target read; dL/dimage written
         "bin_keygen": 12 * E + 16 * N,
         "bin_scatter": 32 * E,               # gaussian, tile, slot, depth read; key + gaussian written
-        "bin_sort_tiles": 20 * E,            # key + gaussian read; entry + gaussian written
+        "bin_sort_tiles": 24 * E,            # 16-byte record read; entry + gaussian written
         "stats": 24 * N,
         # slots + params + both moments read; params and moments written (no gradient planes)
         "grad_adam": 36 * E + 1416 * N,

@@ -46,7 +46,7 @@ __device__ __forceinline__ uint32_t search_offsets(Off off, uint32_t lo, uint32_
   return lo;
 }
 
-constexpr uint32_t kKeygenSpan = 2048;  // Gaussian offsets staged per keygen workgroup
+constexpr uint32_t kKeygenSpan = 1024;  // Gaussians (offsets, rectangles) staged per keygen workgroup
 
 // marks[b] = the Gaussian that holds entry b * EPB (the first entry of keygen workgroup b,
 // EPB entries per workgroup), written by that Gaussian.
@@ -63,27 +63,33 @@ __global__ void __launch_bounds__(256) block_marks_kernel(const uint32_t* __rest
 
 // The Gaussians of one keygen workgroup's entries [b * kKeygenEntries, ...): they lie in
 // [g0, ghi) with entry_offset[g0] <= every entry < entry_offset[ghi].  Their offsets are
-// staged in LDS (call from every lane) unless the span exceeds kKeygenSpan Gaussians (long
+// staged in LDS (call from every lane) unless the span exceeds SPAN Gaussians (long
 // runs of culled ones); then lookups search global memory.
 struct KeygenSpan {
   uint32_t g0, ghi;
   bool staged;
 };
 
-template <uint32_t EPB = kKeygenEntries>
+// (With `rect`, the span's rectangles are staged in s_rect as well.)  SPAN: the staging
+// capacity in Gaussians.
+template <uint32_t EPB = kKeygenEntries, uint32_t SPAN = kKeygenSpan>
 __device__ __forceinline__ KeygenSpan keygen_span(uint32_t e_total,
                                                   const uint32_t* __restrict__ entry_offset,
                                                   uint32_t n, const uint32_t* __restrict__ marks,
-                                                  uint32_t* s_off) {
+                                                  uint32_t* s_off,
+                                                  const uint2* __restrict__ rect = nullptr,
+                                                  uint2* s_rect = nullptr) {
   const uint32_t b = blockIdx.x;
   const uint32_t nb = (e_total + EPB - 1) / EPB;
   KeygenSpan sp;
   sp.g0 = marks[b];
   sp.ghi = b + 1 < nb ? marks[b + 1] + 1 : n;
-  sp.staged = sp.ghi - sp.g0 <= kKeygenSpan;
+  sp.staged = sp.ghi - sp.g0 <= SPAN;
   if (sp.staged)
-    for (uint32_t q = threadIdx.x; q <= sp.ghi - sp.g0; q += blockDim.x)
+    for (uint32_t q = threadIdx.x; q <= sp.ghi - sp.g0; q += blockDim.x) {
       s_off[q] = entry_offset[sp.g0 + q];
+      if (rect && q < sp.ghi - sp.g0) s_rect[q] = rect[sp.g0 + q];
+    }
   __syncthreads();
   return sp;
 }
@@ -119,6 +125,8 @@ __device__ __forceinline__ uint32_t keygen_find(const KeygenSpan& sp, uint32_t e
   return search_offsets([&](uint32_t g) { return entry_offset[g]; }, sp.g0, sp.ghi, e);
 }
 
+// The span's rectangles are staged beside the offsets, so that after the search an entry
+// needs no dependent global load before its counter atomic.
 __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict__ entry_offset,
                                                      uint32_t n, uint32_t e_total,
                                                      const uint2* __restrict__ rect,
@@ -131,20 +139,21 @@ __global__ void __launch_bounds__(256) keygen_kernel(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ marks,
                                                      const StepStatus* __restrict__ status) {
   __shared__ uint32_t s_off[kKeygenSpan + 1];
+  __shared__ uint2 s_rect[kKeygenSpan];
   if (status) {
     if (status->abort) return;
     e_total = status->entries;
   }
   const uint32_t eb = blockIdx.x * kKeygenEntries;
   if (eb >= e_total) return;  // uniform: no entry of this workgroup exists
-  const KeygenSpan sp = keygen_span(e_total, entry_offset, n, marks, s_off);
+  const KeygenSpan sp = keygen_span(e_total, entry_offset, n, marks, s_off, rect, s_rect);
 #pragma unroll
   for (uint32_t k = 0; k < kKeygenPerLane; k++) {
     const uint32_t e = eb + k * kKeygenThreads + threadIdx.x;
     if (e >= e_total) break;
     const uint32_t g = keygen_find(sp, e, entry_offset, s_off);
-    const uint32_t local = e - entry_offset[g];
-    const uint2 rc = rect[g];
+    const uint32_t local = e - (sp.staged ? s_off[g - sp.g0] : entry_offset[g]);
+    const uint2 rc = sp.staged ? s_rect[g - sp.g0] : rect[g];
     const uint32_t x0 = rc.x & 0xffffu, y0 = rc.x >> 16, x1 = rc.y & 0xffffu;
     const uint32_t w = x1 - x0;
     const uint2 qr = rect_divmod(local, w);
@@ -590,6 +599,7 @@ __global__ void __launch_bounds__(256) ordered_rows_kernel(
 
 constexpr uint32_t kOrderedPerLane = 4;
 constexpr uint32_t kOrderedEntries = kKeygenThreads * kOrderedPerLane;  // per workgroup
+constexpr uint32_t kOrderedSpan = 2048;  // rows whose offsets are staged per workgroup
 
 // Position j of the depth-ordered entry list: the row (soff, marks: its offsets in depth
 // order), whose record gives the entry's tile (the local index row-major over the
@@ -600,14 +610,14 @@ __global__ void __launch_bounds__(256) keygen_ordered_kernel(
     uint32_t e_total, const uint32_t* __restrict__ marks, uint32_t* __restrict__ tkeys,
     uint32_t* __restrict__ tvals, uint32_t* __restrict__ tgauss,
     uint32_t* __restrict__ entry_gaussian, const StepStatus* __restrict__ status) {
-  __shared__ uint32_t s_off[kKeygenSpan + 1];
+  __shared__ uint32_t s_off[kOrderedSpan + 1];
   if (status) {
     if (status->abort) return;
     e_total = status->entries;
   }
   const uint32_t eb = blockIdx.x * kOrderedEntries;
   if (eb >= e_total) return;  // uniform: no entry of this workgroup exists
-  const KeygenSpan sp = keygen_span<kOrderedEntries>(e_total, soff, n, marks, s_off);
+  const KeygenSpan sp = keygen_span<kOrderedEntries, kOrderedSpan>(e_total, soff, n, marks, s_off);
 #pragma unroll
   for (uint32_t u = 0; u < kOrderedPerLane; u++) {
     const uint32_t j = eb + u * kKeygenThreads + threadIdx.x;
@@ -767,20 +777,31 @@ __global__ void __launch_bounds__(kPlanThreads) tile_plan_kernel(
   __shared__ uint32_t sh[kPlanThreads / 64 + 1];
   __shared__ OrderShared o;
   const uint32_t t = threadIdx.x;
-  // 1. per tile (coalesced over tiles): the counter sets' offsets and the total
+  // 1. per tile (coalesced over tiles): the counter sets' offsets and the total, four
+  // tiles per thread at a time so that their 32 loads are in flight together
+  constexpr uint32_t kPlanBatch = 4;
   uint32_t m = 0;
-  for (uint32_t q = t; q < T; q += kPlanThreads) {
-    uint32_t c[kTileSub];
+  for (uint32_t q0 = t; q0 < T; q0 += kPlanBatch * kPlanThreads) {
+    uint32_t c[kPlanBatch][kTileSub];
 #pragma unroll
-    for (uint32_t k = 0; k < kTileSub; k++) c[k] = tile_count[k * T + q];
-    uint32_t acc = 0;
+    for (uint32_t j = 0; j < kPlanBatch; j++) {
+      const uint32_t q = q0 + j * kPlanThreads;
 #pragma unroll
-    for (uint32_t k = 0; k < kTileSub; k++) {
-      tile_sub[k * T + q] = acc;
-      acc += c[k];
+      for (uint32_t k = 0; k < kTileSub; k++) c[j][k] = q < T ? tile_count[k * T + q] : 0u;
     }
-    cnt[plan_slot(q)] = acc;
-    m = max(m, acc);
+#pragma unroll
+    for (uint32_t j = 0; j < kPlanBatch; j++) {
+      const uint32_t q = q0 + j * kPlanThreads;
+      if (q >= T) break;
+      uint32_t acc = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kTileSub; k++) {
+        tile_sub[k * T + q] = acc;
+        acc += c[j][k];
+      }
+      cnt[plan_slot(q)] = acc;
+      m = max(m, acc);
+    }
   }
   __syncthreads();
   // 2. tile_start: thread t scans tiles [16 t, 16 t + 16)
