@@ -58,7 +58,8 @@ def algorithmic_bytes(name, N, V, E, P):
     return table.get(name)
 
 
-def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_thresholds=()):
+def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_thresholds=(),
+                    credit_views=0, credit_ratio=1.0):
     """Full training with densification on the ellipsoid scene (300k surface points, 100
     cameras 1297x840 f = 1100 on a radius-6 hemisphere, targets from a 2M-Gaussian ground
     truth, `iters` iterations, densify every 100 from 500 to 5000 with threshold 2e-4,
@@ -66,6 +67,9 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
       plain   -- the reference's threshold heuristic (with the scorer logging and fitting
                  online; the model is bitwise the plain method's),
       learned -- the scorer trained in the first run makes the decisions.
+    credit_views > 0 labels the logged clones / splits by credit (the image share of the
+    Gaussians they added at the outcome, over that many views, against credit_ratio times
+    the median Gaussian's share) instead of survival.
     Returns PSNR over all views, Gaussian counts and wall-clock iterations/s of each run."""
     import numpy as np
     from splatrs_amd import abi, scenes
@@ -98,7 +102,8 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
         if mode:
             tr.attach_scorer(scorer, rec.learn_config(
                 mode=mode, samples_per_event=32768, max_events=64, outcome_delay=500,
-                eval_views=8, threshold=threshold,
+                eval_views=8, threshold=threshold, credit_views=credit_views,
+                credit_ratio=credit_ratio,
                 fit=rec.scorer_fit_config(epochs=2, batch=4096, lr=1e-3, seed=1)))
         tot = {"cloned": 0, "split": 0, "pruned": 0}
 
@@ -128,13 +133,29 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
     learned = frontier[0]
     ctx.close()
     grown = lambda r: r["cloned"] + r["split"]
+    # the plain heuristic's PSNR at each learned run's Gaussian count, linear between the
+    # plain runs (threshold 2e-4 and the extra ones) that bracket it
+    pts = sorted([(plain["gaussians"], plain["psnr_db"])] +
+                 [(r["gaussians"], r["psnr_db"]) for r in frontier if r["policy"] == "plain"])
+    for r in frontier:
+        if r["policy"] != "learned":
+            continue
+        for (n0, p0), (n1, p1) in zip(pts, pts[1:]):
+            if n0 <= r["gaussians"] <= n1 and n1 > n0:
+                at = p0 + (p1 - p0) * (r["gaussians"] - n0) / (n1 - n0)
+                r["plain_psnr_at_count_db"] = round(at, 3)
+                r["gain_at_matched_count_db"] = round(r["psnr_db"] - at, 3)
+    keys = ("policy", "threshold", "grad_threshold", "psnr_db", "gaussians",
+            "plain_psnr_at_count_db", "gain_at_matched_count_db")
     return {"workload": "300k points on 20 ellipsoids -> densify, 100 cams 1297x840, "
-                        f"{iters} iterations", "plain": plain, "learned": learned,
+                        f"{iters} iterations",
+            "labels": (f"credit over {credit_views} views, bar {credit_ratio} x median share"
+                       if credit_views else "survival"),
+            "plain": plain, "learned": learned,
             "psnr_delta_db": round(learned["psnr_db"] - plain["psnr_db"], 3),
             "gaussian_ratio": round(learned["gaussians"] / max(plain["gaussians"], 1), 4),
             "densified_ratio": round(grown(learned) / max(grown(plain), 1), 4),
-            "frontier": [{k: r[k] for k in ("policy", "threshold", "grad_threshold", "psnr_db",
-                                             "gaussians") if k in r} for r in frontier]}
+            "frontier": [{k: r[k] for k in keys if k in r} for r in frontier]}
 
 
 def densify_bench(n, rounds, log=print):
@@ -301,10 +322,15 @@ def main():
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain-vs-learned densification comparison")
     ap.add_argument("--learned-iters", type=int, default=7000)
-    ap.add_argument("--learned-thresholds", default="0.2",
+    ap.add_argument("--learned-thresholds", default="0.3,0.4",
                     help="scorer decision thresholds of the learned runs (comma separated)")
-    ap.add_argument("--plain-thresholds", default="",
+    ap.add_argument("--plain-thresholds", default="0.00025,0.0003",
                     help="extra plain runs at these densify gradient thresholds")
+    ap.add_argument("--credit-views", type=int, default=8,
+                    help="label the scorer's samples by image-share credit over this many "
+                         "views (0: by survival)")
+    ap.add_argument("--credit-ratio", type=float, default=1.0,
+                    help="credit bar, as a fraction of the median Gaussian's share")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched many-scene configuration")
     ap.add_argument("--densify-only", action="store_true",
@@ -325,7 +351,8 @@ def main():
         r = learned_compare(args.learned_iters,
                             log=lambda m: print(m, file=sys.stderr, flush=True),
                             learned_thresholds=[float(x) for x in args.learned_thresholds.split(",")],
-                            plain_thresholds=[float(x) for x in args.plain_thresholds.split(",") if x])
+                            plain_thresholds=[float(x) for x in args.plain_thresholds.split(",") if x],
+                            credit_views=args.credit_views, credit_ratio=args.credit_ratio)
         print(json.dumps(r), flush=True)
         return
     if args.calibrate_copy:
@@ -518,7 +545,8 @@ def main():
                         ("learned", lambda: learned_compare(
                             args.learned_iters, err,
                             [float(x) for x in args.learned_thresholds.split(",")],
-                            [float(x) for x in args.plain_thresholds.split(",") if x]))):
+                            [float(x) for x in args.plain_thresholds.split(",") if x],
+                            args.credit_views, args.credit_ratio))):
             try:
                 out[key] = fn()
             except Exception as e:  # the headline line is printed regardless

@@ -692,11 +692,13 @@ int splat_scorer_fit(splat_scorer* s, const float* features, const uint8_t* labe
 /* Learned densification inside splat_train (no counterpart in the reference).
  * mode 1 (learn): the reference's decisions are applied unchanged; a sample of them is
  *   logged at each densification event with the 12 features, labelled `outcome_delay`
- *   iterations later (a clone/split counts as positive while any descendant is alive with
- *   an opacity of at least useful_ratio times its parent's at the decision and at least
- *   useful_opacity),
- *   weighted by the PSNR change over that window, and the scorer is fit online on every
- *   labelled sample.
+ *   iterations later, weighted by the PSNR change over that window, and the scorer is fit
+ *   online on every labelled sample.  A clone/split is positive, with credit_views > 0,
+ *   when the Gaussians it added (a clone's copy, a split's children, and their
+ *   descendants) hold at least credit_ratio times the median Gaussian's share of the
+ *   images of credit_views evenly spaced views (sum of alpha T over the pixels); with
+ *   credit_views = 0, while any descendant is alive with an opacity of at least
+ *   useful_ratio times its parent's at the decision and at least useful_opacity.
  * mode 2 (learned): the scorer's logits decide clone / split / prune at each event; the
  *   reference's prune rules still apply.
  * Features (binary32, per Gaussian): log2(mean grad / thr + 1e-6) / 8, log2(max grad / thr
@@ -718,6 +720,11 @@ typedef struct {
                                   window) */
   float useful_ratio;          /* ... and at least this fraction of its parent's opacity at
                                   the decision */
+  uint32_t credit_views;       /* > 0: label by credit instead: the image share (sum of
+                                  alpha T over the pixels of this many evenly spaced views)
+                                  of the Gaussians the decision added, at the outcome */
+  float credit_ratio;          /* a clone / split is useful when its credit reaches this
+                                  fraction of the median Gaussian's share at the outcome */
   uint32_t pad_;
   splat_scorer_fit_config fit; /* each online update */
 } splat_learn_config;

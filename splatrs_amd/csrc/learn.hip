@@ -11,11 +11,14 @@
 //     passed, each logged clone/split is labelled by whether any descendant of it is still
 //     alive with an opacity of at least useful_ratio times its parent's at the decision (and
 //     useful_opacity): a densification whose offspring were pruned or faded is labelled
-//     negative;
+//     negative; or (credit_views > 0) by its credit, the share of the images of
+//     credit_views evenly spaced views (the sum of alpha T over their pixels) that the
+//     Gaussians it added hold at the outcome, against the median Gaussian's share;
 //     samples are weighted by the PSNR change over the same window, and the scorer is fit
 //     on every labelled sample so far;
 //   * mode 2 (learned): the scorer's logits decide clone / split / prune; the reference's
 //     prune rules still apply on top (fallback).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -153,6 +156,35 @@ __global__ void __launch_bounds__(256) alive_kernel(const uint32_t* __restrict__
       atomicAdd(&alive[s - b], 1u);
     s = s_parent[s];
   }
+}
+
+// Credits of the samples [b, b + c): every live row walks up its lineage and adds its image
+// share to each sample of this event it meets, in fixed point (2^-16 pixel units), so the
+// sums do not depend on the order of the atomics.
+__global__ void __launch_bounds__(256) credit_kernel(const uint32_t* __restrict__ origin,
+                                                     uint32_t n, const float* __restrict__ share,
+                                                     const uint32_t* __restrict__ s_parent,
+                                                     uint32_t b, uint32_t c,
+                                                     unsigned long long* __restrict__ credit) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float sh = share[i];
+  if (!(sh > 0.0f)) return;
+  const unsigned long long q = (unsigned long long)(rmin(sh, 1e12f) * 65536.0f);
+  uint32_t s = origin[i];
+  for (int depth = 0; depth < 64 && s != kNoSample && s >= b; depth++) {
+    if (s < b + c) atomicAdd(&credit[s - b], q);
+    s = s_parent[s];
+  }
+}
+
+// alive[k] = 1 where the credit reaches `need` (fixed point), the input of label_kernel.
+__global__ void __launch_bounds__(256) credit_alive_kernel(
+    const unsigned long long* __restrict__ credit, uint32_t c, unsigned long long need,
+    uint32_t* __restrict__ alive) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= c) return;
+  alive[k] = credit[k] >= need ? 1u : 0u;
 }
 
 // Labels: clone / split only if the reference densified the row and a descendant is
@@ -297,39 +329,74 @@ void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float exte
            logit(L.cfg.prune_threshold), dec);
     decide = dec;
   }
-  trainer_densify(t, cfg, extent, seed, decide, sample_of, iteration, report);
-  if (!learn_mode) return;
   // label the events whose outcome window has passed, then refit on all labels so far
   bool fresh = false;
-  while (L.next_event < L.events.size() &&
-         iteration >= L.events[L.next_event].iteration + L.cfg.outcome_delay) {
-    const LearnEvent& e = L.events[L.next_event];
-    if (e.count) {
-      uint32_t* alive = (uint32_t*)L.alive.reserve((size_t)e.count * 4 + 4);
-      fill_u32(ctx, alive, e.count, 0u);
-      // a descendant counts while its opacity reached its parent's at the decision (and
-      // useful_opacity); after an opacity reset inside the window every survivor counts
-      const int reset_inside = last_reset > e.iteration ? 1 : 0;
-      launch(ctx, "learn_alive", alive_kernel, dim3(blocks_for(t->n, 256)), dim3(256), 0,
-             (const uint32_t*)t->origin.as<uint32_t>(), (uint32_t)t->n,
-             (const float*)(t->params.as<float>() + 10 * t->stride), L.cfg.useful_opacity,
-             L.cfg.useful_ratio, reset_inside, (const float*)L.feats.as<float>(),
-             (const uint32_t*)L.parent.as<uint32_t>(), e.base, e.count, alive);
-      const float wpos = rclamp(1.0f + (psnr - e.psnr), 0.25f, 4.0f);
-      uint32_t* cnt = (uint32_t*)L.counter.reserve(16);
-      fill_u32(ctx, cnt, 1, 0u);
-      launch(ctx, "learn_label", label_kernel, dim3(blocks_for(e.count, 256)), dim3(256), 0,
-             (const uint8_t*)L.dec.as<uint8_t>(), (const uint32_t*)alive, e.base, e.count, wpos,
-             L.label.as<uint8_t>(), L.weight.as<float>(), cnt);
-      uint32_t h = 0;
-      SPLAT_HIP(hipMemcpyAsync(&h, cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
-      SPLAT_HIP(hipStreamSynchronize(ctx->stream));
-      L.positives += h;
-      fresh = true;
+  const auto resolve_due = [&] {
+    std::vector<unsigned long long> need_q;  // credit labels: the usefulness bar
+    float* share = nullptr;
+    while (L.next_event < L.events.size() &&
+           iteration >= L.events[L.next_event].iteration + L.cfg.outcome_delay) {
+      const LearnEvent& e = L.events[L.next_event];
+      if (e.count) {
+        uint32_t* alive = (uint32_t*)L.alive.reserve((size_t)e.count * 4 + 4);
+        if (L.cfg.credit_views > 0) {
+          if (!share) {  // the model's image shares, once per call
+            const uint32_t V = (uint32_t)t->views.size();
+            const uint32_t cv = std::min<uint32_t>(L.cfg.credit_views, V);
+            std::vector<uint32_t> list(cv);
+            for (uint32_t k = 0; k < cv; k++) list[k] = (uint32_t)(((uint64_t)k * V) / cv);
+            share = (float*)t->store["learn_share"].reserve(t->n * 4 + 4);
+            trainer_image_share(t, list.data(), cv, sh_degree, share);
+            std::vector<float> h(t->n);
+            SPLAT_HIP(hipMemcpyAsync(h.data(), share, t->n * 4, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+            SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+            std::nth_element(h.begin(), h.begin() + h.size() / 2, h.end());
+            const double med = h.empty() ? 0.0 : (double)h[h.size() / 2];
+            need_q.push_back((unsigned long long)(std::max(L.cfg.credit_ratio * med, 1e-6) *
+                                                  65536.0) + 1ull);
+          }
+          unsigned long long* credit =
+              (unsigned long long*)t->store["learn_credit"].reserve((size_t)e.count * 8 + 8);
+          fill_u32(ctx, reinterpret_cast<uint32_t*>(credit), (size_t)e.count * 2, 0u);
+          launch(ctx, "learn_credit", credit_kernel, dim3(blocks_for(t->n, 256)), dim3(256), 0,
+                 (const uint32_t*)t->origin.as<uint32_t>(), (uint32_t)t->n, (const float*)share,
+                 (const uint32_t*)L.parent.as<uint32_t>(), e.base, e.count, credit);
+          launch(ctx, "learn_credit_alive", credit_alive_kernel, dim3(blocks_for(e.count, 256)),
+                 dim3(256), 0, (const unsigned long long*)credit, e.count, need_q[0], alive);
+        } else {
+          fill_u32(ctx, alive, e.count, 0u);
+          // a descendant counts while its opacity reached its parent's at the decision (and
+          // useful_opacity); after an opacity reset inside the window every survivor counts
+          const int reset_inside = last_reset > e.iteration ? 1 : 0;
+          launch(ctx, "learn_alive", alive_kernel, dim3(blocks_for(t->n, 256)), dim3(256), 0,
+                 (const uint32_t*)t->origin.as<uint32_t>(), (uint32_t)t->n,
+                 (const float*)(t->params.as<float>() + 10 * t->stride), L.cfg.useful_opacity,
+                 L.cfg.useful_ratio, reset_inside, (const float*)L.feats.as<float>(),
+                 (const uint32_t*)L.parent.as<uint32_t>(), e.base, e.count, alive);
+        }
+        const float wpos = rclamp(1.0f + (psnr - e.psnr), 0.25f, 4.0f);
+        uint32_t* cnt = (uint32_t*)L.counter.reserve(16);
+        fill_u32(ctx, cnt, 1, 0u);
+        launch(ctx, "learn_label", label_kernel, dim3(blocks_for(e.count, 256)), dim3(256), 0,
+               (const uint8_t*)L.dec.as<uint8_t>(), (const uint32_t*)alive, e.base, e.count,
+               wpos, L.label.as<uint8_t>(), L.weight.as<float>(), cnt);
+        uint32_t h = 0;
+        SPLAT_HIP(hipMemcpyAsync(&h, cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
+        SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+        L.positives += h;
+        fresh = true;
+      }
+      L.resolved = e.base + e.count;
+      L.next_event++;
     }
-    L.resolved = e.base + e.count;
-    L.next_event++;
-  }
+  };
+  // credits are taken on the model as trained through the window, before this event's
+  // densification; survival on the model after it
+  if (learn_mode && L.cfg.credit_views > 0) resolve_due();
+  trainer_densify(t, cfg, extent, seed, decide, sample_of, iteration, report);
+  if (!learn_mode) return;
+  if (L.cfg.credit_views == 0) resolve_due();
   if (fresh && L.resolved >= L.cfg.fit.batch) {
     L.last_loss = scorer_fit_device(t->scorer, L.feats.as<float>(), L.label.as<uint8_t>(),
                                     L.weight.as<float>(), L.resolved, L.cfg.fit);
@@ -353,6 +420,8 @@ void splat_learn_config_default(splat_learn_config* c) {
   c->eval_views = 8;
   c->useful_opacity = 0.05f;
   c->useful_ratio = 0.5f;
+  c->credit_views = 0;
+  c->credit_ratio = 1.0f;
   splat_scorer_fit_config_default(&c->fit);
 }
 
@@ -370,6 +439,8 @@ int splat_trainer_attach_scorer(splat_trainer* t, splat_scorer* s, const splat_l
                       cfg->prune_threshold < 1.0f && cfg->useful_opacity > 0.0f &&
                       cfg->useful_opacity < 1.0f && cfg->useful_ratio >= 0.0f,
                   "thresholds must lie in (0, 1)");
+    SPLAT_REQUIRE(cfg->credit_views == 0 || cfg->credit_ratio > 0.0f,
+                  "credit_ratio must be positive");
     SPLAT_REQUIRE(s->ctx == t->ctx, "scorer and trainer must share a context");
     auto L = std::make_unique<LearnState>();
     L->cfg = *cfg;

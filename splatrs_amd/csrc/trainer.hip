@@ -467,6 +467,59 @@ std::vector<double> trainer_view_sse(splat_trainer* t, const uint32_t* views, ui
   return sse;
 }
 
+// dL/dimage of the contribution pass: 1 in the red channel of every pixel.
+__global__ void __launch_bounds__(256) unit_red_kernel(float* __restrict__ d, size_t npx) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npx) return;
+  d[3 * p] = 1.0f;
+  d[3 * p + 1] = 0.0f;
+  d[3 * p + 2] = 0.0f;
+}
+
+// contrib[i] (+)= the red colour gradient of row i, which under dL/dimage = (1, 0, 0) is
+// the row's sum of alpha T over the view's pixels.
+__global__ void __launch_bounds__(256) add_share_kernel(const splat_proj_grad* __restrict__ g,
+                                                        uint32_t n, int first,
+                                                        float* __restrict__ contrib) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float c = g[i].color[0];
+  contrib[i] = first ? c : contrib[i] + c;
+}
+
+void trainer_image_share(splat_trainer* t, const uint32_t* views, uint32_t count,
+                         int32_t sh_degree, float* contrib) {
+  drain(t);
+  splat_ctx* ctx = t->ctx;
+  const size_t n = t->n;
+  if (n == 0) return;
+  const float black[3] = {0.0f, 0.0f, 0.0f};
+  splat_proj_grad* grads = (splat_proj_grad*)t->store["share_grads"].reserve((n + 1) * 36);
+  for (uint32_t q = 0; q < count; q++) {
+    const TrainerView& tv = *t->views[views[q]];
+    const uint32_t W = tv.cam.width, H = tv.cam.height;
+    const size_t npx = (size_t)W * H;
+    splat_proj_opts opts{0.2f, 0.3f, 0, sh_degree};
+    ProjBuffers pb = proj_buffers(t);
+    project_forward(ctx, t->params.as<float>(), n, t->stride, tv.cam, opts, pb);
+    BinBuffers bb{};
+    const size_t E = bin_build(ctx, pb, n, (W + kTile - 1) / kTile, (H + kTile - 1) / kTile, bb,
+                               &t->entry_cap, t->store, nullptr);
+    float* img = (float*)t->store["image"].reserve(npx * 12 + 4);
+    float* final_t = (float*)t->store["final_t"].reserve(npx * 4 + 4);
+    uint32_t* n_contrib = (uint32_t*)t->store["n_contrib"].reserve(npx * 4 + 4);
+    float* dimage = (float*)t->store["share_dimage"].reserve(npx * 12 + 4);
+    float* slots = (float*)t->store["share_slots"].reserve((E + 1) * kSlots * 4);
+    raster_forward(ctx, pb, bb, W, H, black, img, final_t, n_contrib);
+    launch(ctx, "share_dimage", unit_red_kernel, dim3(blocks_for(npx, 256)), dim3(256), 0, dimage,
+           npx);
+    raster_backward(ctx, pb, bb, W, H, black, final_t, n_contrib, dimage, false, slots, false);
+    reduce_slots(ctx, bb.entry_offset, slots, n, grads, nullptr);
+    launch(ctx, "share_add", add_share_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+           (const splat_proj_grad*)grads, (uint32_t)n, q == 0 ? 1 : 0, contrib);
+  }
+}
+
 float trainer_eval_psnr(splat_trainer* t, uint32_t views, int32_t sh_degree, const float bg[3]) {
   views = std::min<uint32_t>(views, (uint32_t)t->views.size());
   if (views == 0) return 0.0f;

@@ -98,6 +98,11 @@ void trainer_densify(splat_trainer* t, const splat_densify_config& cfg, float ex
                      uint32_t iteration, splat_densify_report* report);
 // Mean PSNR (src/loss.rs:175-190 against the RGB8 targets) over the first `views` views.
 float trainer_eval_psnr(splat_trainer* t, uint32_t views, int32_t sh_degree, const float bg[3]);
+// Each row's share of the listed views' images: the sum of alpha T over their pixels (the
+// red colour gradient of a backward pass under dL/dimage = (1, 0, 0), black background),
+// into contrib[n].
+void trainer_image_share(splat_trainer* t, const uint32_t* views, uint32_t count,
+                         int32_t sh_degree, float* contrib);
 // learn.hip
 void learn_track_alloc(splat_trainer* t);
 void learn_zero_stats(splat_trainer* t);
