@@ -58,7 +58,7 @@ def algorithmic_bytes(name, N, V, E, P):
     return table.get(name)
 
 
-def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_thresholds=(),
+def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0),), plain_thresholds=(),
                     credit_views=0, credit_ratio=1.0):
     """Full training with densification on the ellipsoid scene (300k surface points, 100
     cameras 1297x840 f = 1100 on a radius-6 hemisphere, targets from a 2M-Gaussian ground
@@ -69,7 +69,10 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
       learned -- the scorer trained in the first run makes the decisions.
     credit_views > 0 labels the logged clones / splits by credit (the image share of the
     Gaussians they added at the outcome, over that many views, against credit_ratio times
-    the median Gaussian's share) instead of survival.
+    the median Gaussian's share) instead of survival.  learned_runs: (threshold,
+    share_prune, share_prune_from) of each mode-2 run; share_prune > 0 also prunes, at the
+    events from iteration share_prune_from on, the Gaussians whose share of all views'
+    images is below that fraction of the median share.
     Returns PSNR over all views, Gaussian counts and wall-clock iterations/s of each run."""
     import numpy as np
     from splatrs_amd import abi, scenes
@@ -95,7 +98,7 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
                position_lr_max_steps=iters, log_interval=500, seed=0)
     scorer = abi.Scorer(ctx, seed=0)
 
-    def run(mode, threshold=0.5, grad_threshold=0.0002):
+    def run(mode, threshold=0.5, grad_threshold=0.0002, share_prune=0.0, share_prune_from=0):
         tr = abi.Trainer(ctx, init)
         for c, v in zip(cams, views):
             tr.add_view(c, v)
@@ -103,7 +106,8 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
             tr.attach_scorer(scorer, rec.learn_config(
                 mode=mode, samples_per_event=32768, max_events=64, outcome_delay=500,
                 eval_views=8, threshold=threshold, credit_views=credit_views,
-                credit_ratio=credit_ratio,
+                credit_ratio=credit_ratio, share_prune=share_prune,
+                share_prune_from=share_prune_from,
                 fit=rec.scorer_fit_config(epochs=2, batch=4096, lr=1e-3, seed=1)))
         tot = {"cloned": 0, "split": 0, "pruned": 0}
 
@@ -124,10 +128,14 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
                "samples": int(st["resolved"]), "positives": int(st["positives"]),
                "fit_updates": int(st["updates"]), "fit_loss": round(float(st["last_loss"]), 4)}
         tr.close()
-        log(f"mode {mode} threshold {threshold} grad threshold {grad_threshold}: {out}")
+        log(f"mode {mode} threshold {threshold} grad threshold {grad_threshold} share prune "
+            f"{share_prune}: {out}")
         return out
     plain = run(1)  # the reference's decisions; the scorer is trained on their outcomes
-    frontier = [dict(policy="learned", threshold=th, **run(2, th)) for th in learned_thresholds]
+    frontier = [dict(policy="learned", threshold=th,
+                     **({"share_prune": sp, "share_prune_from": sf} if sp else {}),
+                     **run(2, th, share_prune=sp, share_prune_from=sf))
+                for th, sp, sf in learned_runs]
     frontier += [dict(policy="plain", grad_threshold=gt, **run(0, grad_threshold=gt))
                  for gt in plain_thresholds]
     learned = frontier[0]
@@ -138,14 +146,15 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
     pts = sorted([(plain["gaussians"], plain["psnr_db"])] +
                  [(r["gaussians"], r["psnr_db"]) for r in frontier if r["policy"] == "plain"])
     for r in frontier:
-        if r["policy"] != "learned":
+        if r["policy"] != "learned" or "share_prune" in r:
             continue
         for (n0, p0), (n1, p1) in zip(pts, pts[1:]):
             if n0 <= r["gaussians"] <= n1 and n1 > n0:
                 at = p0 + (p1 - p0) * (r["gaussians"] - n0) / (n1 - n0)
                 r["plain_psnr_at_count_db"] = round(at, 3)
                 r["gain_at_matched_count_db"] = round(r["psnr_db"] - at, 3)
-    keys = ("policy", "threshold", "grad_threshold", "psnr_db", "gaussians",
+    keys = ("policy", "threshold", "share_prune", "share_prune_from", "grad_threshold",
+            "psnr_db", "gaussians",
             "plain_psnr_at_count_db", "gain_at_matched_count_db")
     return {"workload": "300k points on 20 ellipsoids -> densify, 100 cams 1297x840, "
                         f"{iters} iterations",
@@ -156,6 +165,16 @@ def learned_compare(iters, log=print, learned_thresholds=(0.2,), plain_threshold
             "gaussian_ratio": round(learned["gaussians"] / max(plain["gaussians"], 1), 4),
             "densified_ratio": round(grown(learned) / max(grown(plain), 1), 4),
             "frontier": [{k: r[k] for k in keys if k in r} for r in frontier]}
+
+
+def learned_runs(spec):
+    """'threshold[:share_prune[:from]],...' -> [(threshold, share_prune, from), ...]."""
+    out = []
+    for item in spec.split(","):
+        f = item.split(":")
+        out.append((float(f[0]), float(f[1]) if len(f) > 1 else 0.0,
+                    int(f[2]) if len(f) > 2 else 0))
+    return out
 
 
 def densify_bench(n, rounds, log=print):
@@ -322,9 +341,11 @@ def main():
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain-vs-learned densification comparison")
     ap.add_argument("--learned-iters", type=int, default=7000)
-    ap.add_argument("--learned-thresholds", default="0.3,0.4",
-                    help="scorer decision thresholds of the learned runs (comma separated)")
-    ap.add_argument("--plain-thresholds", default="0.00025,0.0003",
+    ap.add_argument("--learned-runs", default="0.2:0.15:4500,0.35",
+                    help="learned runs, comma separated, each threshold[:share_prune[:from "
+                         "iteration]]: the scorer's decision threshold and the image-share "
+                         "pruning bar (fraction of the median share; 0 or absent: off)")
+    ap.add_argument("--plain-thresholds", default="0.000225,0.00025",
                     help="extra plain runs at these densify gradient thresholds")
     ap.add_argument("--credit-views", type=int, default=8,
                     help="label the scorer's samples by image-share credit over this many "
@@ -350,7 +371,7 @@ def main():
     if args.learned_only:
         r = learned_compare(args.learned_iters,
                             log=lambda m: print(m, file=sys.stderr, flush=True),
-                            learned_thresholds=[float(x) for x in args.learned_thresholds.split(",")],
+                            learned_runs=learned_runs(args.learned_runs),
                             plain_thresholds=[float(x) for x in args.plain_thresholds.split(",") if x],
                             credit_views=args.credit_views, credit_ratio=args.credit_ratio)
         print(json.dumps(r), flush=True)
@@ -543,8 +564,7 @@ def main():
         err = lambda m: print(m, file=sys.stderr, flush=True)
         for key, fn in (("batched", lambda: batched_bench(args.batched_steps, err)),
                         ("learned", lambda: learned_compare(
-                            args.learned_iters, err,
-                            [float(x) for x in args.learned_thresholds.split(",")],
+                            args.learned_iters, err, learned_runs(args.learned_runs),
                             [float(x) for x in args.plain_thresholds.split(",") if x],
                             args.credit_views, args.credit_ratio))):
             try:

@@ -208,6 +208,15 @@ __global__ void __launch_bounds__(256) label_kernel(const uint8_t* __restrict__ 
   if (y & 3u) atomicAdd(positives, 1u);
 }
 
+// Rows whose image share is below `bar` are pruned (and neither cloned nor split).
+__global__ void __launch_bounds__(256) share_prune_kernel(const float* __restrict__ share,
+                                                          uint32_t n, float bar,
+                                                          uint8_t* __restrict__ dec) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (share[i] < bar) dec[i] = (uint8_t)((dec[i] & ~(kDecClone | kDecSplit)) | kDecPrune);
+}
+
 // Learned decisions from the logits; the reference's prune rules stay in force.
 __global__ void __launch_bounds__(256) learned_decide_kernel(const float* __restrict__ logits,
                                                              const uint8_t* __restrict__ heur,
@@ -327,6 +336,20 @@ void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float exte
     launch(ctx, "learn_decide", learned_decide_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
            (const float*)logits, (const uint8_t*)heur, (uint32_t)n, logit(L.cfg.threshold),
            logit(L.cfg.prune_threshold), dec);
+    if (L.cfg.share_prune > 0.0f && iteration >= L.cfg.share_prune_from) {  // every view
+      const uint32_t V = (uint32_t)t->views.size();
+      std::vector<uint32_t> list(V);
+      for (uint32_t k = 0; k < V; k++) list[k] = k;
+      float* share = (float*)t->store["learn_share"].reserve(n * 4 + 4);
+      trainer_image_share(t, list.data(), V, sh_degree, share);
+      std::vector<float> h(n);
+      SPLAT_HIP(hipMemcpyAsync(h.data(), share, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+      SPLAT_HIP(hipStreamSynchronize(ctx->stream));
+      std::nth_element(h.begin(), h.begin() + n / 2, h.end());
+      const float bar = L.cfg.share_prune * h[n / 2];
+      launch(ctx, "learn_share_prune", share_prune_kernel, dim3(blocks_for(n, 256)), dim3(256),
+             0, (const float*)share, (uint32_t)n, bar, dec);
+    }
     decide = dec;
   }
   // label the events whose outcome window has passed, then refit on all labels so far
@@ -422,6 +445,8 @@ void splat_learn_config_default(splat_learn_config* c) {
   c->useful_ratio = 0.5f;
   c->credit_views = 0;
   c->credit_ratio = 1.0f;
+  c->share_prune = 0.0f;
+  c->share_prune_from = 0;
   splat_scorer_fit_config_default(&c->fit);
 }
 

@@ -41,7 +41,8 @@ LEARN_CONFIG = np.dtype([("mode", "<i4"), ("samples_per_event", "<u4"), ("max_ev
                          ("outcome_delay", "<u4"), ("threshold", "<f4"),
                          ("prune_threshold", "<f4"), ("eval_views", "<u4"),
                          ("useful_opacity", "<f4"), ("useful_ratio", "<f4"),
-                         ("credit_views", "<u4"), ("credit_ratio", "<f4"), ("pad_", "<u4"),
+                         ("credit_views", "<u4"), ("credit_ratio", "<f4"),
+                         ("share_prune", "<f4"), ("share_prune_from", "<u4"), ("pad_", "<u4"),
                          ("fit", SCORER_FIT_CONFIG)])
 LEARN_STATS = np.dtype([("logged", "<u8"), ("resolved", "<u8"), ("updates", "<u8"),
                         ("positives", "<u8"), ("events", "<u8"), ("last_loss", "<f4"),
@@ -180,7 +181,8 @@ def scorer_fit_config(epochs=4, batch=4096, lr=1e-3, seed=0):
 
 def learn_config(mode=1, samples_per_event=16384, max_events=64, outcome_delay=500,
                  threshold=0.5, prune_threshold=0.5, eval_views=8, useful_opacity=0.05,
-                 useful_ratio=0.5, credit_views=0, credit_ratio=1.0, fit=None):
+                 useful_ratio=0.5, credit_views=0, credit_ratio=1.0, share_prune=0.0,
+                 share_prune_from=0, fit=None):
     c = np.zeros((), LEARN_CONFIG)
     c["mode"] = mode
     c["samples_per_event"] = samples_per_event
@@ -193,6 +195,8 @@ def learn_config(mode=1, samples_per_event=16384, max_events=64, outcome_delay=5
     c["useful_ratio"] = useful_ratio
     c["credit_views"] = credit_views
     c["credit_ratio"] = credit_ratio
+    c["share_prune"] = share_prune
+    c["share_prune_from"] = share_prune_from
     c["fit"] = fit if fit is not None else scorer_fit_config()
     return c
 
