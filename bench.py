@@ -58,8 +58,8 @@ def algorithmic_bytes(name, N, V, E, P):
     return table.get(name)
 
 
-def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0),), plain_thresholds=(),
-                    credit_views=0, credit_ratio=1.0):
+def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0, 0.5),), plain_thresholds=(),
+                    credit_views=0, credit_ratio=1.0, prune_labels=(0.0, 0)):
     """Full training with densification on the ellipsoid scene (300k surface points, 100
     cameras 1297x840 f = 1100 on a radius-6 hemisphere, targets from a 2M-Gaussian ground
     truth, `iters` iterations, densify every 100 from 500 to 5000 with threshold 2e-4,
@@ -69,10 +69,13 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0),), plain_thres
       learned -- the scorer trained in the first run makes the decisions.
     credit_views > 0 labels the logged clones / splits by credit (the image share of the
     Gaussians they added at the outcome, over that many views, against credit_ratio times
-    the median Gaussian's share) instead of survival.  learned_runs: (threshold,
-    share_prune, share_prune_from) of each mode-2 run; share_prune > 0 also prunes, at the
-    events from iteration share_prune_from on, the Gaussians whose share of all views'
-    images is below that fraction of the median share.
+    the median Gaussian's share) instead of survival.  prune_labels = (bar, from): the
+    learn run also labels as prune the rows whose share of all views' images is below bar
+    x the median share when logged (events from iteration `from` on).  learned_runs:
+    (threshold, share_prune, share_prune_from, prune_threshold) of each mode-2 run: the
+    scorer's prune head applies from iteration share_prune_from on (threshold
+    prune_threshold), and share_prune > 0 also prunes, from then on, the Gaussians whose
+    share of all views' images is below that fraction of the median share (measured).
     Returns PSNR over all views, Gaussian counts and wall-clock iterations/s of each run."""
     import numpy as np
     from splatrs_amd import abi, scenes
@@ -98,14 +101,16 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0),), plain_thres
                position_lr_max_steps=iters, log_interval=500, seed=0)
     scorer = abi.Scorer(ctx, seed=0)
 
-    def run(mode, threshold=0.5, grad_threshold=0.0002, share_prune=0.0, share_prune_from=0):
+    def run(mode, threshold=0.5, grad_threshold=0.0002, share_prune=0.0, share_prune_from=0,
+            prune_threshold=0.5):
         tr = abi.Trainer(ctx, init)
         for c, v in zip(cams, views):
             tr.add_view(c, v)
         if mode:
             tr.attach_scorer(scorer, rec.learn_config(
                 mode=mode, samples_per_event=32768, max_events=64, outcome_delay=500,
-                eval_views=8, threshold=threshold, credit_views=credit_views,
+                eval_views=8, threshold=threshold, prune_threshold=prune_threshold,
+                credit_views=credit_views,
                 credit_ratio=credit_ratio, share_prune=share_prune,
                 share_prune_from=share_prune_from,
                 fit=rec.scorer_fit_config(epochs=2, batch=4096, lr=1e-3, seed=1)))
@@ -131,11 +136,13 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0),), plain_thres
         log(f"mode {mode} threshold {threshold} grad threshold {grad_threshold} share prune "
             f"{share_prune}: {out}")
         return out
-    plain = run(1)  # the reference's decisions; the scorer is trained on their outcomes
+    # the reference's decisions; the scorer is trained on their outcomes
+    plain = run(1, share_prune=prune_labels[0], share_prune_from=prune_labels[1])
     frontier = [dict(policy="learned", threshold=th,
-                     **({"share_prune": sp, "share_prune_from": sf} if sp else {}),
-                     **run(2, th, share_prune=sp, share_prune_from=sf))
-                for th, sp, sf in learned_runs]
+                     **({"share_prune": sp} if sp else {}),
+                     **({"prune_head_from": sf, "prune_threshold": pt} if sf else {}),
+                     **run(2, th, share_prune=sp, share_prune_from=sf, prune_threshold=pt))
+                for th, sp, sf, pt in learned_runs]
     frontier += [dict(policy="plain", grad_threshold=gt, **run(0, grad_threshold=gt))
                  for gt in plain_thresholds]
     learned = frontier[0]
@@ -146,14 +153,15 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0),), plain_thres
     pts = sorted([(plain["gaussians"], plain["psnr_db"])] +
                  [(r["gaussians"], r["psnr_db"]) for r in frontier if r["policy"] == "plain"])
     for r in frontier:
-        if r["policy"] != "learned" or "share_prune" in r:
+        if r["policy"] != "learned" or "share_prune" in r or "prune_head_from" in r:
             continue
         for (n0, p0), (n1, p1) in zip(pts, pts[1:]):
             if n0 <= r["gaussians"] <= n1 and n1 > n0:
                 at = p0 + (p1 - p0) * (r["gaussians"] - n0) / (n1 - n0)
                 r["plain_psnr_at_count_db"] = round(at, 3)
                 r["gain_at_matched_count_db"] = round(r["psnr_db"] - at, 3)
-    keys = ("policy", "threshold", "share_prune", "share_prune_from", "grad_threshold",
+    keys = ("policy", "threshold", "share_prune", "prune_head_from", "prune_threshold",
+            "grad_threshold",
             "psnr_db", "gaussians",
             "plain_psnr_at_count_db", "gain_at_matched_count_db")
     return {"workload": "300k points on 20 ellipsoids -> densify, 100 cams 1297x840, "
@@ -168,13 +176,19 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0),), plain_thres
 
 
 def learned_runs(spec):
-    """'threshold[:share_prune[:from]],...' -> [(threshold, share_prune, from), ...]."""
+    """'threshold[:share_prune[:from[:prune_threshold]]],...' -> [(threshold, share_prune,
+    from, prune_threshold), ...]."""
     out = []
     for item in spec.split(","):
         f = item.split(":")
         out.append((float(f[0]), float(f[1]) if len(f) > 1 else 0.0,
-                    int(f[2]) if len(f) > 2 else 0))
+                    int(f[2]) if len(f) > 2 else 0, float(f[3]) if len(f) > 3 else 0.5))
     return out
+
+
+def prune_labels(spec):
+    f = spec.split(":")
+    return float(f[0]), int(f[1]) if len(f) > 1 else 0
 
 
 def densify_bench(n, rounds, log=print):
@@ -341,10 +355,15 @@ def main():
     ap.add_argument("--learned-only", action="store_true",
                     help="only run the plain-vs-learned densification comparison")
     ap.add_argument("--learned-iters", type=int, default=7000)
-    ap.add_argument("--learned-runs", default="0.2:0.15:4500,0.35",
+    ap.add_argument("--learned-runs", default="0.2:0:4500:0.4,0.35:0:100000",
                     help="learned runs, comma separated, each threshold[:share_prune[:from "
-                         "iteration]]: the scorer's decision threshold and the image-share "
-                         "pruning bar (fraction of the median share; 0 or absent: off)")
+                         "iteration[:prune_threshold]]]: the scorer's clone / split "
+                         "threshold, the measured image-share pruning bar (fraction of the "
+                         "median share; 0: off), the iteration from which pruning (measured "
+                         "and by the scorer's prune head) applies, the prune head's threshold")
+    ap.add_argument("--prune-labels", default="0.15:2500",
+                    help="bar:from -- the learn run labels rows below bar x the median image "
+                         "share as prune (from that iteration; bar 0: the reference's prunes)")
     ap.add_argument("--plain-thresholds", default="0.000225,0.00025",
                     help="extra plain runs at these densify gradient thresholds")
     ap.add_argument("--credit-views", type=int, default=8,
@@ -373,7 +392,8 @@ def main():
                             log=lambda m: print(m, file=sys.stderr, flush=True),
                             learned_runs=learned_runs(args.learned_runs),
                             plain_thresholds=[float(x) for x in args.plain_thresholds.split(",") if x],
-                            credit_views=args.credit_views, credit_ratio=args.credit_ratio)
+                            credit_views=args.credit_views, credit_ratio=args.credit_ratio,
+                            prune_labels=prune_labels(args.prune_labels))
         print(json.dumps(r), flush=True)
         return
     if args.calibrate_copy:
@@ -566,7 +586,8 @@ def main():
                         ("learned", lambda: learned_compare(
                             args.learned_iters, err, learned_runs(args.learned_runs),
                             [float(x) for x in args.plain_thresholds.split(",") if x],
-                            args.credit_views, args.credit_ratio))):
+                            args.credit_views, args.credit_ratio,
+                            prune_labels(args.prune_labels)))):
             try:
                 out[key] = fn()
             except Exception as e:  # the headline line is printed regardless

@@ -725,11 +725,13 @@ typedef struct {
                                   of the Gaussians the decision added, at the outcome */
   float credit_ratio;          /* a clone / split is useful when its credit reaches this
                                   fraction of the median Gaussian's share at the outcome */
-  float share_prune;           /* mode 2, > 0: also prune, at each event from iteration
-                                  share_prune_from on, the Gaussians whose share of all
-                                  views' images is below this fraction of the median
-                                  Gaussian's */
-  uint32_t share_prune_from;
+  float share_prune;           /* > 0, mode 1: label as prune the logged rows whose share of
+                                  all views' images is below this fraction of the median
+                                  Gaussian's (events from share_prune_from on); mode 2:
+                                  prune those rows (a measured rule, events from
+                                  share_prune_from on) */
+  uint32_t share_prune_from;   /* mode 2: the scorer's prune head (and the rule above)
+                                  applies at the events from this iteration on */
   uint32_t pad_;
   splat_scorer_fit_config fit; /* each online update */
 } splat_learn_config;

@@ -601,6 +601,9 @@ DensifyParams densify_params(const splat_densify_config& cfg, float extent);
 // Densification decisions of one Gaussian as bits: clone, split, prune the row, prune a
 // copy of the row (the copy is not subject to the screen-size rule).
 constexpr uint32_t kDecClone = 1, kDecSplit = 2, kDecPrune = 4, kDecPruneCopy = 8;
+// Not a decision: the row's image share was below the learned prune head's bar when logged
+// (ignored by densification).
+constexpr uint32_t kDecLowShare = 16;
 constexpr uint32_t kNoSample = 0xffffffffu;
 
 // Per-Gaussian lineage carried through densification (the learned scorer's age feature and

@@ -188,7 +188,8 @@ __global__ void __launch_bounds__(256) credit_alive_kernel(
 }
 
 // Labels: clone / split only if the reference densified the row and a descendant is
-// alive; prune as the reference decided.  Densified samples carry the event's weight.
+// alive; prune where the reference pruned or the row's image share was low when logged.
+// Densified samples carry the event's weight.
 __global__ void __launch_bounds__(256) label_kernel(const uint8_t* __restrict__ s_dec,
                                                     const uint32_t* __restrict__ alive,
                                                     uint32_t b, uint32_t c, float wpos,
@@ -202,26 +203,29 @@ __global__ void __launch_bounds__(256) label_kernel(const uint8_t* __restrict__ 
   uint32_t y = 0;
   if ((d & kDecClone) && ok) y |= 1u;
   if ((d & kDecSplit) && ok) y |= 2u;
-  if (d & kDecPrune) y |= 4u;
+  if (d & (kDecPrune | kDecLowShare)) y |= 4u;
   s_label[b + k] = (uint8_t)y;
   s_weight[b + k] = (d & (kDecClone | kDecSplit)) ? wpos : 1.0f;
   if (y & 3u) atomicAdd(positives, 1u);
 }
 
-// Rows whose image share is below `bar` are pruned (and neither cloned nor split).
+// Rows whose image share is below `bar` are pruned (and neither cloned nor split), or
+// (mark_only) tagged kDecLowShare.
 __global__ void __launch_bounds__(256) share_prune_kernel(const float* __restrict__ share,
-                                                          uint32_t n, float bar,
+                                                          uint32_t n, float bar, int mark_only,
                                                           uint8_t* __restrict__ dec) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if (share[i] < bar) dec[i] = (uint8_t)((dec[i] & ~(kDecClone | kDecSplit)) | kDecPrune);
+  if (i >= n || !(share[i] < bar)) return;
+  dec[i] = mark_only ? (uint8_t)(dec[i] | kDecLowShare)
+                     : (uint8_t)((dec[i] & ~(kDecClone | kDecSplit)) | kDecPrune);
 }
 
 // Learned decisions from the logits; the reference's prune rules stay in force.
+// (The prune head is applied only where prune_head is set.)
 __global__ void __launch_bounds__(256) learned_decide_kernel(const float* __restrict__ logits,
                                                              const uint8_t* __restrict__ heur,
                                                              uint32_t n, float t_dens,
-                                                             float t_prune,
+                                                             float t_prune, int prune_head,
                                                              uint8_t* __restrict__ out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -229,7 +233,7 @@ __global__ void __launch_bounds__(256) learned_decide_kernel(const float* __rest
               zp = logits[3 * (size_t)i + 2];
   uint32_t d = heur[i] & (kDecPrune | kDecPruneCopy);
   if (zc > t_dens || zs > t_dens) d |= zc >= zs ? kDecClone : kDecSplit;
-  if (zp > t_prune) d |= kDecPrune | kDecPruneCopy;
+  if (prune_head && zp > t_prune) d |= kDecPrune | kDecPruneCopy;
   out[i] = (uint8_t)d;
 }
 
@@ -281,6 +285,22 @@ uint32_t scan_total(splat_trainer* t, const uint32_t* flag, uint32_t* pos, size_
 }
 }  // namespace
 
+namespace {
+// Image share of every row over all views, and `fraction` times its median.
+float all_view_share(splat_trainer* t, int32_t sh_degree, float fraction, float* share) {
+  const size_t n = t->n;
+  const uint32_t V = (uint32_t)t->views.size();
+  std::vector<uint32_t> list(V);
+  for (uint32_t k = 0; k < V; k++) list[k] = k;
+  trainer_image_share(t, list.data(), V, sh_degree, share);
+  std::vector<float> h(n);
+  SPLAT_HIP(hipMemcpyAsync(h.data(), share, n * 4, hipMemcpyDeviceToHost, t->ctx->stream));
+  SPLAT_HIP(hipStreamSynchronize(t->ctx->stream));
+  std::nth_element(h.begin(), h.begin() + n / 2, h.end());
+  return fraction * h[n / 2];
+}
+}  // namespace
+
 void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float extent,
                    uint64_t seed, uint32_t iteration, int32_t sh_degree, const float bg[3],
                    uint32_t last_reset, splat_densify_report* report) {
@@ -305,6 +325,13 @@ void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float exte
   float psnr = 0.0f;
   if (learn_mode && (L.used < L.capacity || due))
     psnr = trainer_eval_psnr(t, L.cfg.eval_views, sh_degree, bg);
+  if (learn_mode && L.used < L.capacity && L.cfg.share_prune > 0.0f &&
+      iteration >= L.cfg.share_prune_from) {  // prune-head labels: low image share now
+    float* share = (float*)t->store["learn_share"].reserve(n * 4 + 4);
+    const float bar = all_view_share(t, sh_degree, L.cfg.share_prune, share);
+    launch(ctx, "learn_low_share", share_prune_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+           (const float*)share, (uint32_t)n, bar, 1, heur);
+  }
   if (learn_mode && L.used < L.capacity) {
     uint32_t* flag = (uint32_t*)t->store["learn_flag"].reserve((n + 1) * 4);
     uint32_t* pos = (uint32_t*)t->store["learn_pos"].reserve((n + 1) * 4);
@@ -335,20 +362,12 @@ void learn_densify(splat_trainer* t, const splat_densify_config& cfg, float exte
     uint8_t* dec = (uint8_t*)t->store["learn_dec"].reserve(n + 4);
     launch(ctx, "learn_decide", learned_decide_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
            (const float*)logits, (const uint8_t*)heur, (uint32_t)n, logit(L.cfg.threshold),
-           logit(L.cfg.prune_threshold), dec);
-    if (L.cfg.share_prune > 0.0f && iteration >= L.cfg.share_prune_from) {  // every view
-      const uint32_t V = (uint32_t)t->views.size();
-      std::vector<uint32_t> list(V);
-      for (uint32_t k = 0; k < V; k++) list[k] = k;
+           logit(L.cfg.prune_threshold), iteration >= L.cfg.share_prune_from ? 1 : 0, dec);
+    if (L.cfg.share_prune > 0.0f && iteration >= L.cfg.share_prune_from) {  // measured rule
       float* share = (float*)t->store["learn_share"].reserve(n * 4 + 4);
-      trainer_image_share(t, list.data(), V, sh_degree, share);
-      std::vector<float> h(n);
-      SPLAT_HIP(hipMemcpyAsync(h.data(), share, n * 4, hipMemcpyDeviceToHost, ctx->stream));
-      SPLAT_HIP(hipStreamSynchronize(ctx->stream));
-      std::nth_element(h.begin(), h.begin() + n / 2, h.end());
-      const float bar = L.cfg.share_prune * h[n / 2];
+      const float bar = all_view_share(t, sh_degree, L.cfg.share_prune, share);
       launch(ctx, "learn_share_prune", share_prune_kernel, dim3(blocks_for(n, 256)), dim3(256),
-             0, (const float*)share, (uint32_t)n, bar, dec);
+             0, (const float*)share, (uint32_t)n, bar, 0, dec);
     }
     decide = dec;
   }

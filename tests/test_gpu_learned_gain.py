@@ -9,16 +9,16 @@ reference's schedule (src/train.rs:249-265).
 Mode 1 trains with the reference's decisions (src/densify.rs:125-221) and teaches the
 scorer: every logged clone / split is labelled by its credit, the share of the images of 8
 evenly spaced views (sum of alpha T over their pixels) that the Gaussians it added hold
-500 iterations later, against the median Gaussian's share.  Then:
+500 iterations later, against the median Gaussian's share; from iteration 2500 on, a
+logged row whose share of all views' images is below 0.15 of the median share is labelled
+prune.  Then, with the trained scorer:
 
-1. the scorer's clone decisions alone (threshold 0.35) against the plain heuristic at
-   gradient thresholds 2.25e-4 and 2.5e-4, whose final Gaussian counts bracket it: at its
-   Gaussian count its PSNR over all 100 views exceeds the plain heuristic's (linear between
-   the bracketing runs) by at least 0.05 dB (measured +0.101 dB: 36.238 dB at 492,070
-   Gaussians against 36.137 dB);
-2. the learned variant (threshold 0.2, and from iteration 4500 on the Gaussians whose share
-   of all views' images is below 0.15 of the median share pruned) ends with at most 0.8x
-   the plain run's Gaussians within 0.1 dB of its PSNR (measured 0.783x, -0.072 dB).
+1. its clone decisions alone (threshold 0.35, prune head off) against the plain heuristic
+   at gradient thresholds 2.25e-4 and 2.5e-4, whose final Gaussian counts bracket it: at
+   its Gaussian count its PSNR over all 100 views exceeds the plain heuristic's (linear
+   between the bracketing runs) by at least 0.05 dB;
+2. the learned variant (clone threshold 0.2, the prune head at 0.4 from iteration 4500 on)
+   ends with at most 0.8x the plain run's Gaussians within 0.1 dB of its PSNR.
 
 Every run is deterministic (integer atomics in the credit sums, fixed summation orders
 elsewhere).
@@ -62,14 +62,16 @@ def setup():
                position_lr_max_steps=7000, log_interval=0, seed=0)
     scorer = abi.Scorer(ctx, seed=0)
 
-    def run(mode, threshold=0.5, grad_threshold=0.0002, share_prune=0.0, share_prune_from=0):
+    def run(mode, threshold=0.5, grad_threshold=0.0002, share_prune=0.0, share_prune_from=0,
+            prune_threshold=0.5):
         tr = abi.Trainer(ctx, init)
         for c, v in zip(cams, views):
             tr.add_view(c, v)
         if mode:
             tr.attach_scorer(scorer, rec.learn_config(
                 mode=mode, samples_per_event=32768, max_events=64, outcome_delay=500,
-                eval_views=8, threshold=threshold, credit_views=8, credit_ratio=1.0,
+                eval_views=8, threshold=threshold, prune_threshold=prune_threshold,
+                credit_views=8, credit_ratio=1.0,
                 share_prune=share_prune, share_prune_from=share_prune_from,
                 fit=rec.scorer_fit_config(epochs=2, batch=4096, lr=1e-3, seed=1)))
         tr.train(extent, abi.TrainConfig.default(**dict(cfg, densify_grad_threshold=grad_threshold)))
@@ -79,17 +81,19 @@ def setup():
         n = int(tr.info()[0])
         tr.close()
         say(f"mode {mode} threshold {threshold} grad threshold {grad_threshold} share prune "
-            f"{share_prune} from {share_prune_from}: {psnr:.3f} dB, {n} Gaussians")
+            f"{share_prune} from {share_prune_from} prune threshold {prune_threshold}: "
+            f"{psnr:.3f} dB, {n} Gaussians")
         return psnr, n
 
-    plain = run(1)  # the reference's decisions, labelled by credit: trains the scorer
+    # the reference's decisions, labelled by credit and image share: trains the scorer
+    plain = run(1, share_prune=0.15, share_prune_from=2500)
     yield run, plain
     ctx.close()
 
 
 def test_scorer_beats_the_plain_frontier_at_matched_count(setup):
     run, _ = setup
-    p_l, n_l = run(2, threshold=0.35)
+    p_l, n_l = run(2, threshold=0.35, share_prune_from=100000)  # prune head off
     p_hi, n_hi = run(0, grad_threshold=0.000225)
     p_lo, n_lo = run(0, grad_threshold=0.00025)
     assert n_lo <= n_l <= n_hi, (n_lo, n_l, n_hi)
@@ -101,7 +105,7 @@ def test_scorer_beats_the_plain_frontier_at_matched_count(setup):
 
 def test_learned_variant_reaches_plain_psnr_with_fewer_gaussians(setup):
     run, (p_plain, n_plain) = setup
-    p, n = run(2, threshold=0.2, share_prune=0.15, share_prune_from=4500)
+    p, n = run(2, threshold=0.2, share_prune_from=4500, prune_threshold=0.4)
     say(f"learned variant: {n / n_plain:.3f}x the Gaussians, {p - p_plain:+.3f} dB")
     assert n <= 0.8 * n_plain, (n, n_plain)
     assert p >= p_plain - 0.1, (p, p_plain)
