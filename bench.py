@@ -140,7 +140,8 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0, 0.5),), plain_
     plain = run(1, share_prune=prune_labels[0], share_prune_from=prune_labels[1])
     frontier = [dict(policy="learned", threshold=th,
                      **({"share_prune": sp} if sp else {}),
-                     **({"prune_head_from": sf, "prune_threshold": pt} if sf else {}),
+                     **({"prune_head_from": sf, "prune_threshold": pt} if 0 < sf < iters
+                        else {"prune_head": "off"} if sf >= iters else {}),
                      **run(2, th, share_prune=sp, share_prune_from=sf, prune_threshold=pt))
                 for th, sp, sf, pt in learned_runs]
     frontier += [dict(policy="plain", grad_threshold=gt, **run(0, grad_threshold=gt))
@@ -154,6 +155,8 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0, 0.5),), plain_
                  [(r["gaussians"], r["psnr_db"]) for r in frontier if r["policy"] == "plain"])
     for r in frontier:
         if r["policy"] != "learned" or "share_prune" in r or "prune_head_from" in r:
+            continue  # only runs whose decisions differ from the plain ones in clones alone
+        if "prune_head" not in r and prune_labels[0] > 0:
             continue
         for (n0, p0), (n1, p1) in zip(pts, pts[1:]):
             if n0 <= r["gaussians"] <= n1 and n1 > n0:
@@ -161,7 +164,7 @@ def learned_compare(iters, log=print, learned_runs=((0.2, 0.0, 0, 0.5),), plain_
                 r["plain_psnr_at_count_db"] = round(at, 3)
                 r["gain_at_matched_count_db"] = round(r["psnr_db"] - at, 3)
     keys = ("policy", "threshold", "share_prune", "prune_head_from", "prune_threshold",
-            "grad_threshold",
+            "prune_head", "grad_threshold",
             "psnr_db", "gaussians",
             "plain_psnr_at_count_db", "gain_at_matched_count_db")
     return {"workload": "300k points on 20 ellipsoids -> densify, 100 cams 1297x840, "
