@@ -38,6 +38,16 @@ __device__ __forceinline__ uint32_t item_count(uint32_t n, const uint32_t* n_dev
   return n_dev ? *n_dev : n;
 }
 
+// The logical block of workgroup w: workgroup w runs on die w % 8, and die x takes the
+// contiguous range of logical blocks [x s + min(x, r), ...) of s + (x < r) blocks (s =
+// nb / 8, r = nb % 8) in order, so that neighbouring blocks -- whose digit counts and
+// whose output runs of each digit are adjacent in memory -- complete their cache lines in
+// one die's L2.  A bijection of [0, nb): die x runs exactly s + (x < r) workgroups.
+__device__ __forceinline__ uint32_t die_block(uint32_t w, uint32_t nb) {
+  const uint32_t x = w % 8u, i = w / 8u, s = nb / 8u, r = nb % 8u;
+  return x * s + min(x, r) + i;
+}
+
 template <class F, int... I>
 __device__ __forceinline__ void rounds_impl(F&& f, std::integer_sequence<int, I...>) {
   (f(std::integral_constant<int, I>{}), ...);
@@ -59,14 +69,15 @@ __global__ void __launch_bounds__(kRadixThreads) radix_upsweep_kernel(
   n = item_count(n, n_dev);
   for (uint32_t d = threadIdx.x; d < kRadixBins; d += kRadixThreads) hist[d] = 0;
   __syncthreads();
-  const uint32_t base = blockIdx.x * (kRadixThreads * ROUNDS);
+  const uint32_t blk = die_block(blockIdx.x, nblocks);
+  const uint32_t base = blk * (kRadixThreads * ROUNDS);
   for (uint32_t r = 0; r < ROUNDS; r++) {
     const uint32_t i = base + r * kRadixThreads + threadIdx.x;
     if (i < n) atomicAdd(&hist[(keys[i] >> shift) & mask], 1u);
   }
   __syncthreads();
   for (uint32_t d = threadIdx.x; d <= mask; d += kRadixThreads)
-    counts[d * nblocks + blockIdx.x] = hist[d];
+    counts[d * nblocks + blk] = hist[d];
 }
 
 template <bool TWO, uint32_t ROUNDS>
@@ -85,7 +96,8 @@ __global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
   __shared__ uint32_t wsum[kRadixWaves];
   if (status && status->abort) return;  // uniform
   n = item_count(n, n_dev);
-  const uint32_t base = blockIdx.x * kItems;
+  const uint32_t blk = die_block(blockIdx.x, nblocks);
+  const uint32_t base = blk * kItems;
   if (base >= n) return;  // uniform: the block holds no item
   const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   const uint32_t mask = (1u << nbits) - 1u;
@@ -107,8 +119,8 @@ __global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
 #pragma unroll
   for (uint32_t h = 0; h < 2; h++) {
     const uint32_t d = 2 * t + h;
-    g0[h] = d < bins ? offsets[d * nblocks + blockIdx.x] : 0u;
-    c[h] = d < bins ? offsets[d * nblocks + blockIdx.x + 1] - g0[h] : 0u;
+    g0[h] = d < bins ? offsets[d * nblocks + blk] : 0u;
+    c[h] = d < bins ? offsets[d * nblocks + blk + 1] - g0[h] : 0u;
   }
   const uint32_t pair = c[0] + c[1];
   uint32_t x = pair;
