@@ -309,6 +309,16 @@ __device__ __forceinline__ uint32_t block_inclusive_scan(uint32_t v, uint32_t* s
   return v;
 }
 
+// The logical block of workgroup w of a launch of nb workgroups: workgroup w runs on die
+// w % 8 (each die has its own L2), and die x takes the contiguous range of logical blocks
+// [x s + min(x, r), ...) of s + (x < r) blocks (s = nb / 8, r = nb % 8) in order, so that
+// neighbouring logical blocks, which share or complete each other's cache lines, meet in
+// one L2.  A bijection of [0, nb): die x runs exactly s + (x < r) workgroups.
+__device__ __forceinline__ uint32_t die_block(uint32_t w, uint32_t nb) {
+  const uint32_t x = w % 8u, i = w / 8u, s = nb / 8u, r = nb % 8u;
+  return x * s + min(x, r) + i;
+}
+
 // Learning rate of parameter plane j (LearningRates, src/adam.rs:11-26).
 __device__ __forceinline__ float plane_lr(int j, const splat_learning_rates& lr) {
   if (j < 3) return lr.mean;

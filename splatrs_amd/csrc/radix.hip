@@ -38,15 +38,6 @@ __device__ __forceinline__ uint32_t item_count(uint32_t n, const uint32_t* n_dev
   return n_dev ? *n_dev : n;
 }
 
-// The logical block of workgroup w: workgroup w runs on die w % 8, and die x takes the
-// contiguous range of logical blocks [x s + min(x, r), ...) of s + (x < r) blocks (s =
-// nb / 8, r = nb % 8) in order, so that neighbouring blocks -- whose digit counts and
-// whose output runs of each digit are adjacent in memory -- complete their cache lines in
-// one die's L2.  A bijection of [0, nb): die x runs exactly s + (x < r) workgroups.
-__device__ __forceinline__ uint32_t die_block(uint32_t w, uint32_t nb) {
-  const uint32_t x = w % 8u, i = w / 8u, s = nb / 8u, r = nb % 8u;
-  return x * s + min(x, r) + i;
-}
 
 template <class F, int... I>
 __device__ __forceinline__ void rounds_impl(F&& f, std::integer_sequence<int, I...>) {
@@ -69,6 +60,7 @@ __global__ void __launch_bounds__(kRadixThreads) radix_upsweep_kernel(
   n = item_count(n, n_dev);
   for (uint32_t d = threadIdx.x; d < kRadixBins; d += kRadixThreads) hist[d] = 0;
   __syncthreads();
+  // neighbouring blocks' digit counts are adjacent words: their lines fill in one L2
   const uint32_t blk = die_block(blockIdx.x, nblocks);
   const uint32_t base = blk * (kRadixThreads * ROUNDS);
   for (uint32_t r = 0; r < ROUNDS; r++) {
@@ -96,6 +88,7 @@ __global__ void __launch_bounds__(kRadixThreads) radix_downsweep_kernel(
   __shared__ uint32_t wsum[kRadixWaves];
   if (status && status->abort) return;  // uniform
   n = item_count(n, n_dev);
+  // neighbouring blocks' output runs of each digit are adjacent: their lines fill in one L2
   const uint32_t blk = die_block(blockIdx.x, nblocks);
   const uint32_t base = blk * kItems;
   if (base >= n) return;  // uniform: the block holds no item
