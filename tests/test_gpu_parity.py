@@ -358,6 +358,20 @@ def radix_cases():
              scenes.small_scene_cameras(size=4400, f=5200.0)[3], rec.proj_opts(1))]
 
 
+def test_counting_binning_at_and_beyond_the_tile_plan_limit(ctx):
+    """The counting binning of views of 128 x 128 tiles (the largest that one workgroup
+    plans: counter offsets, tile starts and dispatch order), 129 x 128 and 275 x 275 (the
+    three-kernel path) gives the reference's order, bitwise."""
+    g = scenes.small_scene(2, 100_000)
+    for size, (w, h) in (("2048x2048", (2048, 2048)), ("2064x2048", (2064, 2048)),
+                         ("4400x4400", (4400, 4400))):
+        f = 2400.0 * w / 2048.0
+        cam = scenes.look_at(w, h, f, f, (0.4, -1.2, -3.7), (0.0, 0.0, 0.0))
+        say(f"counting binning: 100k Gaussians {size}")
+        pg = ctx.project_all(g, cam, rec.proj_opts(1))
+        assert_binning_equal(ctx.bin(pg, w, h), O.bin(O.project_all(g, cam, rec.proj_opts(1)), w, h))
+
+
 def test_radix_binning_equals_oracle(ctx):
     """The radix binning (rows by depth, then entries by tile, both stable) gives the
     reference's (tile, depth, entry) order, bitwise, from tiny to crowded to full size."""
