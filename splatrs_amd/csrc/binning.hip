@@ -956,12 +956,17 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
   // small[3], in one launch (the scan writes only small[0] and total64)
   fill3_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, bb.tile_count, kTileSub * T, small + 1,
             3, 0u);
-  exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64);
-  bin_guard(ctx, total64, capacity, status);
-  // The entry kernels cover the capacity and read the step's count from `status`.
-  launch(ctx, "bin_marks", block_marks_kernel<>, dim3(blocks_for(n, 256)), dim3(256), 0,
-         (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
-         (const StepStatus*)status);
+  // the capacity guard and the keygen marks ride on the scan's last pass when it has one
+  const ScanTail tail{bb.block_marks, (uint32_t)(capacity / kKeygenEntries + 2), kKeygenEntries,
+                      (unsigned long long)capacity, total64, status};
+  if (!exclusive_scan_u32(ctx, pb.ntiles, bb.entry_offset, n, small, scan_sums, total64,
+                          bd ? nullptr : &tail)) {
+    bin_guard(ctx, total64, capacity, status);
+    // The entry kernels cover the capacity and read the step's count from `status`.
+    launch(ctx, "bin_marks", block_marks_kernel<>, dim3(blocks_for(n, 256)), dim3(256), 0,
+           (const uint32_t*)bb.entry_offset, (uint32_t)n, bb.block_marks,
+           (const StepStatus*)status);
+  }
   if (bd)
     launch(ctx, "bin_keygen", keygen_batch_kernel, dim3(blocks_for(capacity, kKeygenEntries)),
            dim3(kKeygenThreads), 0, (const uint32_t*)bb.entry_offset, (uint32_t)n, *bd,

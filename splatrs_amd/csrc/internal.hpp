@@ -441,9 +441,20 @@ void planes_to_aos(splat_ctx* ctx, const float* planes, size_t n, size_t stride,
 // `sums` (scan_temp_bytes(n) bytes) defaults to a context scratch buffer; graph-captured
 // code passes its own so the captured pointer stays valid.  `total64` (optional, zeroed by
 // the caller) receives the exact 64-bit sum of the input.
-void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
+// Extra work of the last pass of the entry-offset scan of a graph-captured step: the
+// capacity guard (total64 against `capacity`, into `status`) and the keygen marks.
+struct ScanTail {
+  uint32_t* marks;
+  uint32_t marks_cap, epb;
+  unsigned long long capacity;
+  const unsigned long long* total64;
+  StepStatus* status;
+};
+// Returns whether `tail` was done (only by the multi-pass scan, n > scan_single's limit;
+// the caller does it otherwise).
+bool exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
                         uint32_t* total_dev, uint32_t* sums = nullptr,
-                        unsigned long long* total64 = nullptr);
+                        unsigned long long* total64 = nullptr, const ScanTail* tail = nullptr);
 size_t scan_temp_bytes(size_t n);
 size_t scan_max_items();  // largest n exclusive_scan_u32 takes
 // Fill kernel (used instead of hipMemset so the same code can sit inside a HIP graph).

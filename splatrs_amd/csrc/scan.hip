@@ -105,11 +105,17 @@ __global__ void __launch_bounds__(kTopThreads) scan_top_kernel(uint32_t* __restr
   if (threadIdx.x == 0 && total_out) *total_out = total;
 }
 
+// TAIL (the entry offsets of a binning, `tail` set): also the capacity guard of a
+// graph-captured step (block 0: the 64-bit total against the entry capacity, the sticky
+// abort flag) and the keygen marks: marks[b] = the item (Gaussian) whose range holds entry
+// b * epb, for b < marks_cap (the marks of an aborted step are never read).
+template <bool TAIL>
 __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const uint32_t* __restrict__ in,
                                                                   size_t n,
                                                                   const uint32_t* __restrict__ block_offsets,
                                                                   uint32_t* __restrict__ out,
-                                                                  const uint32_t* __restrict__ total) {
+                                                                  const uint32_t* __restrict__ total,
+                                                                  ScanTail tail) {
   __shared__ uint32_t sh[kScanThreads / 64 + 1];
   const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
   uint32_t vals[kScanItems];
@@ -124,10 +130,24 @@ __global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const uint32_t
   uint32_t run = block_offsets[blockIdx.x] + incl - s;
 #pragma unroll
   for (int k = 0; k < kScanItems; k++) {
-    if (base + k < n) out[base + k] = run;
+    if (base + k < n) {
+      out[base + k] = run;
+      if (TAIL)
+        for (uint32_t b = (uint32_t)(((uint64_t)run + tail.epb - 1) / tail.epb);
+             (uint64_t)b * tail.epb < (uint64_t)run + vals[k] && b < tail.marks_cap; b++)
+          tail.marks[b] = (uint32_t)(base + k);
+    }
     run += vals[k];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = *total;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[n] = *total;
+    if (TAIL) {
+      const unsigned long long t = *tail.total64;
+      tail.status->total = t;
+      tail.status->entries = t > 0xffffffffull ? 0xffffffffu : (uint32_t)t;
+      if (t > tail.capacity) tail.status->abort = 1u;
+    }
+  }
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
@@ -165,12 +185,13 @@ size_t scan_temp_bytes(size_t n) {
   return (blocks_for(n == 0 ? 1 : n, kScanTile) + 1) * sizeof(uint32_t);
 }
 
-void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
-                        uint32_t* total_dev, uint32_t* sums, unsigned long long* total64) {
+bool exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_t n,
+                        uint32_t* total_dev, uint32_t* sums, unsigned long long* total64,
+                        const ScanTail* tail) {
   if (n <= (size_t)kTopThreads * kTopItems) {
     launch(ctx, "scan_single", scan_single_kernel, dim3(1), dim3(kTopThreads), 0, in,
            (uint32_t)n, out, total_dev, total64);
-    return;
+    return false;
   }
   const unsigned nb = blocks_for(n, kScanTile);
   SPLAT_REQUIRE(nb <= (unsigned)(kTopThreads * kTopItems), "scan: input too large");
@@ -178,8 +199,14 @@ void exclusive_scan_u32(splat_ctx* ctx, const uint32_t* in, uint32_t* out, size_
   launch(ctx, "scan_reduce", scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, in, n, sums,
          total64);
   launch(ctx, "scan_top", scan_top_kernel, dim3(1), dim3(kTopThreads), 0, sums, nb, total_dev);
-  launch(ctx, "scan_apply", scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, in, n,
-         (const uint32_t*)sums, out, (const uint32_t*)total_dev);
+  if (tail) {
+    launch(ctx, "scan_apply", scan_apply_kernel<true>, dim3(nb), dim3(kScanThreads), 0, in, n,
+           (const uint32_t*)sums, out, (const uint32_t*)total_dev, *tail);
+    return true;
+  }
+  launch(ctx, "scan_apply", scan_apply_kernel<false>, dim3(nb), dim3(kScanThreads), 0, in, n,
+         (const uint32_t*)sums, out, (const uint32_t*)total_dev, ScanTail{});
+  return false;
 }
 
 }  // namespace splat
