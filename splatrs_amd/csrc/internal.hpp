@@ -603,6 +603,11 @@ void loss_forward_backward_batch(splat_ctx* ctx, const float* pred, const BatchD
 // (`out_dev`, or dyn->loss_out when `dyn` is given).
 void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
                    float lambda, float* out_dev, const StepDyn* dyn = nullptr);
+// A train step's last launch: its loss value (to dyn->loss_out, if set) and its report
+// (reports[dyn->report_slot]).
+void loss_value_and_report(splat_ctx* ctx, const float* scratch, uint32_t width,
+                           uint32_t height, float lambda, const StepDyn* dyn,
+                           const StepStatus* status, StepReport* reports);
 void blur_image(splat_ctx* ctx, const float* src, uint32_t width, uint32_t height,
                 uint32_t channels, float* dst);
 void gaussian_kernel_host(float k[11]);

@@ -420,6 +420,32 @@ __global__ void __launch_bounds__(256) loss_value_kernel(const float2* __restric
   *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
 }
 
+// The last kernel of a train step: the loss value (when dyn->loss_out is set; the partial
+// sums are still those of the step's loss pass) and the step's host-visible report.
+__global__ void __launch_bounds__(256) loss_value_report_kernel(
+    const float2* __restrict__ partial, uint32_t nb, float lambda, float n,
+    const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status,
+    StepReport* __restrict__ reports) {
+  float* out = dyn->loss_out;
+  if (out) {
+    double a, b;
+    sum_partials(partial, nb, &a, &b);
+    if (threadIdx.x == 0) {
+      const float inv_n = 1.0f / n;
+      const float l1 = (float)b * inv_n;
+      const float ssim = (float)a / n;
+      *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
+    }
+  }
+  if (threadIdx.x == 0) {
+    StepReport r;
+    r.seq = dyn->seq;
+    r.abort = status->abort;
+    r.total = status->total;
+    reports[dyn->report_slot] = r;
+  }
+}
+
 // Per-pair loss values of a batched step: one workgroup per pair.
 __global__ void __launch_bounds__(256) loss_values_batch_kernel(const float2* __restrict__ partial,
                                                                 BatchDev bd, float lambda,
@@ -469,6 +495,16 @@ void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_
   const unsigned nb = blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
   launch(ctx, "loss_value", loss_value_kernel, dim3(1), dim3(256), 0,
          reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, out_dev, dyn);
+}
+
+void loss_value_and_report(splat_ctx* ctx, const float* scratch, uint32_t width,
+                           uint32_t height, float lambda, const StepDyn* dyn,
+                           const StepStatus* status, StepReport* reports) {
+  const size_t n = (size_t)width * height * 3;
+  const unsigned nb = blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
+  launch(ctx, "step_report", loss_value_report_kernel, dim3(1), dim3(256), 0,
+         reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, dyn, status,
+         reports);
 }
 
 // Both passes and the loss value on caller-provided device buffers, with `scratch` of

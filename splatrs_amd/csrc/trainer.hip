@@ -143,17 +143,6 @@ __global__ void set_dyn_kernel(StepDyn d, StepDyn* __restrict__ out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *out = d;
 }
 
-__global__ void step_report_kernel(const StepDyn* __restrict__ dyn,
-                                   const StepStatus* __restrict__ status,
-                                   StepReport* __restrict__ reports) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  StepReport r;
-  r.seq = dyn->seq;
-  r.abort = status->abort;
-  r.total = status->total;
-  reports[dyn->report_slot] = r;
-}
-
 }  // namespace
 
 namespace splat {
@@ -285,7 +274,6 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
                  nullptr, P.counter);
   loss_forward_backward(ctx, P.image, nullptr, any_target, P.width, P.height, P.lambda, true,
                         P.dimage, P.loss_scratch, nullptr, P.dyn);
-  loss_value_to(ctx, P.loss_scratch, P.width, P.height, P.lambda, nullptr, P.dyn);
   raster_backward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.final_t, P.n_contrib, P.dimage,
                   P.ordered != 0, P.slots, P.accumulate && P.absgrad, bv, nullptr, P.counter);
   StatsArgs sa{};
@@ -306,8 +294,9 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   std::memset(&lr0, 0, sizeof(lr0));
   grad_adam(ctx, P.params, P.m, P.v, n, P.stride, cam0, opts, P.bb.entry_offset, P.slots, sa,
             1.0f, 1.0f, ast, lr0, P.dyn, P.status);
-  launch(ctx, "step_report", step_report_kernel, dim3(1), dim3(64), 0, (const StepDyn*)P.dyn,
-         (const StepStatus*)P.status, P.reports);
+  // the loss value from the loss pass's partial sums (untouched since), and the report
+  loss_value_and_report(ctx, P.loss_scratch, P.width, P.height, P.lambda, P.dyn, P.status,
+                        P.reports);
 }
 
 hipGraphExec_t graph_for(splat_trainer* t, const StepPlan& P, const uint8_t* any_target) {
