@@ -952,10 +952,6 @@ void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers
                       uint32_t* big_list, uint32_t* scan_sums, StepStatus* status,
                       const BatchDev* bd) {
   const size_t T = (size_t)bb.tiles_x * bb.tiles_y;
-  // the scan's 64-bit total, the tile counters and the queued-tile counts small[1] and
-  // small[3], in one launch (the scan writes only small[0] and total64)
-  fill3_u32(ctx, reinterpret_cast<uint32_t*>(total64), 2, bb.tile_count, kTileSub * T, small + 1,
-            3, 0u);
   // the capacity guard and the keygen marks ride on the scan's last pass when it has one
   const ScanTail tail{bb.block_marks, (uint32_t)(capacity / kKeygenEntries + 2), kKeygenEntries,
                       (unsigned long long)capacity, total64, status};

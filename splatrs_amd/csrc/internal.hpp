@@ -472,6 +472,8 @@ size_t bin_build(splat_ctx* ctx, ProjBuffers pb, size_t n, uint32_t tiles_x, uin
 // are the projections of a batched step and bb.tiles_x * bb.tiles_y is the global tile
 // count (bb.tiles_y = 1).  The entry count goes
 // to `status`; a count above the capacity sets the abort flag and the entry kernels exit.
+// The caller clears, before it, the words the pass accumulates into: total64 (2 words),
+// bb.tile_count (kTileSub x tiles) and small[1..3].
 void bin_build_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinBuffers& bb,
                       size_t capacity, uint32_t* small, unsigned long long* total64,
                       uint32_t* big_list, uint32_t* scan_sums, StepStatus* status,

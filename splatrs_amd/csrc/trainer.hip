@@ -139,8 +139,18 @@ __global__ void __launch_bounds__(256) sq_err_kernel(const float* __restrict__ i
   if (threadIdx.x == 0) partial[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-__global__ void set_dyn_kernel(StepDyn d, StepDyn* __restrict__ out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = d;
+// Before each replay of a graph-captured step (kernel arguments are frozen in a graph):
+// the step's per-iteration values, and the counters its binning accumulates into cleared --
+// the 64-bit entry total (2 words), the ntc words of tile counter sets and the queued-tile
+// counts small[1..3].
+__global__ void set_dyn_kernel(StepDyn d, StepDyn* __restrict__ out, uint32_t* __restrict__ total64,
+                               uint32_t* __restrict__ tile_count, uint32_t ntc,
+                               uint32_t* __restrict__ small) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *out = d;
+  if (i < 2) total64[i] = 0u;
+  if (i < 3) small[1 + i] = 0u;
+  if (i < ntc) tile_count[i] = 0u;
 }
 
 }  // namespace
@@ -367,7 +377,10 @@ void launch_graph_step(splat_trainer* t, uint32_t view, const splat_step_params&
   d.c2 = adam_c(0.999f, adam_step);
   d.seq = (uint32_t)t->seq;
   d.report_slot = (uint32_t)(t->seq % kReportSlots);
-  launch(ctx, "step_args", set_dyn_kernel, dim3(1), dim3(64), 0, d, P.dyn);
+  const uint32_t ntc = kTileSub * P.bb.tiles_x * P.bb.tiles_y;
+  launch(ctx, "step_args", set_dyn_kernel, dim3(blocks_for(std::max<uint32_t>(ntc, 3u), 256)),
+         dim3(256), 0, d, P.dyn, reinterpret_cast<uint32_t*>(P.total64), P.bb.tile_count, ntc,
+         P.small);
   SPLAT_HIP(hipGraphLaunch(exec, ctx->stream));
   t->graph_replays++;
   hipEvent_t ev;
