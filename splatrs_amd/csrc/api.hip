@@ -550,7 +550,7 @@ int splat_densify_and_prune(splat_ctx* ctx, const float* params, const float* m,
     float* mo = ctx->buf<float>("dn_mo", ostride * kParams);
     float* vo = ctx->buf<float>("dn_vo", ostride * kParams);
     const size_t k = densify_planes(ctx, dp, dm, dv, n, stride, dacc, dden, dmr, *cfg, extent,
-                                    dnorm, normals_len, po, mo, vo, cap_rows, ostride, report);
+                                    dnorm, normals_len, 0, po, mo, vo, cap_rows, ostride, report);
     if (k == (size_t)-1) fail(SPLAT_ERR_CAPACITY, "not enough normal draws for the split children");
     *n_out = k;
     if (k > out_capacity) fail(SPLAT_ERR_CAPACITY, "output capacity too small");

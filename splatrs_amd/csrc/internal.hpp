@@ -649,18 +649,37 @@ struct DensifyLineage {
 void densify_decide(splat_ctx* ctx, const float* params, size_t n, size_t stride,
                     const float* grad_accum, const uint32_t* denom, const uint32_t* max_radii,
                     const splat_densify_config& cfg, float extent, uint8_t* out);
+// Counter-based standard normal number i of stream `seed`: SplitMix64 of (seed, 2i) and
+// (seed, 2i + 1) give two uniform doubles, Box-Muller as the reference's normal_sample
+// (src/densify.rs:116-121, u1 kept >= 1e-12).
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ float normal_draw(uint64_t seed, uint64_t i) {
+  const uint64_t a = splitmix64(seed ^ splitmix64(2 * i));
+  const uint64_t b = splitmix64(seed ^ splitmix64(2 * i + 1));
+  double u1 = (double)(a >> 11) * 0x1.0p-53;
+  const double u2 = (double)(b >> 11) * 0x1.0p-53;
+  u1 = u1 > 1e-12 ? u1 : 1e-12;
+  return (float)(sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2));
+}
+
 // `decide` (optional) replaces the reference's clone/split/prune decisions; `lineage`
-// (optional) carries birth/origin planes.
+// (optional) carries birth/origin planes.  The split children's offsets use the standard
+// normals `normals_dev` (3 per child, in split-row order), or, when it is null, the
+// draws normal_draw(normals_seed, i) for the same indices i.
 size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
                       size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
                       const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
-                      const float* normals_dev, size_t normals_len, float* po, float* mo,
-                      float* vo, size_t out_capacity, size_t out_stride,
+                      const float* normals_dev, size_t normals_len, uint64_t normals_seed,
+                      float* po, float* mo, float* vo, size_t out_capacity, size_t out_stride,
                       splat_densify_report* report, const uint8_t* decide = nullptr,
                       const DensifyLineage* lineage = nullptr);
 void reset_opacity_planes(splat_ctx* ctx, float* params, float* m, float* v, size_t n,
                           size_t stride, float limit);
-void normals_fill(splat_ctx* ctx, float* out, size_t count, uint64_t seed);
 void stats_accumulate_dev(splat_ctx* ctx, const uint32_t* radius,
                           const splat_proj_grad* grads, const float* abs_grads, size_t n,
                           uint32_t width, uint32_t height, float* grad_accum, uint32_t* denom,

@@ -158,55 +158,88 @@ __global__ void __launch_bounds__(256) densify_flags_kernel(const float* __restr
   clone[i] = is_clone ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(256) densify_scatter_kernel(
+// Kept originals (parameters and both moments) and kept clones (parameters, zero moments)
+// to their output rows, kDensifyPlanes planes per workgroup row (blockIdx.y): each lane
+// has all of its loads for a row in flight before it stores, and a wave's accesses stay
+// within a few planes at a time.  Group 0 also writes the lineage of those rows.
+constexpr int kDensifyPlanes = 3;
+constexpr int kDensifyGroups = (kParams + kDensifyPlanes - 1) / kDensifyPlanes;
+
+__global__ void __launch_bounds__(256) densify_copy_kernel(
     const float* __restrict__ params, const float* __restrict__ m, const float* __restrict__ v,
     uint32_t n, uint32_t stride, const uint32_t* __restrict__ keep_orig,
-    const uint32_t* __restrict__ keep_clone, const uint32_t* __restrict__ split,
-    const uint32_t* __restrict__ keep_child, const uint32_t* __restrict__ pos_orig,
-    const uint32_t* __restrict__ pos_clone, const uint32_t* __restrict__ pos_split,
-    const uint32_t* __restrict__ pos_child, uint32_t n_orig_kept, uint32_t n_clone_kept,
-    DensifyParams dp, const float* __restrict__ normals, float* __restrict__ po,
+    const uint32_t* __restrict__ keep_clone, const uint32_t* __restrict__ pos_orig,
+    const uint32_t* __restrict__ pos_clone, uint32_t n_orig_kept, float* __restrict__ po,
     float* __restrict__ mo, float* __restrict__ vo, uint32_t out_stride, DensifyLineage ln) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  // lineage: kept rows keep theirs; new rows are born now and descend from the logged
-  // decision on their parent (or from the parent's own origin when it was not logged)
-  const uint32_t new_origin =
-      ln.birth_out ? (ln.sample_of && ln.sample_of[i] != kNoSample ? ln.sample_of[i]
-                                                                   : ln.origin_in[i])
-                   : 0u;
-  if (keep_orig[i] && ln.birth_out) {
-    ln.birth_out[pos_orig[i]] = ln.birth_in[i];
-    ln.origin_out[pos_orig[i]] = ln.origin_in[i];
+  const bool ko = keep_orig[i] != 0, kc = keep_clone[i] != 0;
+  if (!ko && !kc) return;
+  const uint32_t oo = ko ? pos_orig[i] : 0u;
+  const uint32_t oc = kc ? n_orig_kept + pos_clone[i] : 0u;
+  if (blockIdx.y == 0 && ln.birth_out) {
+    // lineage: kept rows keep theirs; a clone is born now and descends from the logged
+    // decision on its parent (or from the parent's own origin when it was not logged)
+    if (ko) {
+      ln.birth_out[oo] = ln.birth_in[i];
+      ln.origin_out[oo] = ln.origin_in[i];
+    }
+    if (kc) {
+      ln.birth_out[oc] = ln.iteration;
+      ln.origin_out[oc] =
+          ln.sample_of && ln.sample_of[i] != kNoSample ? ln.sample_of[i] : ln.origin_in[i];
+    }
   }
-  if (keep_clone[i] && ln.birth_out) {
-    ln.birth_out[n_orig_kept + pos_clone[i]] = ln.iteration;
-    ln.origin_out[n_orig_kept + pos_clone[i]] = new_origin;
+  const int j0 = (int)blockIdx.y * kDensifyPlanes;
+  float tp[kDensifyPlanes], tm[kDensifyPlanes], tv[kDensifyPlanes];
+#pragma unroll
+  for (int c = 0; c < kDensifyPlanes; c++) {
+    const size_t q = (size_t)(j0 + c) * stride + i;
+    if (j0 + c < kParams) {
+      tp[c] = params[q];
+      if (ko) {
+        tm[c] = m[q];
+        tv[c] = v[q];
+      }
+    }
   }
-  if (split[i] && keep_child[i] && ln.birth_out) {
+#pragma unroll
+  for (int c = 0; c < kDensifyPlanes; c++) {
+    if (j0 + c >= kParams) break;
+    const size_t plane = (size_t)(j0 + c) * out_stride;
+    if (ko) {
+      po[plane + oo] = tp[c];
+      mo[plane + oo] = tm[c];
+      vo[plane + oo] = tv[c];
+    }
+    if (kc) {
+      po[plane + oc] = tp[c];
+      mo[plane + oc] = 0.0f;
+      vo[plane + oc] = 0.0f;
+    }
+  }
+}
+
+// The kept children of split rows (src/densify.rs:160-184) and their lineage.
+__global__ void __launch_bounds__(256) densify_children_kernel(
+    const float* __restrict__ params, uint32_t n, uint32_t stride,
+    const uint32_t* __restrict__ split, const uint32_t* __restrict__ keep_child,
+    const uint32_t* __restrict__ pos_split, const uint32_t* __restrict__ pos_child,
+    uint32_t n_orig_kept, uint32_t n_clone_kept, DensifyParams dp,
+    const float* __restrict__ normals, uint64_t normals_seed, float* __restrict__ po,
+    float* __restrict__ mo, float* __restrict__ vo, uint32_t out_stride, DensifyLineage ln) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !split[i] || !keep_child[i]) return;
+  if (ln.birth_out) {  // children are born now, descended as clones are
+    const uint32_t new_origin =
+        ln.sample_of && ln.sample_of[i] != kNoSample ? ln.sample_of[i] : ln.origin_in[i];
     for (uint32_t c = 0; c < dp.children; c++) {
       const uint32_t o = n_orig_kept + n_clone_kept + pos_child[i] * dp.children + c;
       ln.birth_out[o] = ln.iteration;
       ln.origin_out[o] = new_origin;
     }
   }
-  if (keep_orig[i]) {
-    const uint32_t o = pos_orig[i];
-    for (int j = 0; j < kParams; j++) {
-      po[(size_t)j * out_stride + o] = params[(size_t)j * stride + i];
-      mo[(size_t)j * out_stride + o] = m[(size_t)j * stride + i];
-      vo[(size_t)j * out_stride + o] = v[(size_t)j * stride + i];
-    }
-  }
-  if (keep_clone[i]) {
-    const uint32_t o = n_orig_kept + pos_clone[i];
-    for (int j = 0; j < kParams; j++) {
-      po[(size_t)j * out_stride + o] = params[(size_t)j * stride + i];
-      mo[(size_t)j * out_stride + o] = 0.0f;
-      vo[(size_t)j * out_stride + o] = 0.0f;
-    }
-  }
-  if (split[i] && keep_child[i]) {
+  {
     float g[kParams];
     for (int j = 0; j < kParams; j++) g[j] = params[(size_t)j * stride + i];
     const float s[3] = {splat_expf(g[3]), splat_expf(g[4]), splat_expf(g[5])};
@@ -232,8 +265,10 @@ __global__ void __launch_bounds__(256) densify_scatter_kernel(
         {2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y)}};
     for (uint32_t c = 0; c < dp.children; c++) {
       const size_t draw = ((size_t)pos_split[i] * dp.children + c) * 3;
-      const float loc[3] = {normals[draw] * s[0], normals[draw + 1] * s[1],
-                            normals[draw + 2] * s[2]};
+      float nd[3];
+      for (int k = 0; k < 3; k++)
+        nd[k] = normals ? normals[draw + k] : normal_draw(normals_seed, draw + k);
+      const float loc[3] = {nd[0] * s[0], nd[1] * s[1], nd[2] * s[2]};
       const uint32_t o = n_orig_kept + n_clone_kept + pos_child[i] * dp.children + c;
       for (int j = 0; j < kParams; j++) {
         float val = g[j];
@@ -306,8 +341,8 @@ void densify_decide(splat_ctx* ctx, const float* params, size_t n, size_t stride
 size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const float* v,
                       size_t n, size_t stride, const float* grad_accum, const uint32_t* denom,
                       const uint32_t* max_radii, const splat_densify_config& cfg, float extent,
-                      const float* normals_dev, size_t normals_len, float* po, float* mo,
-                      float* vo, size_t out_capacity, size_t out_stride,
+                      const float* normals_dev, size_t normals_len, uint64_t normals_seed,
+                      float* po, float* mo, float* vo, size_t out_capacity, size_t out_stride,
                       splat_densify_report* report, const uint8_t* decide,
                       const DensifyLineage* lineage) {
   const DensifyParams dp = densify_params(cfg, extent);
@@ -339,13 +374,17 @@ size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const
   report->split = n_split;
   report->pruned = ((size_t)n - n_split - n_orig_kept) + ((size_t)n_clone - n_clone_kept) +
                    ((size_t)n_split - n_child_par) * children;
-  if (normals_len < 3ull * children * n_split) return (size_t)-1;
+  if (normals_dev && normals_len < 3ull * children * n_split) return (size_t)-1;
   if (n_out > out_capacity) return n_out;
-  launch(ctx, "densify_scatter", densify_scatter_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-         params, m, v, (uint32_t)n, (uint32_t)stride, (const uint32_t*)ko, (const uint32_t*)kc,
-         (const uint32_t*)sp, (const uint32_t*)kch, (const uint32_t*)po_o,
-         (const uint32_t*)po_c, (const uint32_t*)po_s, (const uint32_t*)po_ch, n_orig_kept,
-         n_clone_kept, dp, normals_dev, po, mo, vo, (uint32_t)out_stride, ln);
+  launch(ctx, "densify_copy", densify_copy_kernel, dim3(blocks_for(n, 256), kDensifyGroups),
+         dim3(256), 0, params, m, v, (uint32_t)n, (uint32_t)stride, (const uint32_t*)ko,
+         (const uint32_t*)kc, (const uint32_t*)po_o, (const uint32_t*)po_c, n_orig_kept, po, mo,
+         vo, (uint32_t)out_stride, ln);
+  if (n_split > 0)
+    launch(ctx, "densify_children", densify_children_kernel, dim3(blocks_for(n, 256)), dim3(256),
+           0, params, (uint32_t)n, (uint32_t)stride, (const uint32_t*)sp, (const uint32_t*)kch,
+           (const uint32_t*)po_s, (const uint32_t*)po_ch, n_orig_kept, n_clone_kept, dp,
+           normals_dev, normals_seed, po, mo, vo, (uint32_t)out_stride, ln);
   return n_out;
 }
 

@@ -93,26 +93,6 @@ ProjBuffers proj_buffers(splat_trainer* t) {
   return pb;
 }
 
-// Counter-based standard normals: SplitMix64 of (seed, index) gives two uniform doubles,
-// Box-Muller as the reference's normal_sample (src/densify.rs:116-121, u1 kept >= 1e-12).
-__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
-  x += 0x9e3779b97f4a7c15ull;
-  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-  return x ^ (x >> 31);
-}
-
-__global__ void normals_kernel(float* out, size_t count, uint64_t seed) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const uint64_t a = splitmix64(seed ^ splitmix64(2 * i));
-  const uint64_t b = splitmix64(seed ^ splitmix64(2 * i + 1));
-  double u1 = (double)(a >> 11) * 0x1.0p-53;
-  const double u2 = (double)(b >> 11) * 0x1.0p-53;
-  u1 = u1 > 1e-12 ? u1 : 1e-12;
-  out[i] = (float)(sqrt(-2.0 * log(u1)) * cos(2.0 * 3.141592653589793 * u2));
-}
-
 __global__ void __launch_bounds__(256) count_visible_kernel(const uint32_t* __restrict__ radius,
                                                             uint32_t n, uint32_t* count) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -154,13 +134,6 @@ __global__ void set_dyn_kernel(StepDyn d, StepDyn* __restrict__ out, uint32_t* _
 }
 
 }  // namespace
-
-namespace splat {
-void normals_fill(splat_ctx* ctx, float* out, size_t count, uint64_t seed) {
-  launch(ctx, "normals", normals_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, out, count,
-         seed);
-}
-}  // namespace splat
 
 namespace {
 
@@ -546,12 +519,13 @@ void trainer_densify(splat_trainer* t, const splat_densify_config& cfg, float ex
   *report = splat_densify_report{0, 0, 0};
   if (n == 0) return;
   const size_t children = cfg.split_children;
-  const size_t nnorm = 3 * children * n;
-  float* normals = (float*)t->store["normals"].reserve((nnorm + 1) * 4);
-  normals_fill(ctx, normals, nnorm, seed);
   const size_t cap2 = std::max(t->cap, n * (1 + std::max<size_t>(children, 1)));
   const size_t stride2 = plane_stride(cap2);
-  DevBuf po, mo, vo, bo, oo;
+  // Output planes: the previous call's input planes, kept for reuse (no allocation once
+  // they are large enough); swapped with the trainer's below.
+  DevBuf &po = t->store["densify_params"], &mo = t->store["densify_m"],
+         &vo = t->store["densify_v"], &bo = t->store["densify_birth"],
+         &oo = t->store["densify_origin"];
   po.reserve(stride2 * kParams * 4);
   mo.reserve(stride2 * kParams * 4);
   vo.reserve(stride2 * kParams * 4);
@@ -566,7 +540,7 @@ void trainer_densify(splat_trainer* t, const splat_densify_config& cfg, float ex
   }
   const size_t k = densify_planes(ctx, t->params.as<float>(), t->m.as<float>(), t->v.as<float>(),
                                   n, t->stride, t->grad_accum.as<float>(), t->denom.as<uint32_t>(),
-                                  t->max_radii.as<uint32_t>(), cfg, extent, normals, nnorm,
+                                  t->max_radii.as<uint32_t>(), cfg, extent, nullptr, 0, seed,
                                   po.as<float>(), mo.as<float>(), vo.as<float>(), cap2, stride2,
                                   report, decide, t->track ? &ln : nullptr);
   SPLAT_REQUIRE(k != (size_t)-1 && k <= cap2, "densify output exceeds its buffer");
