@@ -161,22 +161,22 @@ __global__ void __launch_bounds__(256) densify_flags_kernel(const float* __restr
 // Kept originals (parameters and both moments) and kept clones (parameters, zero moments)
 // to their output rows, kDensifyPlanes planes per workgroup row (blockIdx.y): each lane
 // has all of its loads for a row in flight before it stores, and a wave's accesses stay
-// within a few planes at a time.  Group 0 also writes the lineage of those rows.
+// within a few planes at a time.  Group 0 also writes the lineage of those rows.  A row's
+// keep flags are the steps of the exclusive scans of the flags (pos[i + 1] - pos[i]; the
+// scans hold n + 1 words), so each group reads two arrays per row instead of four.
 constexpr int kDensifyPlanes = 3;
 constexpr int kDensifyGroups = (kParams + kDensifyPlanes - 1) / kDensifyPlanes;
 
 __global__ void __launch_bounds__(256) densify_copy_kernel(
     const float* __restrict__ params, const float* __restrict__ m, const float* __restrict__ v,
-    uint32_t n, uint32_t stride, const uint32_t* __restrict__ keep_orig,
-    const uint32_t* __restrict__ keep_clone, const uint32_t* __restrict__ pos_orig,
+    uint32_t n, uint32_t stride, const uint32_t* __restrict__ pos_orig,
     const uint32_t* __restrict__ pos_clone, uint32_t n_orig_kept, float* __restrict__ po,
     float* __restrict__ mo, float* __restrict__ vo, uint32_t out_stride, DensifyLineage ln) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const bool ko = keep_orig[i] != 0, kc = keep_clone[i] != 0;
+  const uint32_t oo = pos_orig[i], oc = n_orig_kept + pos_clone[i];
+  const bool ko = pos_orig[i + 1] != oo, kc = n_orig_kept + pos_clone[i + 1] != oc;
   if (!ko && !kc) return;
-  const uint32_t oo = ko ? pos_orig[i] : 0u;
-  const uint32_t oc = kc ? n_orig_kept + pos_clone[i] : 0u;
   if (blockIdx.y == 0 && ln.birth_out) {
     // lineage: kept rows keep theirs; a clone is born now and descends from the logged
     // decision on its parent (or from the parent's own origin when it was not logged)
@@ -377,9 +377,8 @@ size_t densify_planes(splat_ctx* ctx, const float* params, const float* m, const
   if (normals_dev && normals_len < 3ull * children * n_split) return (size_t)-1;
   if (n_out > out_capacity) return n_out;
   launch(ctx, "densify_copy", densify_copy_kernel, dim3(blocks_for(n, 256), kDensifyGroups),
-         dim3(256), 0, params, m, v, (uint32_t)n, (uint32_t)stride, (const uint32_t*)ko,
-         (const uint32_t*)kc, (const uint32_t*)po_o, (const uint32_t*)po_c, n_orig_kept, po, mo,
-         vo, (uint32_t)out_stride, ln);
+         dim3(256), 0, params, m, v, (uint32_t)n, (uint32_t)stride, (const uint32_t*)po_o,
+         (const uint32_t*)po_c, n_orig_kept, po, mo, vo, (uint32_t)out_stride, ln);
   if (n_split > 0)
     launch(ctx, "densify_children", densify_children_kernel, dim3(blocks_for(n, 256)), dim3(256),
            0, params, (uint32_t)n, (uint32_t)stride, (const uint32_t*)sp, (const uint32_t*)kch,
