@@ -637,7 +637,10 @@ __global__ void __launch_bounds__(256) keygen_ordered_kernel(
 // The tile-sorted entries: tile ranges, (optional) keys and, when tvals is given, the
 // sorted lists (otherwise the sort wrote them in place).  Position j starts every tile in
 // (tile[j - 1], tile[j]]; the last position also ends the tiles after its own, so each
-// tile_start word is written exactly once when there are entries.
+// tile_start word is written exactly once when there are entries.  Each lane takes
+// kFinalizePerLane consecutive positions (one 16-byte load of their keys when all exist).
+constexpr uint32_t kFinalizePerLane = 4;
+
 __global__ void __launch_bounds__(256) radix_finalize_kernel(
     const uint32_t* __restrict__ tkeys, const uint32_t* __restrict__ tvals,
     const uint32_t* __restrict__ tgauss, uint32_t e_total, uint32_t num_tiles,
@@ -648,20 +651,39 @@ __global__ void __launch_bounds__(256) radix_finalize_kernel(
     if (status->abort) return;
     e_total = status->entries;
   }
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= e_total) return;
-  const uint32_t t = tkeys[j];
-  if (tvals) {
-    sorted_entry[j] = tvals[j];
-    sorted_gauss[j] = tgauss[j];
+  const uint32_t j0 = (blockIdx.x * blockDim.x + threadIdx.x) * kFinalizePerLane;
+  if (j0 >= e_total) return;
+  uint32_t tk[kFinalizePerLane];
+  if (j0 + kFinalizePerLane <= e_total) {  // j0 is a multiple of 4: an aligned 16-byte load
+    const uint4 q = *reinterpret_cast<const uint4*>(tkeys + j0);
+    tk[0] = q.x;
+    tk[1] = q.y;
+    tk[2] = q.z;
+    tk[3] = q.w;
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kFinalizePerLane; k++)
+      tk[k] = j0 + k < e_total ? tkeys[j0 + k] : 0u;
   }
-  if (keys_out) {
-    const uint32_t g = tvals ? tgauss[j] : sorted_gauss[j];
-    keys_out[j] = ((uint64_t)t << 32) | f2bits(depth[g]);
+  uint32_t u0 = j0 == 0 ? 0u : tkeys[j0 - 1] + 1u;  // first tile position j0 starts
+#pragma unroll
+  for (uint32_t k = 0; k < kFinalizePerLane; k++) {
+    const uint32_t j = j0 + k;
+    if (j >= e_total) break;
+    const uint32_t t = tk[k];
+    if (tvals) {
+      sorted_entry[j] = tvals[j];
+      sorted_gauss[j] = tgauss[j];
+    }
+    if (keys_out) {
+      const uint32_t g = tvals ? tgauss[j] : sorted_gauss[j];
+      keys_out[j] = ((uint64_t)t << 32) | f2bits(depth[g]);
+    }
+    for (uint32_t u = u0; u <= t; u++) tile_start[u] = j;
+    u0 = t + 1;
+    if (j + 1 == e_total)
+      for (uint32_t u = t + 1; u <= num_tiles; u++) tile_start[u] = e_total;
   }
-  for (uint32_t u = j == 0 ? 0u : tkeys[j - 1] + 1u; u <= t; u++) tile_start[u] = j;
-  if (j + 1 == e_total)
-    for (uint32_t u = t + 1; u <= num_tiles; u++) tile_start[u] = e_total;
 }
 
 // Per-counter offsets and tile totals, then tile_start = exclusive scan of the totals.
@@ -1057,7 +1079,8 @@ void bin_build_radix_device(splat_ctx* ctx, ProjBuffers pb, size_t n, const BinB
   // 5. lists and tile ranges
   fill_u32(ctx, bb.tile_start, T + 1, 0u);
   launch(ctx, "bin_finalize", radix_finalize_kernel,
-         dim3(blocks_for(status ? capacity : e_host, 256)), dim3(256), 0,
+         dim3(blocks_for(blocks_for(status ? capacity : e_host, kFinalizePerLane), 256)),
+         dim3(256), 0,
          (const uint32_t*)rs.ek[c2], direct ? nullptr : (const uint32_t*)rs.ev[c2],
          direct ? nullptr : (const uint32_t*)rs.eg[c2],
          e_host, (uint32_t)T, (const float*)pb.depth, bb.sorted_entry,
