@@ -158,8 +158,9 @@ __global__ void __launch_bounds__(256) densify_flags_kernel(const float* __restr
   clone[i] = is_clone ? 1u : 0u;
 }
 
-// Kept originals (parameters and both moments) and kept clones (parameters, zero moments)
-// to their output rows, kDensifyPlanes planes per workgroup row (blockIdx.y): each lane
+// Kept originals (parameters and both moments) and kept clones (parameters, zero moments:
+// src/densify.rs:148-158, the retains of :192-197 and :203-218) to their output rows,
+// kDensifyPlanes planes per workgroup row (blockIdx.y): each lane
 // has all of its loads for a row in flight before it stores, and a wave's accesses stay
 // within a few planes at a time.  Group 0 also writes the lineage of those rows.  A row's
 // keep flags are the steps of the exclusive scans of the flags (pos[i + 1] - pos[i]; the
@@ -220,7 +221,7 @@ __global__ void __launch_bounds__(256) densify_copy_kernel(
   }
 }
 
-// The kept children of split rows (src/densify.rs:160-184) and their lineage.
+// The kept children of split rows (src/densify.rs:160-186) and their lineage.
 __global__ void __launch_bounds__(256) densify_children_kernel(
     const float* __restrict__ params, uint32_t n, uint32_t stride,
     const uint32_t* __restrict__ split, const uint32_t* __restrict__ keep_child,
