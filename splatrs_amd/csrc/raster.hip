@@ -736,8 +736,10 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
 // span, staged through LDS in windows with coalesced loads, and each lane adds its slot of
 // its row's entries in order, so every sum keeps the reference's sequential order.  NS is
 // the record width the backward wrote: 9, or 11 with the absolute sums (slots 9 and 10,
-// into abs_grads).
-constexpr uint32_t kReduceWindow = 256;  // entries staged per window
+// into abs_grads).  A window of 128 entries holds the span of a wave's rows in the dense
+// batched workload (about 100 entries) and keeps the workgroup's LDS at 4.6 KB (9 slots),
+// so that LDS does not limit how many waves a CU holds (256 entries: 9.2 KB, about half).
+constexpr uint32_t kReduceWindow = 128;  // entries staged per window
 
 template <int NS>
 __global__ void __launch_bounds__(64) reduce_slots_kernel(const uint32_t* __restrict__ entry_offset,
