@@ -548,7 +548,8 @@ __global__ void __launch_bounds__(256) raster_bwd_kernel(
   // words per entry in `slots`: the absolute sums (slots 9, 10) only when asked for
   constexpr uint32_t SS = ABS ? kSlots : 9;
   // ORDERED: per-pixel terms of the batch; otherwise per-row partial sums of the SS slots
-  // (20 KB at B = 32 without the absolute sums: eight workgroups per CU).
+  // (18 KB at B = 32 without the absolute sums; the workgroup's LDS is then about 20.4 KB,
+  // seven workgroups per CU, as many as its 72 VGPRs allow).
   __shared__ float red[ORDERED ? B * kSlots * 256 : B * SS * 16];
   // whether the background term (T_final bg . dL) can be non-zero for a finite dL
   const bool bg_any = (bg0 != 0.0f) | (bg1 != 0.0f) | (bg2 != 0.0f);
