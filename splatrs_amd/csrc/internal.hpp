@@ -241,6 +241,16 @@ struct StepReport {
   unsigned long long total;
 };
 
+// Where a train step's loss pass writes its loss value and report (reports null: not
+// written): the pass's per-block partial sums, their count, the number of loss terms.
+struct StepLossReport {
+  const float2* partial;
+  uint32_t nb;
+  float n;
+  const StepStatus* status;
+  StepReport* reports;
+};
+
 // Tables of a batched step: many (scene, view) pairs composited in one launch per stage.
 // Pair v renders scene pair_scene[v] through cams[v]; its projections occupy
 // [work_base[v], work_base[v+1]) of the projection buffers (one per scene Gaussian), its
@@ -583,10 +593,14 @@ struct LossOut {
   float* partials;   // per-block (ssim_sum, l1_sum) pairs
   int num_partials;
 };
+// With `reports` (a graph-captured train step, `dyn` and `status` given): the loss value
+// (to dyn->loss_out when set) and the step's report (reports[dyn->report_slot]) are
+// written by the gradient pass's first workgroup.
 void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* target_f32,
                            const uint8_t* target_u8, uint32_t width, uint32_t height,
                            float lambda, bool want_l1, float* dimage, float* scratch,
-                           double* loss_parts_host, const StepDyn* dyn = nullptr);
+                           double* loss_parts_host, const StepDyn* dyn = nullptr,
+                           StepReport* reports = nullptr, const StepStatus* status = nullptr);
 size_t loss_scratch_floats(uint32_t width, uint32_t height);
 hipError_t loss_launch_bare(hipStream_t stream, const float* pred, const float* target,
                             uint32_t width, uint32_t height, float lambda, float* dimage,
@@ -605,11 +619,6 @@ void loss_forward_backward_batch(splat_ctx* ctx, const float* pred, const BatchD
 // (`out_dev`, or dyn->loss_out when `dyn` is given).
 void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_t height,
                    float lambda, float* out_dev, const StepDyn* dyn = nullptr);
-// A train step's last launch: its loss value (to dyn->loss_out, if set) and its report
-// (reports[dyn->report_slot]).
-void loss_value_and_report(splat_ctx* ctx, const float* scratch, uint32_t width,
-                           uint32_t height, float lambda, const StepDyn* dyn,
-                           const StepStatus* status, StepReport* reports);
 void blur_image(splat_ctx* ctx, const float* src, uint32_t width, uint32_t height,
                 uint32_t channels, float* dst);
 void gaussian_kernel_host(float k[11]);

@@ -243,6 +243,56 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
   if (threadIdx.x == 0) partial[lb.flat] = make_float2(bs, bl);
 }
 
+// Sum of the per-block partials in binary64 by one 256-lane workgroup (lane-strided sums,
+// then a fixed tree), so the result does not depend on timing.
+__device__ __forceinline__ void sum_partials(const float2* __restrict__ partial, uint32_t nb,
+                                             double* a_out, double* b_out) {
+  __shared__ double sa[256], sb[256];
+  double a = 0, b = 0;
+  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
+    a += partial[i].x;
+    b += partial[i].y;
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  *a_out = sa[0];
+  *b_out = sb[0];
+}
+
+// A graph-captured train step's loss value (to dyn->loss_out when set) and its
+// host-visible report, from the loss pass's partial sums: every thread of one workgroup
+// calls it.  The report's status words are final by then (only the binning's capacity
+// guard writes them), and the host reads a report only after the step's completion event.
+__device__ __forceinline__ void write_step_report(const StepLossReport& rep, float lambda,
+                                                  const StepDyn* __restrict__ dyn) {
+  float* out = dyn->loss_out;
+  if (out) {
+    double a, b;
+    sum_partials(rep.partial, rep.nb, &a, &b);
+    if (threadIdx.x == 0) {
+      const float inv_n = 1.0f / rep.n;
+      const float l1 = (float)b * inv_n;
+      const float ssim = (float)a / rep.n;
+      *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
+    }
+  }
+  if (threadIdx.x == 0) {
+    StepReport r;
+    r.seq = dyn->seq;
+    r.abort = rep.status->abort;
+    r.total = rep.status->total;
+    rep.reports[dyn->report_slot] = r;
+  }
+}
+
 // Pass 2.  MODE 0: photometric-loss gradient; MODE 1: d(mean SSIM)/dx only.
 template <bool U8, int MODE, bool BATCH>
 __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict__ xin,
@@ -254,8 +304,10 @@ __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict_
                                                         float inv_n, float w1, float lambda,
                                                         float* __restrict__ grad,
                                                         const StepDyn* __restrict__ dyn,
-                                                        BatchDev bd) {
+                                                        BatchDev bd, StepLossReport rep) {
   if (dyn) yin = dyn->target;
+  // the partial sums of pass 1 are complete: workgroup 0 also writes the step's report
+  if (rep.reports && blockIdx.x == 0 && blockIdx.y == 0) write_step_report(rep, lambda, dyn);
   size_t pix0;
   uint32_t pair;
   const LossBlock lb = loss_block<BATCH>(bd, &W, &H, &pix0, &pair);
@@ -381,30 +433,6 @@ __global__ void blur_v_kernel(const float* __restrict__ tmp, uint32_t W, uint32_
 
 // loss = (1 - lambda) * (sum |x - y| * inv_n) + lambda * (1 - sum SSIM / n), from the
 // per-block partial sums (summed in binary64), written to *out without a host round trip.
-// Sum of the per-block partials in binary64 by one 256-lane workgroup (lane-strided sums,
-// then a fixed tree), so the result does not depend on timing.
-__device__ __forceinline__ void sum_partials(const float2* __restrict__ partial, uint32_t nb,
-                                             double* a_out, double* b_out) {
-  __shared__ double sa[256], sb[256];
-  double a = 0, b = 0;
-  for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) {
-    a += partial[i].x;
-    b += partial[i].y;
-  }
-  sa[threadIdx.x] = a;
-  sb[threadIdx.x] = b;
-  __syncthreads();
-  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      sa[threadIdx.x] += sa[threadIdx.x + s];
-      sb[threadIdx.x] += sb[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  *a_out = sa[0];
-  *b_out = sb[0];
-}
-
 __global__ void __launch_bounds__(256) loss_value_kernel(const float2* __restrict__ partial,
                                                          uint32_t nb, float lambda, float n,
                                                          float* __restrict__ out,
@@ -418,32 +446,6 @@ __global__ void __launch_bounds__(256) loss_value_kernel(const float2* __restric
   const float l1 = (float)b * inv_n;
   const float ssim = (float)a / n;
   *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
-}
-
-// The last kernel of a train step: the loss value (when dyn->loss_out is set; the partial
-// sums are still those of the step's loss pass) and the step's host-visible report.
-__global__ void __launch_bounds__(256) loss_value_report_kernel(
-    const float2* __restrict__ partial, uint32_t nb, float lambda, float n,
-    const StepDyn* __restrict__ dyn, const StepStatus* __restrict__ status,
-    StepReport* __restrict__ reports) {
-  float* out = dyn->loss_out;
-  if (out) {
-    double a, b;
-    sum_partials(partial, nb, &a, &b);
-    if (threadIdx.x == 0) {
-      const float inv_n = 1.0f / n;
-      const float l1 = (float)b * inv_n;
-      const float ssim = (float)a / n;
-      *out = (1.0f - lambda) * l1 + lambda * (1.0f - ssim);
-    }
-  }
-  if (threadIdx.x == 0) {
-    StepReport r;
-    r.seq = dyn->seq;
-    r.abort = status->abort;
-    r.total = status->total;
-    reports[dyn->report_slot] = r;
-  }
 }
 
 // Per-pair loss values of a batched step: one workgroup per pair.
@@ -483,7 +485,8 @@ void loss_forward_backward_batch(splat_ctx* ctx, const float* pred, const BatchD
          (const StepDyn*)nullptr, bd);
   launch(ctx, "loss_grad", ssim_grad_kernel<true, 0, true>, dim3(total_blocks), dim3(256), 0,
          pred, (const void*)nullptr, 0u, 0u, taps, (const float*)d_mu, (const float*)d_exx,
-         (const float*)d_exy, 0.0f, 0.0f, lambda, dimage, (const StepDyn*)nullptr, bd);
+         (const float*)d_exy, 0.0f, 0.0f, lambda, dimage, (const StepDyn*)nullptr, bd,
+         StepLossReport{});
   if (losses)
     launch(ctx, "loss_value", loss_values_batch_kernel, dim3(bd.npairs), dim3(256), 0,
            (const float2*)partial, bd, lambda, losses);
@@ -495,16 +498,6 @@ void loss_value_to(splat_ctx* ctx, const float* scratch, uint32_t width, uint32_
   const unsigned nb = blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
   launch(ctx, "loss_value", loss_value_kernel, dim3(1), dim3(256), 0,
          reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, out_dev, dyn);
-}
-
-void loss_value_and_report(splat_ctx* ctx, const float* scratch, uint32_t width,
-                           uint32_t height, float lambda, const StepDyn* dyn,
-                           const StepStatus* status, StepReport* reports) {
-  const size_t n = (size_t)width * height * 3;
-  const unsigned nb = blocks_for((size_t)width * 3, kStrip) * blocks_for(height, kRowsPerBlock);
-  launch(ctx, "step_report", loss_value_report_kernel, dim3(1), dim3(256), 0,
-         reinterpret_cast<const float2*>(scratch + 3 * n), nb, lambda, (float)n, dyn, status,
-         reports);
 }
 
 // Both passes and the loss value on caller-provided device buffers, with `scratch` of
@@ -530,7 +523,7 @@ hipError_t loss_launch_bare(hipStream_t stream, const float* pred, const float* 
   hipLaunchKernelGGL((ssim_grad_kernel<false, 0, false>), grid, dim3(256), 0, stream, pred,
                      (const void*)target, width, height, taps, (const float*)d_mu,
                      (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage,
-                     (const StepDyn*)nullptr, BatchDev{});
+                     (const StepDyn*)nullptr, BatchDev{}, StepLossReport{});
   if (loss_out)
     hipLaunchKernelGGL(loss_value_kernel, dim3(1), dim3(256), 0, stream,
                        (const float2*)partial, grid.x * grid.y, lambda, (float)n, loss_out,
@@ -562,7 +555,8 @@ size_t loss_scratch_floats(uint32_t width, uint32_t height) {
 void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* target_f32,
                            const uint8_t* target_u8, uint32_t width, uint32_t height,
                            float lambda, bool want_l1, float* dimage, float* scratch,
-                           double* loss_parts_host, const StepDyn* dyn) {
+                           double* loss_parts_host, const StepDyn* dyn, StepReport* reports,
+                           const StepStatus* status) {
   const size_t n = (size_t)width * height * 3;
   Taps taps;
   gaussian_kernel_host(taps.k);
@@ -574,6 +568,14 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
   const dim3 grid(blocks_for((size_t)width * 3, kStrip), blocks_for(height, kRowsPerBlock));
   const float inv_n = 1.0f / (float)n;
   const float w1 = (1.0f - lambda) * inv_n;
+  StepLossReport rep{};
+  if (reports) {
+    rep.partial = partial;
+    rep.nb = grid.x * grid.y;
+    rep.n = (float)n;
+    rep.status = status;
+    rep.reports = reports;
+  }
   if (target_u8) {
     launch(ctx, "loss_ssim_stats", ssim_stats_kernel<true, false>, grid, dim3(256), 0, pred,
            (const void*)target_u8, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn, BatchDev{});
@@ -581,12 +583,12 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
       launch(ctx, "loss_grad", ssim_grad_kernel<true, 0, false>, grid, dim3(256), 0, pred,
              (const void*)target_u8, width, height, taps, (const float*)d_mu,
              (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
-             BatchDev{});
+             BatchDev{}, rep);
     else
       launch(ctx, "loss_grad", ssim_grad_kernel<true, 1, false>, grid, dim3(256), 0, pred,
              (const void*)target_u8, width, height, taps, (const float*)d_mu,
              (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
-             BatchDev{});
+             BatchDev{}, rep);
   } else {
     launch(ctx, "loss_ssim_stats", ssim_stats_kernel<false, false>, grid, dim3(256), 0, pred,
            (const void*)target_f32, width, height, taps, c1, c2, d_mu, d_exx, d_exy, partial, dyn, BatchDev{});
@@ -594,12 +596,12 @@ void loss_forward_backward(splat_ctx* ctx, const float* pred, const float* targe
       launch(ctx, "loss_grad", ssim_grad_kernel<false, 0, false>, grid, dim3(256), 0, pred,
              (const void*)target_f32, width, height, taps, (const float*)d_mu,
              (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
-             BatchDev{});
+             BatchDev{}, rep);
     else
       launch(ctx, "loss_grad", ssim_grad_kernel<false, 1, false>, grid, dim3(256), 0, pred,
              (const void*)target_f32, width, height, taps, (const float*)d_mu,
              (const float*)d_exx, (const float*)d_exy, inv_n, w1, lambda, dimage, dyn,
-             BatchDev{});
+             BatchDev{}, rep);
   }
   if (loss_parts_host) {
     const size_t nb = (size_t)grid.x * grid.y;

@@ -255,8 +255,9 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   bv.rows = P.bwd_rows != 0;
   raster_forward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.image, P.final_t, P.n_contrib, fv,
                  nullptr, P.counter);
+  // the loss pass also writes the loss value and the step's report
   loss_forward_backward(ctx, P.image, nullptr, any_target, P.width, P.height, P.lambda, true,
-                        P.dimage, P.loss_scratch, nullptr, P.dyn);
+                        P.dimage, P.loss_scratch, nullptr, P.dyn, P.reports, P.status);
   raster_backward(ctx, P.pb, P.bb, P.width, P.height, P.bg, P.final_t, P.n_contrib, P.dimage,
                   P.ordered != 0, P.slots, P.accumulate && P.absgrad, bv, nullptr, P.counter);
   StatsArgs sa{};
@@ -277,9 +278,6 @@ void enqueue_step(splat_trainer* t, const StepPlan& P, const uint8_t* any_target
   std::memset(&lr0, 0, sizeof(lr0));
   grad_adam(ctx, P.params, P.m, P.v, n, P.stride, cam0, opts, P.bb.entry_offset, P.slots, sa,
             1.0f, 1.0f, ast, lr0, P.dyn, P.status);
-  // the loss value from the loss pass's partial sums (untouched since), and the report
-  loss_value_and_report(ctx, P.loss_scratch, P.width, P.height, P.lambda, P.dyn, P.status,
-                        P.reports);
 }
 
 hipGraphExec_t graph_for(splat_trainer* t, const StepPlan& P, const uint8_t* any_target) {
