@@ -26,9 +26,21 @@ struct Taps {
   float k[11];
 };
 
+// View::to_real of an RGB8 target byte (src/dataset.rs:23-25): v / 255.0, the correctly
+// rounded binary32 quotient.  A byte has 256 values, so each 256-lane workgroup divides
+// once per value into an LDS table (every thread calls fill_u8_table, which synchronises)
+// and the loads look the quotients up: the same bits for a fraction of the instructions.
 template <bool U8>
-__device__ __forceinline__ float load_target(const void* t, size_t idx) {
-  if (U8) return (float)static_cast<const uint8_t*>(t)[idx] / 255.0f;  // View::to_real
+__device__ __forceinline__ void fill_u8_table(float* lut) {
+  if (U8) {
+    lut[threadIdx.x] = (float)threadIdx.x / 255.0f;
+    __syncthreads();
+  }
+}
+
+template <bool U8>
+__device__ __forceinline__ float load_target(const void* t, size_t idx, const float* lut) {
+  if (U8) return lut[static_cast<const uint8_t*>(t)[idx]];
   return static_cast<const float*>(t)[idx];
 }
 
@@ -135,6 +147,8 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
   }
   __shared__ float sx[kStrip + 2 * kHalo], sy[kStrip + 2 * kHalo];
   __shared__ float red[4];
+  __shared__ float s_u8[U8 ? 256 : 1];
+  fill_u8_table<U8>(s_u8);
   const int width3 = (int)W * 3;
   const int q0 = lb.bx * kStrip;
   const int q = q0 + threadIdx.x;
@@ -159,13 +173,13 @@ __global__ void __launch_bounds__(256) ssim_stats_kernel(const float* __restrict
       const bool in = r >= 0 && r < (int)H && t < kStrip + 2 * kHalo && qq >= 0 && qq < width3;
       const size_t idx = (size_t)(in ? r : 0) * width3 + (in ? qq : 0);
       fx[k] = in ? xin[idx] : 0.0f;
-      fy[k] = in ? load_target<U8>(yin, idx) : 0.0f;
+      fy[k] = in ? load_target<U8>(yin, idx, s_u8) : 0.0f;
     }
     const int orow = r - 5;
     if (orow >= ybeg && orow < yend && col_ok) {
       const size_t idx = (size_t)orow * width3 + q;
       ox = xin[idx];
-      oy = load_target<U8>(yin, idx);
+      oy = load_target<U8>(yin, idx, s_u8);
     }
   };
   fetch(ybeg - 5);
@@ -323,6 +337,8 @@ __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict_
     w1 = (1.0f - lambda) * inv_n;
   }
   __shared__ float s0[kStrip + 2 * kHalo], s1[kStrip + 2 * kHalo], s2[kStrip + 2 * kHalo];
+  __shared__ float s_u8[U8 ? 256 : 1];
+  fill_u8_table<U8>(s_u8);
   const int width3 = (int)W * 3;
   const int q0 = lb.bx * kStrip;
   const int q = q0 + threadIdx.x;
@@ -352,7 +368,7 @@ __global__ void __launch_bounds__(256) ssim_grad_kernel(const float* __restrict_
     if (orow >= ybeg && orow < yend && col_ok) {
       const size_t idx = (size_t)orow * width3 + q;
       ox = xin[idx];
-      oy = load_target<U8>(yin, idx);
+      oy = load_target<U8>(yin, idx, s_u8);
     }
   };
   fetch(ybeg - 5);
