@@ -340,7 +340,7 @@ __device__ __forceinline__ float plane_lr(int j, const splat_learning_rates& lr)
 }
 
 // Streaming 16-byte loads and stores with the non-temporal hint: data touched once per
-// pass (the Adam planes) should not displace reusable lines from the caches.
+// pass (Adam's moment planes) should not displace reusable lines from the caches.
 typedef float float4_vec __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 load_stream(const float* p) {
   const float4_vec t = __builtin_nontemporal_load(reinterpret_cast<const float4_vec*>(p));

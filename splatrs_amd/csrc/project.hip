@@ -692,7 +692,10 @@ __device__ __forceinline__ void adam_groups(float* __restrict__ params, float* _
       const uint32_t q = q0 + u * R;
       const uint32_t j = q / kPerPlane, r = (q % kPerPlane) * 4u;
       const size_t at = q < kGroups && r + 4 <= rows ? (size_t)j * stride + ib + r : (size_t)ib;
-      P[u] = load_stream(params + at);
+      // the parameter planes are read and written with ordinary (cached) accesses: the
+      // projection backward above has just read them, and the next step's projection
+      // reads what is stored here; the moments stream with the non-temporal hint
+      P[u] = *reinterpret_cast<const float4*>(params + at);
       M[u] = load_stream(m + at);
       V[u] = load_stream(v + at);
     }
@@ -709,7 +712,7 @@ __device__ __forceinline__ void adam_groups(float* __restrict__ params, float* _
         adam_update(P[u].y, G.y, M[u].y, V[u].y, rate, b1, b2, c1, c2, eps);
         adam_update(P[u].z, G.z, M[u].z, V[u].z, rate, b1, b2, c1, c2, eps);
         adam_update(P[u].w, G.w, M[u].w, V[u].w, rate, b1, b2, c1, c2, eps);
-        store_stream(params + at, P[u]);
+        *reinterpret_cast<float4*>(params + at) = P[u];
         store_stream(m + at, M[u]);
         store_stream(v + at, V[u]);
       } else {
