@@ -469,8 +469,6 @@ size_t scan_temp_bytes(size_t n);
 size_t scan_max_items();  // largest n exclusive_scan_u32 takes
 // Fill kernel (used instead of hipMemset so the same code can sit inside a HIP graph).
 void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v);
-void fill3_u32(splat_ctx* ctx, uint32_t* p0, size_t n0, uint32_t* p1, size_t n1, uint32_t* p2,
-               size_t n2, uint32_t v);
 
 // binning.hip
 // Builds the sorted per-tile lists.  Returns E (synchronises once to read it).

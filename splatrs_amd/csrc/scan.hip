@@ -157,26 +157,8 @@ __global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
 
 }  // namespace
 
-// Three ranges set to v in one launch: thread i covers word i of their concatenation.
-__global__ void fill3_u32_kernel(uint32_t* p0, size_t n0, uint32_t* p1, size_t n1, uint32_t* p2,
-                                 size_t n2, uint32_t v) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n0)
-    p0[i] = v;
-  else if (i < n0 + n1)
-    p1[i - n0] = v;
-  else if (i < n0 + n1 + n2)
-    p2[i - n0 - n1] = v;
-}
-
 void fill_u32(splat_ctx* ctx, uint32_t* p, size_t n, uint32_t v) {
   launch(ctx, "fill", fill_u32_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, p, n, v);
-}
-
-void fill3_u32(splat_ctx* ctx, uint32_t* p0, size_t n0, uint32_t* p1, size_t n1, uint32_t* p2,
-               size_t n2, uint32_t v) {
-  launch(ctx, "fill", fill3_u32_kernel, dim3(blocks_for(n0 + n1 + n2, 256)), dim3(256), 0, p0,
-         n0, p1, n1, p2, n2, v);
 }
 
 size_t scan_max_items() { return (size_t)kTopThreads * kTopItems * kScanTile; }
