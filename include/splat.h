@@ -483,7 +483,10 @@ int splat_trainer_reserve_entries(splat_trainer* t, uint64_t entries);
 int splat_trainer_graph_stats(splat_trainer* t, uint64_t* captures, uint64_t* replays,
                               uint64_t* aborts);
 /* densify_and_prune on the resident model (src/densify.rs:125-221); the normal draws
- * for split children come from a counter-based generator keyed by `seed`. */
+ * for split children come from a counter-based generator keyed by `seed`.  The new
+ * parameter and moment planes are written into a second set, which is swapped in; the
+ * old set is kept for the next call, so the trainer holds two sets (3 x 59 binary32 planes
+ * each) at its grown capacity. */
 int splat_trainer_densify(splat_trainer* t, const splat_densify_config* cfg, float extent,
                           uint64_t seed, splat_densify_report* report);
 /* reset_opacity (src/densify.rs:224-230). */
