@@ -597,7 +597,7 @@ __global__ void __launch_bounds__(256) ordered_rows_kernel(
   if (c) rows[i] = recs[g];
 }
 
-constexpr uint32_t kOrderedPerLane = 4;
+constexpr uint32_t kOrderedPerLane = 8;
 constexpr uint32_t kOrderedEntries = kKeygenThreads * kOrderedPerLane;  // per workgroup
 constexpr uint32_t kOrderedSpan = 2048;  // rows whose offsets are staged per workgroup
 
