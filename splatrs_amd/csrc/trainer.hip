@@ -93,12 +93,21 @@ ProjBuffers proj_buffers(splat_trainer* t) {
   return pb;
 }
 
+// Rows with a positive radius, added to *count: the grid strides over the rows and each
+// workgroup adds its total once (a few hundred atomics on the counter, not one per wave).
+constexpr uint32_t kCountBlocks = 256;
 __global__ void __launch_bounds__(256) count_visible_kernel(const uint32_t* __restrict__ radius,
                                                             uint32_t n, uint32_t* count) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool vis = i < n && radius[i] > 0;
-  const uint64_t mask = __ballot(vis);
-  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(count, (uint32_t)__popcll(mask));
+  __shared__ uint32_t s_cnt[4];
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    c += (uint32_t)__popcll(__ballot(radius[i] > 0));  // the wave's count, in every lane
+  if ((threadIdx.x & 63u) == 0) s_cnt[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t total = (s_cnt[0] + s_cnt[1]) + (s_cnt[2] + s_cnt[3]);
+    if (total) atomicAdd(count, total);
+  }
 }
 
 // Per-block sums of squared errors (binary64) between a clamped image and RGB8 targets.
@@ -712,8 +721,9 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
         const float* lp = t->loss_target ? t->loss_target : (const float*)t->store["step_loss"].ptr;
         uint32_t* cnt = (uint32_t*)t->store["visible_count"].reserve(4);
         fill_u32(ctx, cnt, 1, 0u);
-        launch(ctx, "count_visible", count_visible_kernel, dim3(blocks_for(t->n, 256)), dim3(256),
-               0, (const uint32_t*)t->store["radius"].ptr, (uint32_t)t->n, cnt);
+        launch(ctx, "count_visible", count_visible_kernel,
+               dim3(std::min(blocks_for(t->n, 256), kCountBlocks)), dim3(256), 0,
+               (const uint32_t*)t->store["radius"].ptr, (uint32_t)t->n, cnt);
         uint32_t vis = 0;
         float loss = 0.0f;
         SPLAT_HIP(hipMemcpyAsync(&vis, cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -796,7 +806,8 @@ int splat_trainer_step(splat_trainer* t, uint32_t view, const splat_step_params*
       if (p->want_loss) {
         uint32_t* cnt = (uint32_t*)t->store["visible_count"].reserve(4);
         fill_u32(ctx, cnt, 1, 0u);
-        launch(ctx, "count_visible", count_visible_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+        launch(ctx, "count_visible", count_visible_kernel,
+               dim3(std::min(blocks_for(n, 256), kCountBlocks)), dim3(256), 0,
                (const uint32_t*)pb.radius, (uint32_t)n, cnt);
         uint32_t vis = 0;
         SPLAT_HIP(hipMemcpyAsync(&vis, cnt, 4, hipMemcpyDeviceToHost, ctx->stream));
