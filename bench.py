@@ -415,7 +415,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+              "(the training path runs on one GPU; a sharded version is not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
