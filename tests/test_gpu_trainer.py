@@ -310,3 +310,25 @@ def test_tuned_and_persistent_shapes_give_the_same_model(ctx):
         for o in out[1:]:
             assert np.array_equal(np.asarray(out[0][k]).view(np.uint8),
                                   np.asarray(o[k]).view(np.uint8)), k
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_step_reports_visible_count(ctx, graph):
+    """A step's num_visible (asked for with want_loss) is the number of rows its
+    projection keeps (radius > 0), here over more rows than one pass of the counting grid
+    covers (256 workgroups of 256 lanes) and with a partial last wave."""
+    n = 200_003
+    g = scenes.large_scene(0, n)  # about half of the rows lie outside the view
+    cam = scenes.hd_camera()
+    rgb = np.zeros((int(cam["height"]), int(cam["width"]), 3), np.uint8)
+    tr = abi.Trainer(ctx, g)
+    tr.add_view(cam, rgb)
+    tr.set_graph(graph)
+    lr = rec.learning_rates(rec.position_lr(1, scenes.scene_extent([cam])))
+    say(f"visible count: {n} Gaussians, graph={graph}")
+    r = tr.step(0, rec.step_params(3, lr, want_loss=True))
+    tr.close()
+    want = int(np.count_nonzero(O.project_all(g, cam, rec.proj_opts(3))["radius"] > 0))
+    say(f"device {int(r['num_visible'])} oracle {want}")
+    assert 0 < want < n
+    assert int(r["num_visible"]) == want
